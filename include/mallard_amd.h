@@ -1,0 +1,203 @@
+/*
+ * mallard_amd: C ABI of the MI355X-native explicit residual engine for
+ * Mallard's compressible Euler path on unstructured 2D meshes.
+ *
+ * Plain C types only. Every call returns 0 on success and a nonzero status
+ * otherwise; exceptions and HIP errors never cross this boundary, the text of
+ * the last failure is returned by mlr_last_error(ctx) (or mlr_last_error(NULL)
+ * for failures that have no context). A context owns all device memory and one
+ * HIP stream; calls on one context must come from one thread at a time.
+ *
+ * Arrays passed in or out are host arrays in the reference numbering of
+ * cells and faces (mesh.cpp generators); the library renumbers internally.
+ * Conserved state is AoS double[N_cells][4] = (rho, rho u, rho v, rho E);
+ * primitives are double[N_cells][5] = (u, v, p, T, h), as Kokkos::View<rtype
+ * *[N]> LayoutRight in the reference.
+ */
+#ifndef MALLARD_AMD_H
+#define MALLARD_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mlr_ctx mlr_ctx;
+
+/* Mesh type codes (mesh.h:23-35). */
+enum { MLR_MESH_FILE = 0, MLR_MESH_CARTESIAN = 1, MLR_MESH_CARTESIAN_TRI = 2, MLR_MESH_WEDGE = 3 };
+/* Face reconstruction (face_reconstruction.h:30-33). */
+enum { MLR_RECON_FO = 0, MLR_RECON_TENO = 1 };
+/* Basis (basis.h:24-32). */
+enum { MLR_BASIS_MONOMIAL = 0, MLR_BASIS_LEGENDRE = 1 };
+/* TENO weighting: 0 = as the reference computes it
+ * (face_reconstruction.cpp:948-1031); 1 = corrected variant (ENO selection
+ * among CT2-passing directional stencils, zero-mean face evaluation,
+ * positivity scaling, vacuum-guarded two-rarefaction estimate). */
+enum { MLR_TENO_REFERENCE = 0, MLR_TENO_CORRECTED = 1 };
+/* Riemann solvers (riemann_solver.h:21-31; Roe has no reference counterpart). */
+enum { MLR_RS_RUSANOV = 0, MLR_RS_HLL = 1, MLR_RS_HLLC = 2, MLR_RS_ROE = 3 };
+/* Time integrators (time_integrator.h:29-33). */
+enum { MLR_TI_FE = 0, MLR_TI_RK4 = 1, MLR_TI_SSPRK3 = 2 };
+/* Boundary conditions (boundary.h:22-37). */
+enum { MLR_BC_SYMMETRY = 0, MLR_BC_EXTRAPOLATION = 1, MLR_BC_WALL_ADIABATIC = 2, MLR_BC_UPT = 3,
+       MLR_BC_P_OUT = 4 };
+
+/* One [[boundaries]] entry (solver.cpp:188-237): zone name, type and the
+ * parameters UPT (u, p, T) and P_OUT (p) read. */
+typedef struct {
+    char zone[32];
+    int32_t type;
+    double u[2];
+    double p, T;
+} mlr_bc_desc;
+
+/* [numerics], [physics] and [run] settings (solver.cpp:110-273,
+ * physics.cpp:27-53, face_reconstruction.cpp:109-152). quad orders <= 0 take
+ * the reference defaults (Dunavant p+1, Gauss-Legendre (p+1)/2). cfl > 0
+ * selects CFL time steps, otherwise the fixed dt is used. */
+typedef struct {
+    int32_t recon, basis, order;
+    double stencil_factor;
+    int32_t quad_order_cell, quad_order_face, teno_variant;
+    int32_t riemann, integrator;
+    double gamma, p_ref, T_ref, rho_ref, p_min, p_max;
+    double cfl, dt;
+    int32_t n_bc;
+    mlr_bc_desc bc[14];
+    int32_t n_threads; /* host precompute threads, 0 = default */
+} mlr_numerics_desc;
+
+/* Structured generators of the reference (mesh.cpp:305-848). jitter moves
+ * interior nodes by jitter*(dx,dy)*(2u-1), u from a splitmix64 stream keyed by
+ * (seed, node id); jitter = 0 is the reference generator. */
+typedef struct {
+    int32_t type;
+    uint32_t nx, ny;
+    double Lx, Ly, jitter;
+    uint64_t seed;
+} mlr_mesh_gen;
+
+/* Unstructured mesh in reference numbering (the arrays of mesh.h:228-253):
+ * 2 nodes per face, faces_of_cell CSR sharing offsets_nodes_of_cell, zone 0
+ * named "interior". */
+typedef struct {
+    uint32_t n_nodes;
+    const double* node_coords;
+    uint32_t n_cells;
+    const uint32_t* offsets_nodes_of_cell;
+    const uint32_t* nodes_of_cell;
+    const uint32_t* faces_of_cell;
+    uint32_t n_faces;
+    const uint32_t* nodes_of_face;
+    const int32_t* cells_of_face;
+    uint32_t n_zones;
+    const char* const* zone_names;
+    const uint32_t* zone_offsets;
+    const uint32_t* zone_faces;
+} mlr_mesh_desc;
+
+/* Solver::init(input_file) (solver.cpp:39-80): reads a Mallard TOML input
+ * (mesh, physics, numerics, boundaries, run, initialize) and builds the
+ * device state. */
+int mlr_ctx_create_toml(const char* input_toml, mlr_ctx** out);
+/* Same with a generated mesh and explicit settings (Mesh::init, mesh.cpp:32-66). */
+int mlr_ctx_create_generated(const mlr_mesh_gen* gen, const mlr_numerics_desc* num, mlr_ctx** out);
+/* Same with a caller mesh (the host arrays copy_host_to_device uploads, mesh.cpp:263-282). */
+int mlr_ctx_create(const mlr_mesh_desc* mesh, const mlr_numerics_desc* num, mlr_ctx** out);
+int mlr_ctx_destroy(mlr_ctx* ctx);
+const char* mlr_last_error(const mlr_ctx* ctx);
+
+/* sizes[0..11] = n_cells, n_faces (allocated), n_faces_real, face slots per
+ * cell, face quadrature points, n_dof, cells per stencil, stencils per cell,
+ * device bytes of the layout, step, recon, teno_variant */
+int mlr_get_sizes(const mlr_ctx* ctx, int64_t* sizes);
+
+/* Initial states (solver_initialize.cpp:35-186). states[16] = (rho,u,v,p)
+ * for the upper-right, upper-left, lower-left and lower-right quadrants of
+ * the split point (xs, ys), chosen by the cell centroid. */
+int mlr_init_quadrant(mlr_ctx* ctx, double xs, double ys, const double* states);
+int mlr_init_constant(mlr_ctx* ctx, const double* u, double p, double T);
+/* Host <-> device state copies (Solver::copy_host_to_device / _to_host,
+ * solver.cpp:322-334). prim may be NULL on input (computed from U). */
+int mlr_set_state(mlr_ctx* ctx, const double* U, const double* prim);
+int mlr_get_state(mlr_ctx* ctx, double* U, double* prim);
+
+/* Solver::calc_rhs (solver_rhs.cpp:44-55): residual R(U) of the current
+ * state, double[N][4]. */
+int mlr_calc_rhs(mlr_ctx* ctx, double* rhs);
+/* Solver::calc_dt (solver.cpp:580-590) with the configured cfl or dt. */
+int mlr_compute_dt(mlr_ctx* ctx, double* dt_out);
+/* Solver::take_step (solver.cpp:521-531) with the dt of the last
+ * mlr_compute_dt. */
+int mlr_step(mlr_ctx* ctx);
+/* n iterations of the run loop body (calc_dt + take_step, solver.cpp:358-364),
+ * captured once in a hipGraph and replayed; dt values written to dts if not
+ * NULL. */
+int mlr_run_steps(mlr_ctx* ctx, int64_t n, double* dts);
+/* Solver::run (solver.cpp:352-373) with the configured stop conditions. */
+int mlr_run(mlr_ctx* ctx);
+int mlr_set_cfl(mlr_ctx* ctx, double cfl);
+int mlr_get_time(const mlr_ctx* ctx, double* t, int64_t* step);
+/* Face states of the last residual, face_solution[n_faces][Q][2][4] as in
+ * FaceReconstruction::calc_face_values (face_reconstruction.cpp:95-99,
+ * :1081-1106); entries no cell writes are NaN. */
+int mlr_get_face_values(mlr_ctx* ctx, double* out);
+/* TENO cut-off outcome of the last residual per cell: bit v set when the
+ * central stencil was selected for variable v (face_reconstruction.cpp:958). */
+int mlr_get_decisions(mlr_ctx* ctx, uint8_t* out);
+/* cfl_local after the last dt (solver.cpp:584, :705). */
+int mlr_get_cfl_local(mlr_ctx* ctx, double* out);
+
+/* Index parity hooks: new -> reference cell permutation, and one cell's
+ * stencils / pseudo-inverses recomputed in reference numbering
+ * (face_reconstruction.cpp:406-433, :477-741). ids[G*M], sizes[G],
+ * aplus[G*K*M] (row-major K x M), ta[G*M], ipt[K], oi[K*K]. */
+int mlr_get_permutation(const mlr_ctx* ctx, uint32_t* perm);
+int mlr_get_stencils(const mlr_ctx* ctx, uint32_t cell, uint32_t* ids, uint32_t* sizes);
+int mlr_get_teno_matrices(const mlr_ctx* ctx, uint32_t cell, double* aplus, double* ta, double* ipt,
+                          double* oi);
+/* Device-side stencil ids of one cell mapped back to reference numbering
+ * (the uploaded layout, target cell first), ids[G*M]. */
+int mlr_get_device_stencils(const mlr_ctx* ctx, uint32_t cell, uint32_t* ids);
+
+/* Host-only precompute (no GPU needed): the mesh generator, the SFC
+ * renumbering and the packed TENO layout exactly as a context would upload
+ * them, for the index and precompute parity tests. The query functions map
+ * the packed arrays back to reference numbering:
+ *   mesh_arrays: node_coords[2Nn], cell_coords[2Nc], cell_volume[Nc],
+ *     face_area[Nf], face_normals[2Nf], nodes_of_cell[], faces_of_cell[],
+ *     nodes_of_face[2Nf], cells_of_face[2Nf] (any pointer may be NULL)
+ *   cell_stencils: ids[G*M] (target first, 0xFFFFFFFF for empty stencils)
+ *   cell_matrices: aplus[G*K*M] row-major K x M with the zero row/column,
+ *     ta[G*M], fbasis[S*Q*(K-1)]
+ *   teno_tables: ipt[K], oi[K*K]
+ *   slots: per face slot of the cell (S entries each): neighbour (reference
+ *     id, or the cell itself on a boundary), info word, unit normal x/y,
+ *     area, spectral-radius length. */
+typedef struct mlr_layout mlr_layout;
+int mlr_layout_create(const mlr_mesh_gen* gen, const mlr_numerics_desc* num, mlr_layout** out);
+int mlr_layout_destroy(mlr_layout* lay);
+int mlr_layout_sizes(const mlr_layout* lay, int64_t* sizes);
+int mlr_layout_permutation(const mlr_layout* lay, uint32_t* perm);
+int mlr_layout_mesh_arrays(const mlr_layout* lay, double* node_coords, double* cell_coords,
+                           double* cell_volume, double* face_area, double* face_normals,
+                           uint32_t* nodes_of_cell, uint32_t* faces_of_cell, uint32_t* nodes_of_face,
+                           int32_t* cells_of_face);
+int mlr_layout_cell_stencils(const mlr_layout* lay, uint32_t cell, uint32_t* ids);
+int mlr_layout_cell_matrices(const mlr_layout* lay, uint32_t cell, double* aplus, double* ta,
+                             double* fbasis);
+int mlr_layout_teno_tables(const mlr_layout* lay, double* ipt, double* oi);
+int mlr_layout_slots(const mlr_layout* lay, uint32_t cell, uint32_t* nb, uint32_t* info, double* nx,
+                     double* ny, double* area, double* dxn);
+
+/* Device math probes (0 sqrt, 1 divide, 2 pow, 3 x^6) for the parity tests. */
+int mlr_probe_math(int which, const double* x, const double* y, double* out, int n);
+/* The host x^6 used by the precompute and the tests. */
+int mlr_host_pow6(const double* x, double* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

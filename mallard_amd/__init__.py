@@ -1,0 +1,337 @@
+"""Python host bindings of the MI355X residual engine (ctypes over the C ABI in
+include/mallard_amd.h).
+
+The names mirror the reference's interfaces: ``Solver`` (solver.h:30-279) with
+``calc_rhs``, ``calc_dt``, ``take_step``, ``run``; reconstruction, Riemann
+solver, integrator and boundary keys are the reference's TOML strings.
+Everything computes in lib/libmallard_amd.so (HIP kernels for gfx950); there
+is no CPU fallback, and importing the module fails if the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmallard_amd.so")
+
+MESH = {"file": 0, "cartesian": 1, "cartesian_tri": 2, "wedge": 3}
+RECON = {"FO": 0, "TENO": 1}
+BASIS = {"monomial": 0, "legendre": 1}
+TENO_VARIANT = {"reference": 0, "corrected": 1}
+RIEMANN = {"Rusanov": 0, "HLL": 1, "HLLC": 2, "Roe": 3}
+INTEGRATOR = {"FE": 0, "RK4": 1, "SSPRK3": 2}
+BC = {"symmetry": 0, "extrapolation": 1, "wall_adiabatic": 2, "upt": 3, "p_out": 4}
+
+
+class MlrBC(C.Structure):
+    _fields_ = [("zone", C.c_char * 32), ("type", C.c_int32), ("u", C.c_double * 2),
+                ("p", C.c_double), ("T", C.c_double)]
+
+
+class MlrNumerics(C.Structure):
+    _fields_ = [("recon", C.c_int32), ("basis", C.c_int32), ("order", C.c_int32),
+                ("stencil_factor", C.c_double), ("quad_order_cell", C.c_int32),
+                ("quad_order_face", C.c_int32), ("teno_variant", C.c_int32),
+                ("riemann", C.c_int32), ("integrator", C.c_int32),
+                ("gamma", C.c_double), ("p_ref", C.c_double), ("T_ref", C.c_double),
+                ("rho_ref", C.c_double), ("p_min", C.c_double), ("p_max", C.c_double),
+                ("cfl", C.c_double), ("dt", C.c_double), ("n_bc", C.c_int32),
+                ("bc", MlrBC * 14), ("n_threads", C.c_int32)]
+
+
+class MlrMeshGen(C.Structure):
+    _fields_ = [("type", C.c_int32), ("nx", C.c_uint32), ("ny", C.c_uint32), ("Lx", C.c_double),
+                ("Ly", C.c_double), ("jitter", C.c_double), ("seed", C.c_uint64)]
+
+
+# every entry point of include/mallard_amd.h with its ctypes signature
+_P = C.c_void_p
+_SIGS = {
+    "mlr_ctx_create_toml": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "mlr_ctx_create_generated": (C.c_int, [C.POINTER(MlrMeshGen), C.POINTER(MlrNumerics), C.POINTER(_P)]),
+    "mlr_ctx_create": (C.c_int, [_P, C.POINTER(MlrNumerics), C.POINTER(_P)]),
+    "mlr_ctx_destroy": (C.c_int, [_P]),
+    "mlr_last_error": (C.c_char_p, [_P]),
+    "mlr_get_sizes": (C.c_int, [_P, _P]),
+    "mlr_init_quadrant": (C.c_int, [_P, C.c_double, C.c_double, _P]),
+    "mlr_init_constant": (C.c_int, [_P, _P, C.c_double, C.c_double]),
+    "mlr_set_state": (C.c_int, [_P, _P, _P]),
+    "mlr_get_state": (C.c_int, [_P, _P, _P]),
+    "mlr_calc_rhs": (C.c_int, [_P, _P]),
+    "mlr_compute_dt": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "mlr_step": (C.c_int, [_P]),
+    "mlr_run_steps": (C.c_int, [_P, C.c_int64, _P]),
+    "mlr_run": (C.c_int, [_P]),
+    "mlr_set_cfl": (C.c_int, [_P, C.c_double]),
+    "mlr_get_time": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "mlr_get_face_values": (C.c_int, [_P, _P]),
+    "mlr_get_decisions": (C.c_int, [_P, _P]),
+    "mlr_get_cfl_local": (C.c_int, [_P, _P]),
+    "mlr_get_permutation": (C.c_int, [_P, _P]),
+    "mlr_get_stencils": (C.c_int, [_P, C.c_uint32, _P, _P]),
+    "mlr_get_teno_matrices": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
+    "mlr_get_device_stencils": (C.c_int, [_P, C.c_uint32, _P]),
+    "mlr_layout_create": (C.c_int, [C.POINTER(MlrMeshGen), C.POINTER(MlrNumerics), C.POINTER(_P)]),
+    "mlr_layout_destroy": (C.c_int, [_P]),
+    "mlr_layout_sizes": (C.c_int, [_P, _P]),
+    "mlr_layout_permutation": (C.c_int, [_P, _P]),
+    "mlr_layout_mesh_arrays": (C.c_int, [_P] + [_P] * 9),
+    "mlr_layout_cell_stencils": (C.c_int, [_P, C.c_uint32, _P]),
+    "mlr_layout_cell_matrices": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
+    "mlr_layout_teno_tables": (C.c_int, [_P, _P, _P]),
+    "mlr_layout_slots": (C.c_int, [_P, C.c_uint32] + [_P] * 6),
+    "mlr_probe_math": (C.c_int, [C.c_int, _P, _P, _P, C.c_int]),
+    "mlr_host_pow6": (C.c_int, [_P, _P, C.c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load lib/libmallard_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"mallard_amd native library missing: {LIB_PATH} "
+                               "(build it with `make -C mallard_amd` or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _f64(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+class MallardError(RuntimeError):
+    pass
+
+
+def numerics(recon="TENO", basis="legendre", order=2, stencil_factor=2.0, quad_order_cell=0,
+             quad_order_face=0, teno_variant="reference", riemann="HLLC", integrator="SSPRK3",
+             gamma=1.4, p_ref=101325.0, T_ref=298.15, rho_ref=1.225, p_min=-1e20, p_max=1e20,
+             cfl=0.4, dt=0.0, bcs=None, n_threads=0):
+    """Numerics descriptor with the reference's defaults and key strings.
+
+    bcs: list of (zone, type, params) with the [[boundaries]] keys.
+    """
+    d = MlrNumerics()
+    d.recon, d.basis, d.order = RECON[recon], BASIS[basis], order
+    d.stencil_factor, d.quad_order_cell, d.quad_order_face = stencil_factor, quad_order_cell, quad_order_face
+    d.teno_variant = TENO_VARIANT[teno_variant]
+    d.riemann, d.integrator = RIEMANN[riemann], INTEGRATOR[integrator]
+    d.gamma, d.p_ref, d.T_ref, d.rho_ref, d.p_min, d.p_max = gamma, p_ref, T_ref, rho_ref, p_min, p_max
+    d.cfl, d.dt = cfl, dt
+    bcs = bcs if bcs is not None else [(z, "symmetry", {}) for z in ("left", "right", "top", "bottom")]
+    if len(bcs) > 14:
+        raise ValueError("at most 14 boundary conditions")
+    d.n_bc = len(bcs)
+    for i, (zone, kind, par) in enumerate(bcs):
+        b = d.bc[i]
+        b.zone = zone.encode()
+        b.type = BC[kind]
+        u = par.get("u", (0.0, 0.0))
+        b.u[0], b.u[1] = u
+        b.p, b.T = par.get("p", 0.0), par.get("T", 0.0)
+    d.n_threads = n_threads
+    return d
+
+
+def mesh_gen(kind, nx, ny, Lx=1.0, Ly=1.0, jitter=0.0, seed=0):
+    g = MlrMeshGen()
+    g.type, g.nx, g.ny, g.Lx, g.Ly, g.jitter, g.seed = MESH[kind], nx, ny, Lx, Ly, jitter, seed
+    return g
+
+
+class Layout:
+    """Host-only precompute (mesh, SFC order, packed TENO layout); no GPU."""
+
+    def __init__(self, gen, num):
+        L = lib()
+        h = C.c_void_p()
+        if L.mlr_layout_create(C.byref(gen), C.byref(num), C.byref(h)):
+            raise MallardError(L.mlr_last_error(None).decode())
+        self.h = h
+        s = np.zeros(12, np.int64)
+        L.mlr_layout_sizes(self.h, _p(s))
+        (self.n_cells, self.n_faces, self.n_faces_real, self.S, self.Q, self.K, self.M, self.G,
+         self.device_bytes, self.n_nodes, self.n_noc, self.n_zones) = map(int, s)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mlr_layout_destroy(self.h)
+            self.h = None
+
+    def permutation(self):
+        p = np.zeros(self.n_cells, np.uint32)
+        lib().mlr_layout_permutation(self.h, _p(p))
+        return p
+
+    def mesh_arrays(self):
+        nc, nf = self.n_cells, self.n_faces
+        out = dict(node_coords=np.zeros((self.n_nodes, 2)), cell_coords=np.zeros((nc, 2)),
+                   cell_volume=np.zeros(nc), face_area=np.zeros(nf), face_normals=np.zeros((nf, 2)),
+                   nodes_of_cell=np.zeros(self.n_noc, np.uint32), faces_of_cell=np.zeros(self.n_noc, np.uint32),
+                   nodes_of_face=np.zeros(2 * nf, np.uint32), cells_of_face=np.zeros((nf, 2), np.int32))
+        lib().mlr_layout_mesh_arrays(self.h, *[_p(out[k]) for k in (
+            "node_coords", "cell_coords", "cell_volume", "face_area", "face_normals", "nodes_of_cell",
+            "faces_of_cell", "nodes_of_face", "cells_of_face")])
+        return out
+
+    def cell_stencils(self, cell):
+        ids = np.zeros(self.G * self.M, np.uint32)
+        if lib().mlr_layout_cell_stencils(self.h, cell, _p(ids)):
+            raise MallardError(lib().mlr_last_error(None).decode())
+        return ids.reshape(self.G, self.M)
+
+    def cell_matrices(self, cell):
+        a = np.zeros((self.G, self.K, self.M))
+        t = np.zeros((self.G, self.M))
+        fb = np.zeros((self.S, self.Q, self.K - 1))
+        if lib().mlr_layout_cell_matrices(self.h, cell, _p(a), _p(t), _p(fb)):
+            raise MallardError(lib().mlr_last_error(None).decode())
+        return a, t, fb
+
+    def teno_tables(self):
+        ipt = np.zeros(self.K)
+        oi = np.zeros(self.K * self.K)
+        lib().mlr_layout_teno_tables(self.h, _p(ipt), _p(oi))
+        return ipt, oi.reshape(self.K, self.K)
+
+    def slots(self, cell):
+        S = self.S
+        nb = np.zeros(S, np.uint32)
+        info = np.zeros(S, np.uint32)
+        nx, ny, area, dxn = (np.zeros(S) for _ in range(4))
+        lib().mlr_layout_slots(self.h, cell, _p(nb), _p(info), _p(nx), _p(ny), _p(area), _p(dxn))
+        return dict(nb=nb, info=info, nx=nx, ny=ny, area=area, dxn=dxn)
+
+
+class Solver:
+    """GPU solver context (Solver, solver.h:30-279) behind the C ABI."""
+
+    def __init__(self, toml_path=None, gen=None, num=None):
+        L = lib()
+        h = C.c_void_p()
+        if toml_path is not None:
+            rc = L.mlr_ctx_create_toml(os.fspath(toml_path).encode(), C.byref(h))
+        else:
+            rc = L.mlr_ctx_create_generated(C.byref(gen), C.byref(num), C.byref(h))
+        if rc:
+            raise MallardError(L.mlr_last_error(None).decode())
+        self.h = h
+        s = np.zeros(12, np.int64)
+        L.mlr_get_sizes(self.h, _p(s))
+        self.n_cells, self.n_faces, self.n_faces_real, self.S, self.Q, self.K, self.M, self.G = map(int, s[:8])
+        self.device_bytes = int(s[8])
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mlr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise MallardError(lib().mlr_last_error(self.h).decode())
+
+    def init_quadrant(self, xs, ys, states):
+        st = _f64(states).reshape(16)
+        self._check(lib().mlr_init_quadrant(self.h, xs, ys, _p(st)))
+
+    def init_constant(self, u, p, T):
+        self._check(lib().mlr_init_constant(self.h, _p(_f64(u)), p, T))
+
+    def set_state(self, U, prim=None):
+        U = _f64(U)
+        prim = _f64(prim) if prim is not None else None
+        self._check(lib().mlr_set_state(self.h, _p(U), _p(prim)))
+
+    def state(self):
+        U = np.zeros((self.n_cells, 4))
+        W = np.zeros((self.n_cells, 5))
+        self._check(lib().mlr_get_state(self.h, _p(U), _p(W)))
+        return U, W
+
+    def calc_rhs(self):
+        r = np.zeros((self.n_cells, 4))
+        self._check(lib().mlr_calc_rhs(self.h, _p(r)))
+        return r
+
+    def calc_dt(self):
+        dt = C.c_double()
+        self._check(lib().mlr_compute_dt(self.h, C.byref(dt)))
+        return dt.value
+
+    def take_step(self):
+        self._check(lib().mlr_step(self.h))
+
+    def run_steps(self, n):
+        dts = np.zeros(n)
+        self._check(lib().mlr_run_steps(self.h, n, _p(dts)))
+        return dts
+
+    def run(self):
+        self._check(lib().mlr_run(self.h))
+
+    def set_cfl(self, cfl):
+        self._check(lib().mlr_set_cfl(self.h, cfl))
+
+    def time(self):
+        t, s = C.c_double(), C.c_int64()
+        lib().mlr_get_time(self.h, C.byref(t), C.byref(s))
+        return t.value, s.value
+
+    def face_values(self):
+        out = np.zeros((self.n_faces, self.Q, 2, 4))
+        self._check(lib().mlr_get_face_values(self.h, _p(out)))
+        return out
+
+    def decisions(self):
+        d = np.zeros(self.n_cells, np.uint8)
+        self._check(lib().mlr_get_decisions(self.h, _p(d)))
+        return d
+
+    def cfl_local(self):
+        d = np.zeros(self.n_cells)
+        self._check(lib().mlr_get_cfl_local(self.h, _p(d)))
+        return d
+
+    def permutation(self):
+        p = np.zeros(self.n_cells, np.uint32)
+        self._check(lib().mlr_get_permutation(self.h, _p(p)))
+        return p
+
+    def device_stencils(self, cell):
+        ids = np.zeros(self.G * self.M, np.uint32)
+        self._check(lib().mlr_get_device_stencils(self.h, cell, _p(ids)))
+        return ids.reshape(self.G, self.M)
+
+
+def probe_math(which, x, y=None):
+    """Device sqrt (0), division (1), pow (2) or x^6 (3) of arrays."""
+    x = _f64(x)
+    y = _f64(y) if y is not None else x
+    out = np.zeros_like(x)
+    rc = lib().mlr_probe_math(which, _p(x), _p(y), _p(out), x.size)
+    if rc:
+        raise MallardError(lib().mlr_last_error(None).decode())
+    return out
+
+
+def host_pow6(x):
+    x = _f64(x)
+    out = np.zeros_like(x)
+    lib().mlr_host_pow6(_p(x), _p(out), x.size)
+    return out

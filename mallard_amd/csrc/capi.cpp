@@ -1,0 +1,496 @@
+// extern "C" boundary (include/mallard_amd.h): converts exceptions and HIP
+// failures into status codes plus a per-context error string.
+#include "../../include/mallard_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "solver.hpp"
+
+struct mlr_ctx {
+    mlr::Solver solver;
+    std::string err;
+};
+
+struct mlr_layout {
+    mlr::Config cfg;
+    mlr::Mesh mesh;
+    mlr::Layout L;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(mlr_ctx* ctx, F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        (ctx ? ctx->err : g_err) = e.what();
+        return 1;
+    } catch (...) {
+        (ctx ? ctx->err : g_err) = "unknown error";
+        return 1;
+    }
+}
+
+mlr::Config config_from_desc(const mlr_numerics_desc* d) {
+    using namespace mlr;
+    if (!d) fail("numerics descriptor is NULL");
+    Config c;
+    if (d->recon != MLR_RECON_FO && d->recon != MLR_RECON_TENO) fail("Unknown face reconstruction type.");
+    c.recon = d->recon == MLR_RECON_TENO ? Recon::TENO : Recon::FO;
+    c.teno.basis = d->basis == MLR_BASIS_LEGENDRE ? BasisType::Legendre : BasisType::Monomial;
+    c.teno.order = d->order;
+    c.teno.factor = d->stencil_factor > 0 ? d->stencil_factor : 2.0;
+    c.teno.quad_order_cell = d->quad_order_cell;
+    c.teno.quad_order_face = d->quad_order_face;
+    if (d->teno_variant != MLR_TENO_REFERENCE && d->teno_variant != MLR_TENO_CORRECTED) fail("Unknown TENO variant.");
+    c.teno.variant = (TenoVariant)d->teno_variant;
+    if (d->riemann < 0 || d->riemann > 3) fail("Unknown Riemann solver type.");
+    c.riemann = (Riemann)d->riemann;
+    if (d->integrator == MLR_TI_FE) c.integrator = Integrator::FE;
+    else if (d->integrator == MLR_TI_RK4) c.integrator = Integrator::RK4;
+    else if (d->integrator == MLR_TI_SSPRK3) c.integrator = Integrator::SSPRK3;
+    else fail("Unknown time integrator type.");
+    c.gamma = d->gamma;
+    c.p_ref = d->p_ref;
+    c.T_ref = d->T_ref;
+    c.rho_ref = d->rho_ref;
+    c.p_min = d->p_min;
+    c.p_max = d->p_max;
+    c.use_cfl = d->cfl > 0;
+    c.cfl = d->cfl;
+    c.dt = d->dt;
+    if (d->n_bc < 0 || d->n_bc > 14) fail("n_bc out of range");
+    for (int b = 0; b < d->n_bc; ++b) {
+        BoundarySpec s;
+        s.zone = std::string(d->bc[b].zone, strnlen(d->bc[b].zone, sizeof(d->bc[b].zone)));
+        if (d->bc[b].type < 0 || d->bc[b].type > 4) fail("Unknown boundary type.");
+        s.type = (BCType)d->bc[b].type;
+        s.u[0] = d->bc[b].u[0];
+        s.u[1] = d->bc[b].u[1];
+        s.p = d->bc[b].p;
+        s.T = d->bc[b].T;
+        c.bcs.push_back(s);
+    }
+    c.n_threads = d->n_threads;
+    c.n_steps = 0;
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mlr_ctx_create_toml(const char* path, mlr_ctx** out) {
+    if (!out) return 1;
+    *out = nullptr;
+    auto ctx = std::make_unique<mlr_ctx>();
+    const int rc = guarded(nullptr, [&] { ctx->solver.init(std::string(path ? path : "")); });
+    if (rc == 0) *out = ctx.release();
+    return rc;
+}
+
+int mlr_ctx_create_generated(const mlr_mesh_gen* gen, const mlr_numerics_desc* num, mlr_ctx** out) {
+    if (!out) return 1;
+    *out = nullptr;
+    auto ctx = std::make_unique<mlr_ctx>();
+    const int rc = guarded(nullptr, [&] {
+        if (!gen) mlr::fail("mesh generator descriptor is NULL");
+        mlr::Config c = config_from_desc(num);
+        if (gen->type == MLR_MESH_CARTESIAN) c.mesh_type = mlr::MeshType::Cartesian;
+        else if (gen->type == MLR_MESH_CARTESIAN_TRI) c.mesh_type = mlr::MeshType::CartesianTri;
+        else if (gen->type == MLR_MESH_WEDGE) c.mesh_type = mlr::MeshType::Wedge;
+        else mlr::fail("MeshType::FILE not implemented.");
+        c.nx = gen->nx;
+        c.ny = gen->ny;
+        c.Lx = gen->Lx;
+        c.Ly = gen->Ly;
+        c.jitter = gen->jitter;
+        c.seed = gen->seed;
+        ctx->solver.init(c);
+    });
+    if (rc == 0) *out = ctx.release();
+    return rc;
+}
+
+int mlr_ctx_create(const mlr_mesh_desc* d, const mlr_numerics_desc* num, mlr_ctx** out) {
+    if (!out) return 1;
+    *out = nullptr;
+    auto ctx = std::make_unique<mlr_ctx>();
+    const int rc = guarded(nullptr, [&] {
+        if (!d) mlr::fail("mesh descriptor is NULL");
+        mlr::Config c = config_from_desc(num);
+        mlr::Mesh m = mlr::make_from_arrays(d->n_nodes, d->node_coords, d->n_cells, d->offsets_nodes_of_cell,
+                                            d->nodes_of_cell, d->faces_of_cell, d->n_faces, d->nodes_of_face,
+                                            d->cells_of_face, d->n_zones, d->zone_names, d->zone_offsets,
+                                            d->zone_faces);
+        ctx->solver.init(c, std::move(m));
+    });
+    if (rc == 0) *out = ctx.release();
+    return rc;
+}
+
+int mlr_ctx_destroy(mlr_ctx* ctx) {
+    delete ctx;
+    return 0;
+}
+
+const char* mlr_last_error(const mlr_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int mlr_get_sizes(const mlr_ctx* ctx, int64_t* s) {
+    if (!ctx || !s) return 1;
+    const mlr::Layout& L = ctx->solver.layout();
+    const mlr::Mesh& m = ctx->solver.mesh();
+    s[0] = m.n_cells;
+    s[1] = m.n_faces;
+    s[2] = m.n_faces_real;
+    s[3] = L.S;
+    s[4] = L.Q;
+    s[5] = L.K;
+    s[6] = L.M;
+    s[7] = L.G;
+    s[8] = (int64_t)L.device_bytes();
+    s[9] = ctx->solver.step();
+    s[10] = (int64_t)ctx->solver.config().recon;
+    s[11] = (int64_t)L.tspec.variant;
+    return 0;
+}
+
+int mlr_init_quadrant(mlr_ctx* ctx, double xs, double ys, const double* states) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] {
+        mlr::Config c = ctx->solver.config();
+        (void)c;
+        const mlr::Mesh& m = ctx->solver.mesh();
+        const auto& ph_cfg = ctx->solver.config();
+        const double R = ph_cfg.p_ref / (ph_cfg.T_ref * ph_cfg.rho_ref);
+        const double cp = R * ph_cfg.gamma / (ph_cfg.gamma - 1.0);
+        const double cv = cp / ph_cfg.gamma;
+        std::vector<double> U(4 * (size_t)m.n_cells), W(5 * (size_t)m.n_cells);
+        for (uint32_t c2 = 0; c2 < m.n_cells; ++c2) {
+            const bool r = m.cc(c2, 0) >= xs, t = m.cc(c2, 1) >= ys;
+            const int q = r ? (t ? 0 : 3) : (t ? 1 : 2);
+            const double rho = states[4 * q], u0 = states[4 * q + 1], u1 = states[4 * q + 2], p = states[4 * q + 3];
+            const double T = p / (rho * R);
+            const double e = cv * T;
+            const double E = e + 0.5 * (u0 * u0 + u1 * u1);
+            const double h = e + p / rho;
+            double* Uc = &U[4 * (size_t)c2];
+            Uc[0] = rho; Uc[1] = rho * u0; Uc[2] = rho * u1; Uc[3] = rho * E;
+            double* Wc = &W[5 * (size_t)c2];
+            Wc[0] = u0; Wc[1] = u1; Wc[2] = p; Wc[3] = T; Wc[4] = h;
+        }
+        ctx->solver.set_state(U.data(), W.data());
+    });
+}
+
+int mlr_init_constant(mlr_ctx* ctx, const double* u, double p, double T) {
+    if (!ctx || !u) return 1;
+    return guarded(ctx, [&] {
+        const mlr::Mesh& m = ctx->solver.mesh();
+        const auto& cf = ctx->solver.config();
+        const double R = cf.p_ref / (cf.T_ref * cf.rho_ref);
+        const double cp = R * cf.gamma / (cf.gamma - 1.0);
+        const double cv = cp / cf.gamma;
+        const double rho = p / (R * T);
+        const double e = cv * T;
+        const double E = e + 0.5 * (u[0] * u[0] + u[1] * u[1]);
+        const double h = e + p / rho;
+        std::vector<double> U(4 * (size_t)m.n_cells), W(5 * (size_t)m.n_cells);
+        for (uint32_t c2 = 0; c2 < m.n_cells; ++c2) {
+            double* Uc = &U[4 * (size_t)c2];
+            Uc[0] = rho; Uc[1] = rho * u[0]; Uc[2] = rho * u[1]; Uc[3] = rho * E;
+            double* Wc = &W[5 * (size_t)c2];
+            Wc[0] = u[0]; Wc[1] = u[1]; Wc[2] = p; Wc[3] = T; Wc[4] = h;
+        }
+        ctx->solver.set_state(U.data(), W.data());
+    });
+}
+
+int mlr_set_state(mlr_ctx* ctx, const double* U, const double* prim) {
+    if (!ctx || !U) return 1;
+    return guarded(ctx, [&] { ctx->solver.set_state(U, prim); });
+}
+
+int mlr_get_state(mlr_ctx* ctx, double* U, double* prim) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.get_state(U, prim); });
+}
+
+int mlr_calc_rhs(mlr_ctx* ctx, double* rhs) {
+    if (!ctx || !rhs) return 1;
+    return guarded(ctx, [&] { ctx->solver.calc_rhs(rhs); });
+}
+
+int mlr_compute_dt(mlr_ctx* ctx, double* dt_out) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] {
+        const double dt = ctx->solver.calc_dt();
+        if (dt_out) *dt_out = dt;
+    });
+}
+
+int mlr_step(mlr_ctx* ctx) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.take_step(); });
+}
+
+int mlr_run_steps(mlr_ctx* ctx, int64_t n, double* dts) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.run_steps(n, dts); });
+}
+
+int mlr_run(mlr_ctx* ctx) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.run(); });
+}
+
+int mlr_set_cfl(mlr_ctx* ctx, double cfl) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.set_cfl(cfl); });
+}
+
+int mlr_get_time(const mlr_ctx* ctx, double* t, int64_t* step) {
+    if (!ctx) return 1;
+    if (t) *t = ctx->solver.time();
+    if (step) *step = ctx->solver.step();
+    return 0;
+}
+
+int mlr_get_face_values(mlr_ctx* ctx, double* out) {
+    if (!ctx || !out) return 1;
+    return guarded(ctx, [&] { ctx->solver.get_face_values(out); });
+}
+
+int mlr_get_decisions(mlr_ctx* ctx, uint8_t* out) {
+    if (!ctx || !out) return 1;
+    return guarded(ctx, [&] { ctx->solver.get_decisions(out); });
+}
+
+int mlr_get_cfl_local(mlr_ctx* ctx, double* out) {
+    if (!ctx || !out) return 1;
+    return guarded(ctx, [&] { ctx->solver.get_cfl_local(out); });
+}
+
+int mlr_get_permutation(const mlr_ctx* ctx, uint32_t* perm) {
+    if (!ctx || !perm) return 1;
+    const auto& p = ctx->solver.layout().perm;
+    std::memcpy(perm, p.data(), p.size() * sizeof(uint32_t));
+    return 0;
+}
+
+int mlr_get_stencils(const mlr_ctx* ctx, uint32_t cell, uint32_t* ids, uint32_t* sizes) {
+    if (!ctx) return 1;
+    return guarded(const_cast<mlr_ctx*>(ctx), [&] {
+        const mlr::Layout& L = ctx->solver.layout();
+        if (!L.teno) mlr::fail("stencils exist only for TENO");
+        if (cell >= ctx->solver.mesh().n_cells) mlr::fail("cell out of range");
+        std::vector<std::vector<uint32_t>> S;
+        mlr::stencils_of_cell(ctx->solver.mesh(), L.M, cell, S);
+        for (int s = 0; s < L.G; ++s) {
+            sizes[s] = (uint32_t)S[s].size();
+            for (int i = 0; i < L.M; ++i) ids[s * L.M + i] = i < (int)S[s].size() ? S[s][i] : 0xFFFFFFFFu;
+        }
+    });
+}
+
+int mlr_get_teno_matrices(const mlr_ctx* ctx, uint32_t cell, double* aplus, double* ta, double* ipt,
+                          double* oi) {
+    if (!ctx) return 1;
+    return guarded(const_cast<mlr_ctx*>(ctx), [&] {
+        const mlr::Layout& L = ctx->solver.layout();
+        if (!L.teno) mlr::fail("matrices exist only for TENO");
+        if (cell >= ctx->solver.mesh().n_cells) mlr::fail("cell out of range");
+        std::vector<std::vector<double>> A, T;
+        std::vector<double> ip;
+        mlr::teno_matrices_of_cell(ctx->solver.mesh(), L.tspec, cell, A, T, ip);
+        const int K = L.K, M = L.M;
+        for (int s = 0; s < L.G; ++s)
+            for (int i = 0; i < K * M; ++i) aplus[(size_t)s * K * M + i] = A[s].empty() ? 0.0 : A[s][i];
+        for (int s = 0; s < L.G; ++s)
+            for (int i = 0; i < M; ++i) ta[s * M + i] = T[s].empty() ? 0.0 : T[s][i];
+        if (ipt) for (int k = 0; k < K; ++k) ipt[k] = ip[k];
+        if (oi) std::memcpy(oi, L.OI.data(), L.OI.size() * sizeof(double));
+    });
+}
+
+int mlr_get_device_stencils(const mlr_ctx* ctx, uint32_t cell, uint32_t* ids) {
+    if (!ctx) return 1;
+    return guarded(const_cast<mlr_ctx*>(ctx), [&] { ctx->solver.device_stencils(cell, ids); });
+}
+
+int mlr_layout_create(const mlr_mesh_gen* gen, const mlr_numerics_desc* num, mlr_layout** out) {
+    if (!out) return 1;
+    *out = nullptr;
+    auto lay = std::make_unique<mlr_layout>();
+    const int rc = guarded(nullptr, [&] {
+        if (!gen) mlr::fail("mesh generator descriptor is NULL");
+        lay->cfg = config_from_desc(num);
+        if (gen->type == MLR_MESH_CARTESIAN) lay->mesh = mlr::make_cartesian(gen->nx, gen->ny, gen->Lx, gen->Ly, gen->jitter, gen->seed);
+        else if (gen->type == MLR_MESH_CARTESIAN_TRI) lay->mesh = mlr::make_cartesian_tri(gen->nx, gen->ny, gen->Lx, gen->Ly, gen->jitter, gen->seed);
+        else if (gen->type == MLR_MESH_WEDGE) lay->mesh = mlr::make_wedge(gen->nx, gen->ny, gen->Lx, gen->Ly);
+        else mlr::fail("MeshType::FILE not implemented.");
+        lay->L = mlr::build_layout(lay->mesh, lay->cfg.recon, lay->cfg.teno, lay->cfg.bcs, lay->cfg.n_threads);
+    });
+    if (rc == 0) *out = lay.release();
+    return rc;
+}
+
+int mlr_layout_destroy(mlr_layout* lay) {
+    delete lay;
+    return 0;
+}
+
+int mlr_layout_sizes(const mlr_layout* lay, int64_t* s) {
+    if (!lay || !s) return 1;
+    const mlr::Layout& L = lay->L;
+    s[0] = lay->mesh.n_cells;
+    s[1] = lay->mesh.n_faces;
+    s[2] = lay->mesh.n_faces_real;
+    s[3] = L.S;
+    s[4] = L.Q;
+    s[5] = L.K;
+    s[6] = L.M;
+    s[7] = L.G;
+    s[8] = (int64_t)L.device_bytes();
+    s[9] = lay->mesh.n_nodes;
+    s[10] = (int64_t)lay->mesh.nodes_of_cell.size();
+    s[11] = (int64_t)lay->mesh.zones.size();
+    return 0;
+}
+
+int mlr_layout_permutation(const mlr_layout* lay, uint32_t* perm) {
+    if (!lay || !perm) return 1;
+    std::memcpy(perm, lay->L.perm.data(), lay->L.perm.size() * 4);
+    return 0;
+}
+
+int mlr_layout_mesh_arrays(const mlr_layout* lay, double* nc, double* cc, double* vol, double* fa, double* fn,
+                           uint32_t* noc, uint32_t* foc, uint32_t* nof, int32_t* cof) {
+    if (!lay) return 1;
+    const mlr::Mesh& m = lay->mesh;
+    auto cp = [](auto* dst, const auto& src) { if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(src[0])); };
+    cp(nc, m.node_coords); cp(cc, m.cell_coords); cp(vol, m.cell_volume); cp(fa, m.face_area);
+    cp(fn, m.face_normals); cp(noc, m.nodes_of_cell); cp(foc, m.faces_of_cell); cp(nof, m.nodes_of_face);
+    cp(cof, m.cells_of_face);
+    return 0;
+}
+
+int mlr_layout_cell_stencils(const mlr_layout* lay, uint32_t cell, uint32_t* ids) {
+    if (!lay || !ids) return 1;
+    return guarded(nullptr, [&] {
+        const mlr::Layout& L = lay->L;
+        if (!L.teno) mlr::fail("stencils exist only for TENO");
+        if (cell >= L.N) mlr::fail("cell out of range");
+        const uint32_t j = L.iperm[cell], N = L.N;
+        const int Mp = L.M - 1;
+        for (int s = 0; s < L.G; ++s) {
+            const bool on = (L.info[j] >> (20 + s)) & 1u;
+            ids[(size_t)s * L.M] = on ? cell : 0xFFFFFFFFu;
+            for (int i = 0; i < Mp; ++i)
+                ids[(size_t)s * L.M + 1 + i] = on ? L.perm[L.st_ids[((size_t)s * Mp + i) * N + j]] : 0xFFFFFFFFu;
+        }
+    });
+}
+
+int mlr_layout_cell_matrices(const mlr_layout* lay, uint32_t cell, double* aplus, double* ta, double* fbasis) {
+    if (!lay) return 1;
+    return guarded(nullptr, [&] {
+        const mlr::Layout& L = lay->L;
+        if (!L.teno) mlr::fail("matrices exist only for TENO");
+        if (cell >= L.N) mlr::fail("cell out of range");
+        const uint32_t j = L.iperm[cell], N = L.N;
+        const int K = L.K, M = L.M, Kp = K - 1, Mp = M - 1;
+        // the target's transformed area is identical in all stencils; the
+        // reference variant folds it into fbasis, so it is recovered from
+        // the cell geometry here
+        double w0[2], w1[2], w2[2], J[4], Ji[4];
+        const mlr::Mesh& m = lay->mesh;
+        for (int i = 0; i < 2; ++i) {
+            w0[i] = m.nc(m.node_of_cell(cell, 0), i);
+            w1[i] = m.nc(m.node_of_cell(cell, 1), i);
+            w2[i] = m.nc(m.node_of_cell(cell, 2), i);
+        }
+        mlr::tri_J(w0, w1, w2, J);
+        mlr::invert2(J, Ji);
+        double t0[2] = {0.0, 0.0}, t1[2], t2[2];
+        for (int i = 0; i < 2; ++i) { t0[i] = w0[i] - w0[i]; t1[i] = w1[i] - w0[i]; t2[i] = w2[i] - w0[i]; }
+        mlr::gemv2(Ji, t0, t0);
+        mlr::gemv2(Ji, t1, t1);
+        mlr::gemv2(Ji, t2, t2);
+        const double ta0 = mlr::tri_area(t0, t1, t2);
+        for (int s = 0; s < L.G; ++s) {
+            const bool on = (L.info[j] >> (20 + s)) & 1u;
+            for (int k = 0; k < K; ++k)
+                for (int i = 0; i < M; ++i) {
+                    double v = 0.0;
+                    if (on && k > 0 && i > 0) v = L.st_A[(((size_t)s * Mp + (i - 1)) * Kp + (k - 1)) * N + j];
+                    if (aplus) aplus[((size_t)s * K + k) * M + i] = v;
+                }
+            if (ta)
+                for (int i = 0; i < M; ++i)
+                    ta[(size_t)s * M + i] = !on ? 0.0 : (i == 0 ? ta0 : L.st_ta[((size_t)s * Mp + (i - 1)) * N + j]);
+        }
+        if (fbasis)
+            for (uint32_t l = 0; l < L.S; ++l)
+                for (uint32_t q = 0; q < L.Q; ++q)
+                    for (int k = 0; k < Kp; ++k)
+                        fbasis[((size_t)l * L.Q + q) * Kp + k] = L.fbasis[(((size_t)l * L.Q + q) * Kp + k) * N + j];
+    });
+}
+
+int mlr_layout_teno_tables(const mlr_layout* lay, double* ipt, double* oi) {
+    if (!lay) return 1;
+    if (ipt) std::memcpy(ipt, lay->L.ipt.data(), lay->L.ipt.size() * 8);
+    if (oi) std::memcpy(oi, lay->L.OI.data(), lay->L.OI.size() * 8);
+    return 0;
+}
+
+int mlr_layout_slots(const mlr_layout* lay, uint32_t cell, uint32_t* nb, uint32_t* info, double* nx, double* ny,
+                     double* area, double* dxn) {
+    if (!lay || cell >= lay->L.N) return 1;
+    const mlr::Layout& L = lay->L;
+    const uint32_t j = L.iperm[cell];
+    for (uint32_t l = 0; l < L.S; ++l) {
+        const size_t o = (size_t)l * L.N + j;
+        if (nb) nb[l] = L.perm[L.nb[o]];
+        if (info) info[l] = L.info[o];
+        if (nx) nx[l] = L.nx[o];
+        if (ny) ny[l] = L.ny[o];
+        if (area) area[l] = L.area[o];
+        if (dxn) dxn[l] = L.dxn[o];
+    }
+    return 0;
+}
+
+int mlr_probe_math(int which, const double* x, const double* y, double* out, int n) {
+    return guarded(nullptr, [&] {
+        if (n <= 0) return;
+        double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+        auto chk = [](hipError_t e) { if (e != hipSuccess) mlr::fail(hipGetErrorString(e)); };
+        chk(hipMalloc(&dx, n * 8));
+        chk(hipMalloc(&dy, n * 8));
+        chk(hipMalloc(&dout, n * 8));
+        chk(hipMemcpy(dx, x, n * 8, hipMemcpyHostToDevice));
+        chk(hipMemcpy(dy, y ? y : x, n * 8, hipMemcpyHostToDevice));
+        chk(mlr::dev::probe_math(dx, dy, dout, n, which, nullptr));
+        chk(hipMemcpy(out, dout, n * 8, hipMemcpyDeviceToHost));
+        (void)hipFree(dx);
+        (void)hipFree(dy);
+        (void)hipFree(dout);
+    });
+}
+
+int mlr_host_pow6(const double* x, double* out, int n) {
+    for (int i = 0; i < n; ++i) out[i] = mlr::pow6_cr(x[i]);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,683 @@
+// HIP kernels of the MI355X residual engine (gfx950, wave64, FP64).
+//
+// Numerics follow the reference operation by operation so that, compiled
+// with -ffp-contract=off, every result matches the CPU oracle bit for bit:
+//   - TENO reconstruction: face_reconstruction.cpp:866-1039
+//   - Riemann fluxes:      riemann_solver.h:205-519
+//   - flux functors / BCs: flux_functor.h:124-178, boundary_*.cpp
+//   - residual assembly:   solver_rhs.cpp:44-55 (canonical one-thread order)
+//   - SSP-RK / RK4 / FE:   time_integrator.cpp:57-163
+//   - spectral radius, dt: solver.cpp:580-742
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device.hpp"
+
+namespace mlr::dev {
+
+namespace {
+
+constexpr uint32_t kBoundaryBit = 1u << 3;
+constexpr int kMaxKp = 9;
+
+
+struct D4 {
+    double v[4];
+};
+
+__device__ inline D4 ld4(const double* __restrict__ p, size_t i) {
+    const double2* q = reinterpret_cast<const double2*>(p + 4 * i);
+    const double2 a = q[0], b = q[1];
+    return D4{{a.x, a.y, b.x, b.y}};
+}
+__device__ inline void st4(double* __restrict__ p, size_t i, const D4& x) {
+    double2* q = reinterpret_cast<double2*>(p + 4 * i);
+    q[0] = make_double2(x.v[0], x.v[1]);
+    q[1] = make_double2(x.v[2], x.v[3]);
+}
+
+// Correctly rounded x^6 (double-double product, exact FMA residuals); equals
+// glibc's pow(x, 6.0) wherever that is correctly rounded.
+__device__ inline double pow6_cr(double x) {
+    const double h2 = x * x;
+    const double l2 = __builtin_fma(x, x, -h2);
+    const double p3 = h2 * x;
+    const double e3 = __builtin_fma(h2, x, -p3) + l2 * x;
+    const double h3 = p3 + e3;
+    const double l3 = e3 - (h3 - p3);
+    const double p6 = h3 * h3;
+    const double e6 = __builtin_fma(h3, h3, -p6) + 2.0 * h3 * l3;
+    return p6 + e6;
+}
+
+// ------------------------------------------------------------ physics
+// physics.h:842-867
+__device__ inline void prims(const Physics& ph, const double* U, double* W) {
+    const double rho = U[0];
+    const double u0 = U[1] / rho, u1 = U[2] / rho;
+    const double E = U[3] / rho;
+    const double e = E - 0.5 * (u0 * u0 + u1 * u1);
+    const double p = fmax(ph.p_min, fmin(ph.p_max, (ph.gamma - 1.0) * rho * e));
+    const double T = e / ph.cv;
+    const double h = e + p / rho;
+    W[0] = u0; W[1] = u1; W[2] = p; W[3] = T; W[4] = h;
+}
+
+// ------------------------------------------------------------ Riemann
+// riemann_solver.h:205-240
+__device__ inline void pvrs(const double* Wl, const double* Wr, double& ps) {
+    const double rl = Wl[0], ul = Wl[1], pl = Wl[2], gl = Wl[3];
+    const double rr = Wr[0], ur = Wr[1], pr = Wr[2], gr = Wr[3];
+    const double al = sqrt(gl * pl / rl), ar = sqrt(gr * pr / rr);
+    const double ravg = 0.5 * (rl + rr), aavg = 0.5 * (al + ar);
+    ps = 0.5 * (pl + pr) + 0.5 * (ul - ur) * ravg * aavg;
+}
+// riemann_solver.h:242-271 (pressure only; the star densities are unused)
+__device__ inline void trrs(const double* Wl, const double* Wr, double& ps, int guard) {
+    const double rl = Wl[0], ul = Wl[1], pl = Wl[2], gl = Wl[3];
+    const double rr = Wr[0], ur = Wr[1], pr = Wr[2], gr = Wr[3];
+    const double al = sqrt(gl * pl / rl), ar = sqrt(gr * pr / rr);
+    const double zl = (gl - 1.0) / (2.0 * gl), zr = (gr - 1.0) / (2.0 * gr);
+    const double Plr = pow((pl / pr), zl);
+    const double us = (Plr * ul / al + ur / ar + 2.0 * (1.0 - Plr) / (gl - 1.0)) / (Plr / al + 1.0 / ar);
+    double bl = 1.0 + (gl - 1.0) / (2.0 * al) * (ul - us);
+    double br = 1.0 + (gr - 1.0) / (2.0 * ar) * (us - ur);
+    if (guard) { bl = fmax(bl, 0.0); br = fmax(br, 0.0); }
+    ps = 0.5 * (pl * pow(bl, (1.0 / zl)) + pr * pow(br, (1.0 / zr)));
+}
+// riemann_solver.h:273-306
+__device__ inline void tsrs(const double* Wl, const double* Wr, double& ps) {
+    const double rl = Wl[0], ul = Wl[1], pl = Wl[2], gl = Wl[3];
+    const double rr = Wr[0], ur = Wr[1], pr = Wr[2], gr = Wr[3];
+    const double gml = (gl - 1.0) / (gl + 1.0), gmr = (gr - 1.0) / (gr + 1.0);
+    const double Al = 2.0 / (gl + 1.0) / rl, Bl = gml * pl;
+    const double Ar = 2.0 / (gr + 1.0) / rr, Br = gmr * pr;
+    const double p0 = fmax(0.0, ps);
+    const double gl0 = sqrt(Al / (p0 + Bl)), gr0 = sqrt(Ar / (p0 + Br));
+    ps = (gl0 * pl + gr0 * pr - (ur - ul)) / (gl0 + gr0);
+}
+// riemann_solver.h:308-329
+__device__ inline double anrs(const double* Wl, const double* Wr, int guard) {
+    const double pl = Wl[2], pr = Wr[2];
+    const double pmax = fmax(pl, pr), pmin = fmin(pl, pr);
+    const double qmax = pmax / pmin, quser = 2.0;
+    double ps;
+    pvrs(Wl, Wr, ps);
+    if ((qmax < quser) && (pmin <= ps) && (ps <= pmax)) {
+    } else if (ps < pmin) {
+        trrs(Wl, Wr, ps, guard);
+    } else {
+        tsrs(Wl, Wr, ps);
+    }
+    return ps;
+}
+
+struct LR {
+    double uln, urn, al, ar, Ul[4], Ur[4], Fl[4], Fr[4];
+};
+
+// riemann_solver.h:338-370
+__device__ inline void prelim(LR& s, double n0, double n1, double rl, const double* ul, double pl, double gl,
+                              double hl, double rr, const double* ur, double pr, double gr, double hr) {
+    s.uln = ul[0] * n0 + ul[1] * n1;
+    s.urn = ur[0] * n0 + ur[1] * n1;
+    const double ul2 = ul[0] * ul[0] + ul[1] * ul[1], ur2 = ur[0] * ur[0] + ur[1] * ur[1];
+    s.al = sqrt(gl * pl / rl);
+    s.ar = sqrt(gr * pr / rr);
+    const double El = (hl + 0.5 * ul2) * rl - pl, Er = (hr + 0.5 * ur2) * rr - pr;
+    s.Ul[0] = rl; s.Ul[1] = rl * ul[0]; s.Ul[2] = rl * ul[1]; s.Ul[3] = El;
+    s.Ur[0] = rr; s.Ur[1] = rr * ur[0]; s.Ur[2] = rr * ur[1]; s.Ur[3] = Er;
+    s.Fl[0] = rl * s.uln;
+    s.Fl[1] = rl * ul[0] * s.uln + pl * n0;
+    s.Fl[2] = rl * ul[1] * s.uln + pl * n1;
+    s.Fl[3] = (El + pl) * s.uln;
+    s.Fr[0] = rr * s.urn;
+    s.Fr[1] = rr * ur[0] * s.urn + pr * n0;
+    s.Fr[2] = rr * ur[1] * s.urn + pr * n1;
+    s.Fr[3] = (Er + pr) * s.urn;
+}
+
+__device__ inline void riemann_flux(int type, int guard, double* F, double n0, double n1, double rl,
+                                    const double* ul, double pl, double gl, double hl, double rr,
+                                    const double* ur, double pr, double gr, double hr) {
+    LR s;
+    prelim(s, n0, n1, rl, ul, pl, gl, hl, rr, ur, pr, gr, hr);
+    if (type == 0) {  // Rusanov, riemann_solver.h:331-375
+        const double Smax = fmax(fabs(s.uln) + s.al, fabs(s.urn) + s.ar);
+        for (int i = 0; i < 4; i++) F[i] = 0.5 * (s.Fl[i] + s.Fr[i] + Smax * (s.Ul[i] - s.Ur[i]));
+        return;
+    }
+    if (type == 3) {  // Roe with Harten entropy fix (textbook; no reference counterpart)
+        const double g = 0.5 * (gl + gr);
+        const double Hl = hl + 0.5 * (ul[0] * ul[0] + ul[1] * ul[1]);
+        const double Hr = hr + 0.5 * (ur[0] * ur[0] + ur[1] * ur[1]);
+        const double sl = sqrt(rl), sr = sqrt(rr), isum = 1.0 / (sl + sr);
+        const double u = (sl * ul[0] + sr * ur[0]) * isum;
+        const double v = (sl * ul[1] + sr * ur[1]) * isum;
+        const double H = (sl * Hl + sr * Hr) * isum;
+        const double q2 = u * u + v * v;
+        const double a2 = (g - 1.0) * (H - 0.5 * q2);
+        const double a = sqrt(fmax(a2, 1e-300));
+        const double un = u * n0 + v * n1;
+        const double ut = -u * n1 + v * n0;
+        const double rho = sl * sr;
+        const double dr = rr - rl, dp = pr - pl;
+        const double dun = s.urn - s.uln;
+        const double dut = (-ur[0] * n1 + ur[1] * n0) - (-ul[0] * n1 + ul[1] * n0);
+        const double al1 = (dp - rho * a * dun) / (2.0 * a2);
+        const double al2 = dr - dp / a2;
+        const double al3 = rho * dut;
+        const double al4 = (dp + rho * a * dun) / (2.0 * a2);
+        const double d = 0.1 * a;
+        auto fix = [&](double lam) {
+            const double am = fabs(lam);
+            return am < d ? (lam * lam + d * d) / (2.0 * d) : am;
+        };
+        const double l1 = fix(un - a), l2 = fabs(un), l3 = fabs(un), l4 = fix(un + a);
+        const double r1[4] = {1.0, u - a * n0, v - a * n1, H - a * un};
+        const double r2[4] = {1.0, u, v, 0.5 * q2};
+        const double r3[4] = {0.0, -n1, n0, ut};
+        const double r4[4] = {1.0, u + a * n0, v + a * n1, H + a * un};
+        for (int i = 0; i < 4; i++)
+            F[i] = 0.5 * (s.Fl[i] + s.Fr[i]) -
+                   0.5 * (l1 * al1 * r1[i] + l2 * al2 * r2[i] + l3 * al3 * r3[i] + l4 * al4 * r4[i]);
+        return;
+    }
+    const double Wl[4] = {rl, s.uln, pl, gl}, Wr[4] = {rr, s.urn, pr, gr};
+    const double ps = anrs(Wl, Wr, guard);
+    const double ql = (ps <= pl) ? 1.0 : sqrt(1.0 + (gl + 1.0) / (2.0 * gl) * (ps / pl - 1.0));
+    const double qr = (ps <= pr) ? 1.0 : sqrt(1.0 + (gr + 1.0) / (2.0 * gr) * (ps / pr - 1.0));
+    const double Sl = s.uln - s.al * ql, Sr = s.urn + s.ar * qr;
+    if (type == 1) {  // HLL, riemann_solver.h:377-439
+        if (0.0 <= Sl) for (int i = 0; i < 4; i++) F[i] = s.Fl[i];
+        else if (Sr <= 0.0) for (int i = 0; i < 4; i++) F[i] = s.Fr[i];
+        else for (int i = 0; i < 4; i++)
+            F[i] = (Sr * s.Fl[i] - Sl * s.Fr[i] + Sl * Sr * (s.Ur[i] - s.Ul[i])) / (Sr - Sl);
+        return;
+    }
+    // HLLC "variant 2", riemann_solver.h:441-519
+    const double Ss = (pr - pl + rl * s.uln * (Sl - s.uln) - rr * s.urn * (Sr - s.urn)) /
+                      (rl * (Sl - s.uln) - rr * (Sr - s.urn));
+    if (0.0 <= Sl) {
+        for (int i = 0; i < 4; i++) F[i] = s.Fl[i];
+    } else if (Sr <= 0.0) {
+        for (int i = 0; i < 4; i++) F[i] = s.Fr[i];
+    } else {
+        const double Ds[4] = {0.0, n0, n1, Ss};
+        const double Plr = 0.5 * (pl + pr + rl * (Sl - s.uln) * (Ss - s.uln) + rr * (Sr - s.urn) * (Ss - s.urn));
+        if (Ss >= 0.0) {
+            for (int i = 0; i < 4; i++) F[i] = (Ss * (Sl * s.Ul[i] - s.Fl[i]) + Sl * Plr * Ds[i]) / (Sl - Ss);
+        } else {
+            for (int i = 0; i < 4; i++) F[i] = (Ss * (Sr * s.Ur[i] - s.Fr[i]) + Sr * Plr * Ds[i]) / (Sr - Ss);
+        }
+    }
+}
+
+// Flux through one face point. Boundary states follow boundary_*.cpp; the
+// wall returns the pressure-only flux (boundary_wall_adiabatic.cpp:38-70).
+__device__ inline void point_flux(const Physics& ph, const BCParam* __restrict__ bct, int riemann, int guard,
+                                  bool bnd, int bc, double n0,
+                                  double n1, const double* Cl, const double* Cr, double* F) {
+    double Wl[5], Wr[5];
+    double rr;
+    prims(ph, Cl, Wl);
+    if (!bnd) {
+        prims(ph, Cr, Wr);
+        rr = Cr[0];
+    } else {
+        const BCParam& b = bct[bc];
+        if (b.type == 0) {  // symmetry: mirror the velocity (boundary_symmetry.cpp:38-69)
+            for (int i = 0; i < 5; i++) Wr[i] = Wl[i];
+            const double un = Wl[0] * n0 + Wl[1] * n1;
+            Wr[0] = Wl[0] - 2.0 * un * n0;
+            Wr[1] = Wl[1] - 2.0 * un * n1;
+            rr = Cl[0];
+        } else if (b.type == 1) {  // extrapolation
+            for (int i = 0; i < 5; i++) Wr[i] = Wl[i];
+            rr = Cl[0];
+        } else if (b.type == 2) {  // adiabatic slip wall
+            F[0] = 0.0; F[1] = Wl[2] * n0; F[2] = Wl[2] * n1; F[3] = 0.0;
+            return;
+        } else if (b.type == 3) {  // fixed u, p, T (boundary_upt.cpp:83-100)
+            rr = b.rho;
+            Wr[0] = b.u0; Wr[1] = b.u1; Wr[2] = b.p; Wr[3] = b.T; Wr[4] = b.h;
+        } else {  // p_out (boundary_p_out.cpp:57-100)
+            const double um = sqrt(Wl[0] * Wl[0] + Wl[1] * Wl[1]);
+            const double a = sqrt(ph.gamma * Wl[2] / Cl[0]);
+            const double pout = (um < a) ? b.p : Wl[2];
+            const double Tb = Wl[3];
+            const double rb = pout / (ph.R * Tb);
+            const double eb = ph.cv * Tb;
+            const double hb = eb + pout / rb;
+            rr = rb;
+            Wr[0] = Wl[0]; Wr[1] = Wl[1]; Wr[2] = pout; Wr[3] = Tb; Wr[4] = hb;
+        }
+    }
+    riemann_flux(riemann, guard, F, n0, n1, Cl[0], Wl, Wl[2], ph.gamma, Wl[4], rr, Wr, Wr[2], ph.gamma, Wr[4]);
+}
+
+__device__ inline double pressure_raw(double gamma, const double* u) {
+    return (gamma - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0]);
+}
+
+// ------------------------------------------------------------ TENO kernel
+// 16 cells per 64-lane wave. Phase A: lane s of a quad owns stencil s
+// (coalesced stencil-major loads, a = A+ b, smoothness, raw weight).
+// Phase B: after an LDS exchange lane v owns conserved variable v (cut-off,
+// weights, face-point sums in the reference's stencil-then-dof order).
+template <int KP>
+__global__ __launch_bounds__(64) void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
+                                                  double* __restrict__ fs, uint8_t* __restrict__ dec,
+                                                  Physics ph) {
+    __shared__ double sh_a[16][4][KP][4];
+    __shared__ double sh_w[16][4][4];
+    __shared__ double sh_f[16][8][4];
+    __shared__ uint32_t sh_d[16][4];
+    const int lc = threadIdx.x >> 2, s = threadIdx.x & 3;
+    const uint32_t N = g.N;
+    const uint32_t c = blockIdx.x * 16 + lc;
+    const bool valid = c < N;
+    const uint32_t cc = valid ? c : 0;
+    const int Mp = t.M - 1;
+    const uint32_t info0 = g.info[cc];
+    const bool active = valid && ((info0 >> (20 + s)) & 1u);
+    const D4 uc = ld4(U, cc);
+
+    double a[KP][4];
+#pragma unroll
+    for (int k = 0; k < KP; ++k)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) a[k][v] = 0.0;
+    double w[4] = {0.0, 0.0, 0.0, 0.0};
+    if (active) {
+        const size_t sN = (size_t)s * Mp;
+        for (int i = 0; i < Mp; ++i) {
+            const size_t o = (sN + i) * N + c;
+            const int32_t id = t.ids[o];
+            const double tt = t.ta[o];
+            const D4 un = ld4(U, (size_t)id);
+            double b[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b[v] = tt * (un.v[v] - uc.v[v]);
+            const double* Ai = t.A + ((sN + i) * KP) * N + c;
+#pragma unroll
+            for (int k = 0; k < KP; ++k) {
+                const double Ak = Ai[(size_t)k * N];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) a[k][v] = a[k][v] + Ak * b[v];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double si = 0.0;
+#pragma unroll
+            for (int k = 0; k < KP; ++k) {
+                double oia = 0.0;
+#pragma unroll
+                for (int j = 0; j < KP; ++j) oia = oia + t.OI[k * KP + j] * a[j][v];
+                si = si + a[k][v] * oia;
+            }
+            w[v] = 1.0 / pow6_cr(si + 1.0e-12);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        sh_w[lc][s][v] = w[v];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) sh_a[lc][s][k][v] = a[k][v];
+    }
+    __syncthreads();
+
+    // ---- phase B: lane v = s
+    const int v = s;
+    const int G = 4;
+    double ws[4];
+#pragma unroll
+    for (int st = 0; st < G; ++st) ws[st] = sh_w[lc][st][v];
+    const double CT1 = 1.0e-7, CT2 = 1.0e-5;
+    double sd = 0.0;
+#pragma unroll
+    for (int st = 1; st < G; ++st) sd += ws[st];
+    uint32_t central = 0;
+    if (ws[0] / (sd + ws[0]) > CT1) {
+        central = 1;
+        ws[0] = 1.0;
+#pragma unroll
+        for (int st = 1; st < G; ++st) ws[st] = 0.0;
+    } else {
+        if (t.variant != 0) ws[0] = 0.0;
+#pragma unroll
+        for (int st = 1; st < G; ++st) {
+            if (ws[st] / sd > CT2) ws[st] = (1.0 / t.K);
+            else if (t.variant != 0) ws[st] = 0.0;
+        }
+        double s2 = 0.0;
+#pragma unroll
+        for (int st = 1; st < G; ++st) s2 += ws[st];
+#pragma unroll
+        for (int st = 1; st < G; ++st) ws[st] /= s2;
+    }
+    double pw[4][KP];
+#pragma unroll
+    for (int st = 0; st < G; ++st)
+#pragma unroll
+        for (int k = 0; k < KP; ++k) pw[st][k] = ws[st] * sh_a[lc][st][k][v];
+
+    const uint32_t S = g.S, Q = g.Q;
+    const double ucv = uc.v[v];
+    for (uint32_t l = 0; l < S; ++l)
+        for (uint32_t q = 0; q < Q; ++q) {
+            const double* B = t.fbasis + ((size_t)(l * Q + q) * KP) * N + cc;
+            double bk[KP];
+#pragma unroll
+            for (int k = 0; k < KP; ++k) bk[k] = B[(size_t)k * N];
+            double acc = ucv;
+#pragma unroll
+            for (int st = 0; st < G; ++st) {
+                if (ws[st] == 0.0) continue;
+                acc = acc + (ws[st] * 0.0) * t.basis0;
+#pragma unroll
+                for (int k = 0; k < KP; ++k) acc = acc + pw[st][k] * bk[k];
+            }
+            sh_f[lc][l * Q + q][v] = acc;
+        }
+    sh_d[lc][v] = central;
+    __syncthreads();
+
+    if (t.variant != 0) {
+        // positivity scaling of the cell's face states (Zhang & Shu 2010),
+        // computed redundantly by the 4 lanes, same order as the oracle
+        double Uc[4] = {uc.v[0], uc.v[1], uc.v[2], uc.v[3]};
+        const double pc = pressure_raw(ph.gamma, Uc);
+        const double eps = fmin(1.0e-13, fmin(Uc[0], pc));
+        double theta = 1.0;
+        for (uint32_t fq = 0; fq < S * Q; ++fq) {
+            double u[4], wv[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) u[k] = sh_f[lc][fq][k];
+            double tt = 1.0;
+            if (u[0] < eps) tt = (Uc[0] - eps) / (Uc[0] - u[0]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) wv[k] = Uc[k] + tt * (u[k] - Uc[k]);
+            if (pressure_raw(ph.gamma, wv) < eps) {
+                double lo = 0.0, hi = tt;
+                for (int it = 0; it < 40; it++) {
+                    const double mid = 0.5 * (lo + hi);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) wv[k] = Uc[k] + mid * (u[k] - Uc[k]);
+                    if (pressure_raw(ph.gamma, wv) < eps) hi = mid;
+                    else lo = mid;
+                }
+                tt = lo;
+            }
+            theta = fmin(theta, tt);
+        }
+        if (theta < 1.0)
+            for (uint32_t fq = 0; fq < S * Q; ++fq) sh_f[lc][fq][v] = Uc[v] + theta * (sh_f[lc][fq][v] - Uc[v]);
+    }
+    if (valid) {
+        for (uint32_t fq = 0; fq < S * Q; ++fq) fs[((size_t)fq * N + c) * 4 + v] = sh_f[lc][fq][v];
+        if (dec && v == 0)
+            dec[c] = (uint8_t)(sh_d[lc][0] | (sh_d[lc][1] << 1) | (sh_d[lc][2] << 2) | (sh_d[lc][3] << 3));
+    }
+}
+
+// ------------------------------------------------------------ flux + update
+// One thread per cell: the residual of the cell in the canonical face order,
+// divided by the volume, then the stage's SSP-RK update (no atomics).
+__global__ __launch_bounds__(256) void flux_update_kernel(Geom g, StageArgs a, Physics ph) {
+    const uint32_t N = g.N;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const uint32_t Q = g.Q;
+    const uint32_t info0 = g.info[c];
+    const uint32_t code = (info0 >> 8) & 0xFFu, nact = (info0 >> 16) & 0x7u;
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t rk = 0; rk < nact; ++rk) {
+        const uint32_t l = (code >> (2 * rk)) & 3u;
+        const size_t o = (size_t)l * N + c;
+        const uint32_t inf = g.info[o];
+        const bool bnd = (inf & kBoundaryBit) != 0;
+        const uint32_t side = (inf >> 2) & 1u;
+        const double n0 = g.nx[o], n1 = g.ny[o], ar = g.area[o];
+        double F[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t q = 0; q < Q; ++q) {
+            D4 own, oth;
+            if (a.fo) {
+                own = ld4(a.Uin, c);
+            } else {
+                own = ld4(a.fs, (size_t)(l * Q + q) * N + c);
+            }
+            double Ft[4];
+            if (!bnd) {
+                const uint32_t nbc = (uint32_t)g.nb[o];
+                const uint32_t nbs = inf & 3u;
+                oth = a.fo ? ld4(a.Uin, nbc) : ld4(a.fs, (size_t)(nbs * Q + q) * N + nbc);
+                const double* L = side ? oth.v : own.v;
+                const double* R = side ? own.v : oth.v;
+                point_flux(ph, g.bc, a.riemann, a.vacuum_guard, false, 0, n0, n1, L, R, Ft);
+            } else {
+                point_flux(ph, g.bc, a.riemann, a.vacuum_guard, true, (inf >> 4) & 0xFu, n0, n1, own.v, own.v, Ft);
+            }
+            const double wq = g.qw[q];
+#pragma unroll
+            for (int i = 0; i < 4; i++) F[i] += wq * Ft[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) F[i] *= 0.5;
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] += side ? ar * F[i] : -ar * F[i];
+    }
+    const double vol = g.volume[c];
+#pragma unroll
+    for (int i = 0; i < 4; i++) r[i] /= vol;
+
+    if (a.op == kRhsOnly) {
+        st4(a.rhs, c, D4{{r[0], r[1], r[2], r[3]}});
+        return;
+    }
+    const double dt = *a.dt;
+    const D4 un = ld4(a.Un, c);
+    D4 out, wa;
+    bool final_stage = false;
+    switch (a.op) {
+    case kFE:
+        for (int i = 0; i < 4; i++) out.v[i] = un.v[i] + dt * r[i];
+        st4(a.Uout, c, out);
+        final_stage = true;
+        break;
+    case kSSP1: {
+        const double c0 = dt * (1.0 / 6.0);
+        for (int i = 0; i < 4; i++) { out.v[i] = un.v[i] + dt * r[i]; wa.v[i] = un.v[i] + c0 * r[i]; }
+        st4(a.Uout, c, out);
+        st4(a.W, c, wa);
+        break;
+    }
+    case kSSP2: {
+        const D4 u1 = ld4(a.Uin, c);
+        const D4 w0 = ld4(a.W, c);
+        const double c1 = dt * (1.0 / 6.0), d4 = dt / 4.0;
+        for (int i = 0; i < 4; i++) {
+            const double y = 3.0 / 4.0 * un.v[i] + 1.0 / 4.0 * u1.v[i];
+            out.v[i] = y + d4 * r[i];
+            wa.v[i] = w0.v[i] + c1 * r[i];
+        }
+        st4(a.Uout, c, out);
+        st4(a.W, c, wa);
+        break;
+    }
+    case kSSP3: {
+        const D4 w0 = ld4(a.W, c);
+        const double c2 = dt * (2.0 / 3.0);
+        for (int i = 0; i < 4; i++) out.v[i] = w0.v[i] + c2 * r[i];
+        st4(a.Uout, c, out);
+        final_stage = true;
+        break;
+    }
+    case kRK4_1: case kRK4_2: case kRK4_3: {
+        const double half = dt / 2.0;
+        const double cs = a.op == kRK4_1 ? dt * (1.0 / 6.0) : dt * (1.0 / 3.0);
+        const double step = a.op == kRK4_3 ? dt : half;
+        D4 w0 = un;
+        if (a.op != kRK4_1) w0 = ld4(a.W, c);
+        for (int i = 0; i < 4; i++) { out.v[i] = un.v[i] + step * r[i]; wa.v[i] = w0.v[i] + cs * r[i]; }
+        st4(a.Uout, c, out);
+        st4(a.W, c, wa);
+        break;
+    }
+    case kRK4_4: {
+        const D4 w0 = ld4(a.W, c);
+        const double c3 = dt * (1.0 / 6.0);
+        for (int i = 0; i < 4; i++) out.v[i] = w0.v[i] + c3 * r[i];
+        st4(a.Uout, c, out);
+        final_stage = true;
+        break;
+    }
+    default: break;
+    }
+    if (final_stage && a.prims) {
+        double W5[5];
+        prims(ph, out.v, W5);
+        st4(a.prims, c, D4{{W5[0], W5[1], W5[2], W5[3]}});
+        a.h[c] = W5[4];
+    }
+}
+
+// ------------------------------------------------------------ dt
+// solver.cpp:592-706: per-cell spectral radius over the cell's faces in
+// local order, max over the grid through a block max and one atomic.
+__global__ __launch_bounds__(256) void spectral_kernel(Geom g, const double* __restrict__ Un,
+                                                       const double* __restrict__ pr, Physics ph,
+                                                       double* __restrict__ lam_cell,
+                                                       unsigned long long* lam_max) {
+    const uint32_t N = g.N;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    double lam = 0.0;
+    bool ok = false;
+    if (c < N) {
+        const D4 wc = ld4(pr, c);
+        const double rc = Un[4 * (size_t)c];
+        const double sc = sqrt(ph.gamma * wc.v[2] / rc);
+        double conv = 0.0, acou = 0.0;
+        const uint32_t S = g.S;
+        for (uint32_t l = 0; l < S; ++l) {
+            const size_t o = (size_t)l * N + c;
+            const uint32_t nbc = (uint32_t)g.nb[o];
+            const double n0 = g.nx[o], n1 = g.ny[o], dxn = g.dxn[o];
+            const D4 wn = ld4(pr, nbc);
+            const double rn = Un[4 * (size_t)nbc];
+            const double sn = sqrt(ph.gamma * wn.v[2] / rn);
+            const double sf = 0.5 * (sc + sn);
+            const double uf0 = 0.5 * (wc.v[0] + wn.v[0]), uf1 = 0.5 * (wc.v[1] + wn.v[1]);
+            const double un = fabs(uf0 * n0 + uf1 * n1);
+            conv += un / dxn;
+            const double qq = sf / dxn;
+            acou += qq * qq;
+        }
+        const double gf = 3.0 / S;
+        conv *= 1.37 * gf;
+        acou = 1.37 * sqrt(gf * acou);
+        lam = conv + acou;
+        lam_cell[c] = lam;
+        ok = lam == lam;  // NaN radii are ignored by the reference's max join
+    }
+    // non-negative doubles order like their bit patterns
+    unsigned long long bits = ok ? (unsigned long long)__double_as_longlong(lam) : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o2 = __shfl_xor(bits, off);
+        bits = o2 > bits ? o2 : bits;
+    }
+    __shared__ unsigned long long sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = bits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = sh[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = sh[i] > m ? sh[i] : m;
+        atomicMax(lam_max, m);
+    }
+}
+
+__global__ void finalize_dt_kernel(const unsigned long long* lam_max_in, double cfl, double fixed_dt,
+                                   double* dt, double* dt_hist, int64_t hist_cap, int64_t* step,
+                                   unsigned long long* lam_max_reset) {
+    double d;
+    if (fixed_dt > 0.0) {
+        d = fixed_dt;
+    } else {
+        const double lm = __longlong_as_double((long long)*lam_max_in);
+        d = cfl / lm;
+    }
+    *dt = d;
+    if (dt_hist && step) dt_hist[*step % hist_cap] = d;
+    if (step) *step += 1;
+    if (lam_max_reset) *lam_max_reset = 0ull;
+}
+
+__global__ void copy_kernel(const double* __restrict__ s, double* __restrict__ d, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+// Device math probes for the parity tests (sqrt, division, pow, pow6).
+__global__ void probe_kernel(const double* x, const double* y, double* out, int n, int which) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    switch (which) {
+    case 0: out[i] = sqrt(x[i]); break;
+    case 1: out[i] = x[i] / y[i]; break;
+    case 2: out[i] = pow(x[i], y[i]); break;
+    case 3: out[i] = pow6_cr(x[i]); break;
+    default: out[i] = 0.0;
+    }
+}
+
+}  // namespace
+
+hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs, uint8_t* dec,
+                      const Physics& ph, hipStream_t stream) {
+    const int Kp = t.K - 1;
+    if (Kp > kMaxKp || g.S != 3 || g.S * g.Q > 8) return hipErrorInvalidValue;
+    const dim3 grid((g.N + 15) / 16), block(64);
+    switch (Kp) {
+    case 2: hipLaunchKernelGGL(teno_kernel<2>, grid, block, 0, stream, g, t, U, fs, dec, ph); break;
+    case 5: hipLaunchKernelGGL(teno_kernel<5>, grid, block, 0, stream, g, t, U, fs, dec, ph); break;
+    case 9: hipLaunchKernelGGL(teno_kernel<9>, grid, block, 0, stream, g, t, U, fs, dec, ph); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, hipStream_t stream) {
+    const dim3 grid((g.N + 255) / 256), block(256);
+    hipLaunchKernelGGL(flux_update_kernel, grid, block, 0, stream, g, a, ph);
+    return hipGetLastError();
+}
+
+hipError_t spectral_radius(const Geom& g, const double* Un, const double* prims, const Physics& ph,
+                           double* lam_cell, unsigned long long* lam_max, hipStream_t stream) {
+    const dim3 grid((g.N + 255) / 256), block(256);
+    hipLaunchKernelGGL(spectral_kernel, grid, block, 0, stream, g, Un, prims, ph, lam_cell, lam_max);
+    return hipGetLastError();
+}
+
+hipError_t finalize_dt(const unsigned long long* lam_max_in, double cfl, double fixed_dt, double* dt,
+                       double* dt_hist, int64_t hist_cap, int64_t* step, unsigned long long* lam_max_reset,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1), 0, stream, lam_max_in, cfl, fixed_dt, dt, dt_hist,
+                       hist_cap, step, lam_max_reset);
+    return hipGetLastError();
+}
+
+hipError_t copy_state(const double* src, double* dst, uint32_t N, hipStream_t stream) {
+    const size_t n = 4 * (size_t)N;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t probe_math(const double* x, const double* y, double* out, int n, int which, hipStream_t stream) {
+    hipLaunchKernelGGL(probe_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, x, y, out, n, which);
+    return hipGetLastError();
+}
+
+}  // namespace mlr::dev

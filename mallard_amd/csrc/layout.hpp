@@ -1,0 +1,94 @@
+// Device data layout: cells renumbered along a Hilbert curve, structure-of-
+// arrays face-slot geometry, and the TENO stencil data in stencil-major,
+// cell-minor order so that consecutive lanes read consecutive addresses.
+//
+// Numbering: "old" = reference numbering (mesh.cpp), "new" = SFC order.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "mesh.hpp"
+#include "numerics.hpp"
+
+namespace mlr {
+
+struct BoundarySpec {
+    std::string zone;
+    BCType type = BCType::Symmetry;
+    double u[2] = {0.0, 0.0};
+    double p = 0.0, T = 0.0;
+};
+
+struct TenoSpec {
+    BasisType basis = BasisType::Monomial;  // face_reconstruction.cpp:110 default
+    int order = 1;
+    double factor = 2.0;
+    int quad_order_cell = -1;  // <0: reference default order + 1
+    int quad_order_face = -1;  // <0: reference default (order + 1) / 2
+    TenoVariant variant = TenoVariant::Reference;
+};
+
+// Slot info word (one per cell face slot):
+//   bits 0-1  neighbour's slot of the shared face
+//   bit  2    side: 1 when this cell is cells_of_face[f][1]
+//   bit  3    boundary face
+//   bits 4-7  boundary condition index (15: zone without a condition)
+//   bits 8-15 (slot 0 only) accumulation order: rank r -> slot (2 bits each)
+//   bits 16-18 (slot 0 only) number of slots that contribute to the residual
+//   bits 20-23 (slot 0 only) bitmask of non-empty TENO stencils
+constexpr uint32_t kSlotBoundary = 1u << 3;
+constexpr uint32_t kNoBC = 15u;
+
+struct Layout {
+    uint32_t N = 0;        // cells
+    uint32_t S = 0;        // face slots per cell (3 triangles, 4 quads)
+    uint32_t Q = 1;        // face quadrature points
+    std::vector<double> qface_w;  // face quadrature weights
+    std::vector<uint32_t> perm;   // new -> old
+    std::vector<uint32_t> iperm;  // old -> new
+    std::vector<uint32_t> face_slot;  // old face id -> (new cell << 2 | slot) of side 0, or ~0u
+
+    std::vector<double> volume;           // [N]
+    std::vector<double> centroid;         // [N][2] (new numbering, for init/output)
+    std::vector<int32_t> nb;              // [S][N] neighbour (new id), self for boundary slots
+    std::vector<uint32_t> info;           // [S][N]
+    std::vector<double> nx, ny, area, dxn;  // [S][N]
+
+    // TENO (empty for FO): G = S + 1 stencils per cell, K = n_dof, M = cells per stencil.
+    bool teno = false;
+    int K = 0, M = 0, G = 0;
+    TenoSpec tspec;
+    std::vector<uint8_t> poly;            // [K][2]
+    std::vector<double> ipt;              // [K]
+    std::vector<double> OI;               // [K][K]
+    std::vector<int32_t> st_ids;          // [G][M-1][N] neighbours in new numbering
+    std::vector<double> st_ta;            // [G][M-1][N]
+    std::vector<double> st_A;             // [G][M-1][K-1][N] A+ without its zero row/column
+    std::vector<double> fbasis;           // [S][Q][K-1][N] psi_k(x_q) + ipt_k/ta_0 (or - ipt_k)
+    double basis0 = 1.0;                  // sign carrier of the k = 0 term
+
+    uint64_t device_bytes() const;
+};
+
+// Hilbert-curve order of cell centroids (ties broken by reference id).
+std::vector<uint32_t> sfc_order(const Mesh& m);
+
+// Builds the layout. bcs lists the boundary conditions in input order.
+Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts,
+                    const std::vector<BoundarySpec>& bcs, int n_threads = 0);
+
+// Reference-numbered stencil lists recomputed for one cell (used by the
+// index-parity tests through the C-ABI): out = G stencils of M ids each (or
+// empty), in reference numbering.
+void stencils_of_cell(const Mesh& m, int M, uint32_t c, std::vector<std::vector<uint32_t>>& out);
+
+// Full K x M pseudo-inverses of a cell's stencils in reference order
+// (K*M doubles per non-empty stencil), for parity tests.
+void teno_matrices_of_cell(const Mesh& m, const TenoSpec& ts, uint32_t c,
+                           std::vector<std::vector<double>>& aplus,
+                           std::vector<std::vector<double>>& ta, std::vector<double>& ipt_out);
+
+}  // namespace mlr

@@ -1,0 +1,691 @@
+#include "solver.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <iostream>
+#include <sstream>
+
+#include "expr.hpp"
+
+namespace mlr {
+
+namespace {
+
+#define HIP_OK(x)                                                                          \
+    do {                                                                                   \
+        const hipError_t e_ = (x);                                                         \
+        if (e_ != hipSuccess) fail(std::string("HIP error: ") + hipGetErrorString(e_) +    \
+                                   " at " + __FILE__ + ":" + std::to_string(__LINE__));    \
+    } while (0)
+
+double wall_seconds() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+T lookup_enum(const std::string& key, std::initializer_list<std::pair<const char*, T>> table,
+              const std::string& what) {
+    for (auto& kv : table)
+        if (key == kv.first) return kv.second;
+    fail("Unknown " + what + ": " + key + ".");
+}
+
+// physics.h:852-867 on the host (same operation order as the device)
+void host_prims(const dev::Physics& ph, const double* U, double* W) {
+    const double rho = U[0];
+    const double u0 = U[1] / rho, u1 = U[2] / rho;
+    const double E = U[3] / rho;
+    const double e = E - 0.5 * (u0 * u0 + u1 * u1);
+    const double p = std::fmax(ph.p_min, std::fmin(ph.p_max, (ph.gamma - 1.0) * rho * e));
+    W[0] = u0; W[1] = u1; W[2] = p; W[3] = e / ph.cv; W[4] = e + p / rho;
+}
+
+constexpr int64_t kDtHistCap = 1 << 20;
+
+}  // namespace
+
+// ------------------------------------------------------------ input
+Config config_from_toml(const toml::Value& in) {
+    using namespace toml;
+    Config c;
+    // [mesh] (mesh.cpp:32-66; jitter/seed are this engine's extension keys)
+    const std::string mt = find_or_str(in, {"mesh", "type"}, "file");
+    c.mesh_type = lookup_enum<MeshType>(mt, {{"file", MeshType::File}, {"cartesian", MeshType::Cartesian},
+                                             {"cartesian_tri", MeshType::CartesianTri}, {"wedge", MeshType::Wedge}},
+                                        "mesh type");
+    if (c.mesh_type == MeshType::File) fail("MeshType::FILE not implemented.");
+    c.nx = (uint32_t)find_or_int(in, {"mesh", "Nx"}, 100);
+    c.ny = (uint32_t)find_or_int(in, {"mesh", "Ny"}, 100);
+    c.Lx = find_or(in, {"mesh", "Lx"}, 1.0);
+    c.Ly = find_or(in, {"mesh", "Ly"}, 1.0);
+    c.jitter = find_or(in, {"mesh", "jitter"}, 0.0);
+    c.seed = (uint64_t)find_or_int(in, {"mesh", "seed"}, 0);
+
+    // [physics] (solver.cpp:88-108, physics.cpp:27-53)
+    const std::string pt = find_or_str(in, {"physics", "type"}, "euler");
+    if (pt != "euler") fail("Unknown physics type: " + pt + ".");
+    const Value& ph = find(in, "physics");
+    for (const char* k : {"gamma", "p_ref", "T_ref", "rho_ref"})
+        if (!ph.contains(k)) fail(std::string("Missing ") + k + " for physics: euler.");
+    c.gamma = find(ph, "gamma").as_float();
+    c.p_ref = find(ph, "p_ref").as_float();
+    c.T_ref = find(ph, "T_ref").as_float();
+    c.rho_ref = find(ph, "rho_ref").as_float();
+    c.p_min = find_or(in, {"physics", "p_min"}, -1e20);
+    c.p_max = find_or(in, {"physics", "p_max"}, 1e20);
+
+    // [numerics] (solver.cpp:110-186, face_reconstruction.cpp:109-152)
+    const Value& fr = find(in, "numerics", "face_reconstruction");
+    c.recon = lookup_enum<Recon>(find_or_str(fr, {"type"}, "FO"), {{"FO", Recon::FO}, {"TENO", Recon::TENO}},
+                                 "face reconstruction type");
+    c.riemann = lookup_enum<Riemann>(find_or_str(in, {"numerics", "riemann_solver"}, "HLLC"),
+                                     {{"Rusanov", Riemann::Rusanov}, {"HLL", Riemann::HLL},
+                                      {"HLLC", Riemann::HLLC}, {"Roe", Riemann::Roe}},
+                                     "Riemann solver type");
+    c.integrator = lookup_enum<Integrator>(find_or_str(in, {"numerics", "time_integrator"}, "LSSSPRK3"),
+                                           {{"FE", Integrator::FE}, {"RK4", Integrator::RK4},
+                                            {"SSPRK3", Integrator::SSPRK3}},
+                                           "time integrator type");
+    c.check_nan = find_or_bool(in, {"numerics", "check_nan"}, false);
+    if (c.recon == Recon::TENO) {
+        c.teno.basis = lookup_enum<BasisType>(find_or_str(fr, {"basis_type"}, "monomial"),
+                                              {{"monomial", BasisType::Monomial}, {"legendre", BasisType::Legendre}},
+                                              "basis type");
+        c.teno.order = (int)(uint8_t)find(fr, "basis_order").as_int();
+        c.teno.factor = find_or(fr, {"max_stencil_size_factor"}, 2.0);
+        const std::string qtc = find_or_str(fr, {"quadrature_type_cell"}, "triangle_dunavant");
+        const std::string qtf = find_or_str(fr, {"quadrature_type_face"}, "gauss_legendre");
+        if (qtc != "triangle_dunavant") fail("Unknown quadrature type: " + qtc);
+        if (qtf != "gauss_legendre") fail("Unknown quadrature type: " + qtf);
+        c.teno.quad_order_cell = (int)(uint8_t)find_or_int(fr, {"quadrature_order_cell"}, c.teno.order + 1);
+        c.teno.quad_order_face = (int)(uint8_t)find_or_int(fr, {"quadrature_order_face"}, (c.teno.order + 1) / 2);
+        c.teno.variant = lookup_enum<TenoVariant>(find_or_str(fr, {"teno_variant"}, "reference"),
+                                                  {{"reference", TenoVariant::Reference},
+                                                   {"corrected", TenoVariant::Corrected}},
+                                                  "TENO variant");
+    }
+
+    // [[boundaries]] (solver.cpp:188-237, boundary_upt.cpp:40-71, boundary_p_out.cpp:36-52)
+    const Value& bl = find(in, "boundaries");
+    if (bl.kind != Value::Array) fail("boundaries must be an array of tables");
+    for (const Value& b : bl.arr) {
+        if (!b.contains("name")) fail("Boundary name not specified.");
+        if (!b.contains("type")) fail("Boundary type not specified.");
+        BoundarySpec s;
+        s.zone = find(b, "name").as_string();
+        const std::string ty = find(b, "type").as_string();
+        s.type = lookup_enum<BCType>(ty, {{"symmetry", BCType::Symmetry}, {"extrapolation", BCType::Extrapolation},
+                                          {"wall_adiabatic", BCType::WallAdiabatic}, {"upt", BCType::UPT},
+                                          {"p_out", BCType::POut}},
+                                     "boundary type");
+        if (s.type == BCType::UPT) {
+            for (const char* k : {"u", "p", "T"})
+                if (!b.contains(k)) fail(std::string("Missing ") + k + " for boundary: " + s.zone + ".");
+            const Value& u = find(b, "u");
+            if (u.kind != Value::Array || u.arr.size() != 2) fail("Invalid u for boundary: " + s.zone + ".");
+            s.u[0] = u.arr[0].as_float();
+            s.u[1] = u.arr[1].as_float();
+            s.p = find(b, "p").as_float();
+            s.T = find(b, "T").as_float();
+        } else if (s.type == BCType::POut) {
+            if (!b.contains("p")) fail("Missing p for boundary: " + s.zone + ".");
+            s.p = find(b, "p").as_float();
+        }
+        c.bcs.push_back(s);
+    }
+
+    // [run] (solver.cpp:239-273)
+    const Value* run = in.get("run");
+    if (!run) fail("Run parameters not specified.");
+    const bool has_dt = run->contains("dt"), has_cfl = run->contains("cfl");
+    if (!has_dt && !has_cfl) fail("Either dt or cfl must be specified.");
+    if (has_dt && has_cfl) fail("Only one of dt or cfl can be specified.");
+    if (!run->contains("n_steps") && !run->contains("t_stop") && !run->contains("t_wall_stop"))
+        fail("Either n_steps, t_stop, or t_wall_stop must be specified.");
+    c.use_cfl = has_cfl;
+    if (has_dt) c.dt = find(*run, "dt").as_float();
+    else c.cfl = find(*run, "cfl").as_float();
+    c.n_steps = run->contains("n_steps") ? (int64_t)(uint32_t)find(*run, "n_steps").as_int() : (int64_t)UINT32_MAX;
+    c.t_stop = find_or(in, {"run", "t_stop"}, -1.0);
+    c.t_wall_stop = find_or(in, {"run", "t_wall_stop"}, -1.0);
+    c.check_interval = (uint32_t)find_or_int(in, {"output", "check_interval"}, 1);
+
+    // [initialize] (solver_initialize.cpp:35-186; "quadrant" is this engine's
+    // shorthand for the four-state Riemann problems)
+    const std::string it = find_or_str(in, {"initialize", "type"}, "constant");
+    const Value* ini = in.get("initialize");
+    if (it == "constant") {
+        c.init.kind = InitSpec::Constant;
+        if (!ini || !ini->contains("u")) fail("Missing u for initialization: constant.");
+        const Value& u = find(*ini, "u");
+        if (u.kind != Value::Array || u.arr.size() != 2) fail("u must be a 2-element array for initialization: constant.");
+        if (!ini->contains("p")) fail("Missing p for initialization: constant.");
+        if (!ini->contains("T")) fail("Missing T for initialization: constant.");
+        c.init.u[0] = u.arr[0].as_float();
+        c.init.u[1] = u.arr[1].as_float();
+        c.init.p = find(*ini, "p").as_float();
+        c.init.T = find(*ini, "T").as_float();
+    } else if (it == "analytical") {
+        c.init.kind = InitSpec::Analytical;
+        if (!ini || !ini->contains("u")) fail("Missing u for initialization: analytical.");
+        const Value& u = find(*ini, "u");
+        if (u.kind != Value::Array || u.arr.size() != 2) fail("u must be a 2-element array for initialization: analytical.");
+        const int n = (int)ini->contains("rho") + (int)ini->contains("p") + (int)ini->contains("T");
+        if (n != 2) fail("Exactly two of rho, p, and T must be specified for initialization: analytical.");
+        c.init.u_expr[0] = u.arr[0].as_string();
+        c.init.u_expr[1] = u.arr[1].as_string();
+        c.init.rho_expr = find_or_str(*ini, {"rho"}, "");
+        c.init.p_expr = find_or_str(*ini, {"p"}, "");
+        c.init.T_expr = find_or_str(*ini, {"T"}, "");
+    } else if (it == "quadrant") {
+        c.init.kind = InitSpec::Quadrant;
+        const Value& sp = find(*ini, "split");
+        c.init.xs = sp.arr.at(0).as_float();
+        c.init.ys = sp.arr.at(1).as_float();
+        const char* names[4] = {"UR", "UL", "LL", "LR"};
+        for (int q = 0; q < 4; ++q) {
+            const Value& st = find(*ini, names[q]);
+            if (st.kind != Value::Array || st.arr.size() != 4) fail(std::string("quadrant state ") + names[q] + " needs [rho, u, v, p]");
+            for (int k = 0; k < 4; ++k) c.init.states[4 * q + k] = st.arr[k].as_float();
+        }
+    } else {
+        fail("Unknown initialization type: " + it + ".");
+    }
+    return c;
+}
+
+// ------------------------------------------------------------ solver
+Solver::Solver() = default;
+
+Solver::~Solver() { free_device(); }
+
+void Solver::free_device() {
+    if (step_exec_) { (void)hipGraphExecDestroy(step_exec_); step_exec_ = nullptr; }
+    if (step_graph_) { (void)hipGraphDestroy(step_graph_); step_graph_ = nullptr; }
+    for (void* p : allocs_) (void)hipFree(p);
+    allocs_.clear();
+    if (stream_) { (void)hipStreamDestroy(stream_); stream_ = nullptr; }
+}
+
+int Solver::init(const std::string& input_file) {
+    const toml::Value in = toml::parse_file(input_file);
+    init(config_from_toml(in));
+    return 0;
+}
+
+void Solver::init(const Config& cfg) {
+    cfg_ = cfg;
+    Mesh m;
+    switch (cfg.mesh_type) {
+    case MeshType::Cartesian: m = make_cartesian(cfg.nx, cfg.ny, cfg.Lx, cfg.Ly, cfg.jitter, cfg.seed); break;
+    case MeshType::CartesianTri: m = make_cartesian_tri(cfg.nx, cfg.ny, cfg.Lx, cfg.Ly, cfg.jitter, cfg.seed); break;
+    case MeshType::Wedge: m = make_wedge(cfg.nx, cfg.ny, cfg.Lx, cfg.Ly); break;
+    default: fail("MeshType::FILE not implemented.");
+    }
+    setup(std::move(m));
+}
+
+void Solver::init(const Config& cfg, Mesh&& mesh) {
+    cfg_ = cfg;
+    setup(std::move(mesh));
+}
+
+void Solver::setup(Mesh&& mesh) {
+    free_device();
+    mesh_ = std::move(mesh);
+    // physics.cpp:69-73
+    ph_.gamma = cfg_.gamma;
+    ph_.R = cfg_.p_ref / (cfg_.T_ref * cfg_.rho_ref);
+    ph_.cp = ph_.R * cfg_.gamma / (cfg_.gamma - 1.0);
+    ph_.cv = ph_.cp / cfg_.gamma;
+    ph_.p_min = cfg_.p_min;
+    ph_.p_max = cfg_.p_max;
+    if (cfg_.integrator != Integrator::FE && cfg_.integrator != Integrator::RK4 &&
+        cfg_.integrator != Integrator::SSPRK3)
+        fail("Unknown time integrator type.");
+    L_ = build_layout(mesh_, cfg_.recon, cfg_.teno, cfg_.bcs, cfg_.n_threads);
+    HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    upload();
+    t_ = 0.0;
+    step_ = 0;
+    wall0_ = wall_seconds();
+    init_solution();
+}
+
+void Solver::upload() {
+    auto dalloc = [&](size_t bytes) {
+        void* p = nullptr;
+        HIP_OK(hipMalloc(&p, bytes ? bytes : 8));
+        allocs_.push_back(p);
+        return p;
+    };
+    auto up = [&](auto*& dst, const auto& v) {
+        using T = std::remove_reference_t<decltype(v[0])>;
+        dst = reinterpret_cast<std::remove_reference_t<decltype(dst)>>(dalloc(v.size() * sizeof(T)));
+        if (!v.empty()) HIP_OK(hipMemcpy(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    };
+    const size_t N = L_.N;
+    up(volume_, L_.volume);
+    up(nb_, L_.nb);
+    up(info_, L_.info);
+    up(nx_, L_.nx);
+    up(ny_, L_.ny);
+    up(area_, L_.area);
+    up(dxn_, L_.dxn);
+    up(qw_, L_.qface_w);
+    std::vector<dev::BCParam> bct(std::max<size_t>(cfg_.bcs.size(), 1));
+    for (size_t b = 0; b < cfg_.bcs.size(); ++b) {
+        const BoundarySpec& s = cfg_.bcs[b];
+        dev::BCParam& p = bct[b];
+        p.type = (int32_t)s.type;
+        p.u0 = s.u[0];
+        p.u1 = s.u[1];
+        p.p = s.p;
+        p.T = s.T;
+        if (s.type == BCType::UPT) {  // boundary_upt.cpp:58-60 (host EOS)
+            p.rho = s.p / (ph_.R * s.T);
+            p.e = ph_.cv * s.T;
+            p.h = p.e + s.p / p.rho;
+        }
+    }
+    up(bc_, bct);
+    if (L_.teno) {
+        up(ids_, L_.st_ids);
+        up(ta_, L_.st_ta);
+        up(A_, L_.st_A);
+        up(fbasis_, L_.fbasis);
+        // host copies of the big tables are not needed any more
+        std::vector<int32_t>().swap(L_.st_ids);
+        std::vector<double>().swap(L_.st_ta);
+        std::vector<double>().swap(L_.st_A);
+        std::vector<double>().swap(L_.fbasis);
+    }
+    auto dz = [&](size_t n) {
+        double* p = (double*)dalloc(n * sizeof(double));
+        HIP_OK(hipMemset(p, 0, n * sizeof(double)));
+        return p;
+    };
+    Un_ = dz(4 * N);
+    Ua_ = dz(4 * N);
+    Ub_ = dz(4 * N);
+    W_ = dz(4 * N);
+    prims_ = dz(4 * N);
+    h_ = dz(N);
+    rhs_ = dz(4 * N);
+    lam_cell_ = dz(N);
+    fs_ = L_.teno ? dz(4 * (size_t)L_.S * L_.Q * N) : dz(1);
+    dt_dev_ = dz(1);
+    dt_hist_ = dz(kDtHistCap);
+    dt_hist_cap_ = kDtHistCap;
+    lam_max_ = (unsigned long long*)dz(1);
+    step_dev_ = (int64_t*)dz(1);
+    dec_ = (uint8_t*)dalloc(N);
+    HIP_OK(hipMemset(dec_, 0, N));
+}
+
+dev::Geom Solver::geom() const {
+    dev::Geom g;
+    g.N = L_.N;
+    g.S = L_.S;
+    g.Q = L_.Q;
+    g.volume = volume_;
+    g.nb = nb_;
+    g.info = info_;
+    g.nx = nx_;
+    g.ny = ny_;
+    g.area = area_;
+    g.dxn = dxn_;
+    g.qw = qw_;
+    g.bc = bc_;
+    return g;
+}
+
+dev::Teno Solver::teno() const {
+    dev::Teno t{};
+    t.K = L_.K;
+    t.M = L_.M;
+    t.variant = (int32_t)L_.tspec.variant;
+    t.basis0 = L_.basis0;
+    t.ids = ids_;
+    t.ta = ta_;
+    t.A = A_;
+    t.fbasis = fbasis_;
+    const int Kp = L_.K - 1;
+    for (int i = 0; i < Kp; ++i)
+        for (int j = 0; j < Kp; ++j) t.OI[i * Kp + j] = L_.OI[(size_t)(i + 1) * L_.K + (j + 1)];
+    return t;
+}
+
+dev::StageArgs Solver::stage(int op, const double* Uin, double* Uout) const {
+    dev::StageArgs a{};
+    a.op = op;
+    a.riemann = (int32_t)cfg_.riemann;
+    a.fo = L_.teno ? 0 : 1;
+    a.vacuum_guard = (L_.teno && L_.tspec.variant == TenoVariant::Corrected) ? 1 : 0;
+    a.Uin = Uin;
+    a.Un = Un_;
+    a.Uout = Uout;
+    a.W = W_;
+    a.prims = prims_;
+    a.h = h_;
+    a.rhs = rhs_;
+    a.fs = fs_;
+    a.dt = dt_dev_;
+    return a;
+}
+
+void Solver::launch_teno(const double* U) {
+    HIP_OK(dev::teno_faces(geom(), teno(), U, fs_, dec_, ph_, stream_));
+}
+
+void Solver::launch_flux(int op, const double* Uin, double* Uout) {
+    HIP_OK(dev::flux_update(geom(), stage(op, Uin, Uout), ph_, stream_));
+}
+
+void Solver::stage_pass(int op, const double* Uin, double* Uout) {
+    if (L_.teno) launch_teno(Uin);
+    launch_flux(op, Uin, Uout);
+}
+
+void Solver::dt_pass(double cfl, double fixed_dt) {
+    if (fixed_dt <= 0.0)
+        HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_max_, stream_));
+    HIP_OK(dev::finalize_dt(lam_max_, cfl, fixed_dt, dt_dev_, dt_hist_, dt_hist_cap_, step_dev_, lam_max_, stream_));
+}
+
+// time_integrator.cpp:57-163 as fused stage passes
+void Solver::launch_step() {
+    dt_pass(cfg_.cfl, cfg_.use_cfl ? 0.0 : cfg_.dt);
+    switch (cfg_.integrator) {
+    case Integrator::FE:
+        stage_pass(dev::kFE, Un_, Ua_);
+        HIP_OK(dev::copy_state(Ua_, Un_, L_.N, stream_));
+        break;
+    case Integrator::SSPRK3:
+        stage_pass(dev::kSSP1, Un_, Ua_);
+        stage_pass(dev::kSSP2, Ua_, Ub_);
+        stage_pass(dev::kSSP3, Ub_, Un_);
+        break;
+    case Integrator::RK4:
+        stage_pass(dev::kRK4_1, Un_, Ua_);
+        stage_pass(dev::kRK4_2, Ua_, Ub_);
+        stage_pass(dev::kRK4_3, Ub_, Ua_);
+        stage_pass(dev::kRK4_4, Ua_, Un_);
+        break;
+    }
+}
+
+void Solver::record_step() {
+    if (step_exec_) return;
+    HIP_OK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    launch_step();
+    HIP_OK(hipStreamEndCapture(stream_, &step_graph_));
+    HIP_OK(hipGraphInstantiate(&step_exec_, step_graph_, nullptr, nullptr, 0));
+}
+
+void Solver::synchronize() { HIP_OK(hipStreamSynchronize(stream_)); }
+
+void Solver::set_cfl(double cfl) {
+    cfg_.cfl = cfl;
+    cfg_.use_cfl = true;
+    // the captured step carries cfl as a kernel argument
+    if (step_exec_) { HIP_OK(hipGraphExecDestroy(step_exec_)); step_exec_ = nullptr; }
+    if (step_graph_) { HIP_OK(hipGraphDestroy(step_graph_)); step_graph_ = nullptr; }
+}
+
+void Solver::device_stencils(uint32_t cell, uint32_t* out) const {
+    if (!L_.teno) fail("stencils exist only for TENO");
+    if (cell >= L_.N) fail("cell out of range");
+    const uint32_t j = L_.iperm[cell];
+    const int Mp = L_.M - 1;
+    uint32_t info0 = 0;
+    HIP_OK(hipMemcpy(&info0, info_ + j, 4, hipMemcpyDeviceToHost));
+    for (int s = 0; s < L_.G; ++s) {
+        const bool on = (info0 >> (20 + s)) & 1u;
+        out[(size_t)s * L_.M] = on ? cell : 0xFFFFFFFFu;
+        for (int i = 0; i < Mp; ++i) {
+            int32_t id = -1;
+            HIP_OK(hipMemcpy(&id, ids_ + ((size_t)s * Mp + i) * L_.N + j, 4, hipMemcpyDeviceToHost));
+            out[(size_t)s * L_.M + 1 + i] = on ? L_.perm[id] : 0xFFFFFFFFu;
+        }
+    }
+}
+
+void Solver::run_steps(int64_t n, double* dts_out) {
+    if (n <= 0) return;
+    if (cfg_.use_cfl && cfg_.cfl < 0.0) fail("dt negative: " + std::to_string(cfg_.cfl) + ".");
+    int64_t s0 = 0;
+    HIP_OK(hipMemcpyAsync(&s0, step_dev_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    if (use_graphs_) record_step();
+    for (int64_t i = 0; i < n; ++i) {
+        if (use_graphs_) HIP_OK(hipGraphLaunch(step_exec_, stream_));
+        else launch_step();
+    }
+    synchronize();
+    std::vector<double> d(dt_hist_cap_);
+    HIP_OK(hipMemcpy(d.data(), dt_hist_, sizeof(double) * dt_hist_cap_, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) {
+        const double dt = d[(s0 + i) % dt_hist_cap_];
+        if (dts_out) dts_out[i] = dt;
+        if (dt < 0.0) fail("dt negative: " + std::to_string(dt) + ".");
+        t_ += dt;
+        dt_ = dt;
+    }
+    step_ += n;
+}
+
+double Solver::calc_dt() {
+    if (cfg_.use_cfl) {
+        HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_max_, stream_));
+        HIP_OK(dev::finalize_dt(lam_max_, cfg_.cfl, 0.0, dt_dev_, nullptr, 1, nullptr, lam_max_, stream_));
+    } else {
+        HIP_OK(dev::finalize_dt(lam_max_, 0.0, cfg_.dt, dt_dev_, nullptr, 1, nullptr, nullptr, stream_));
+    }
+    HIP_OK(hipMemcpyAsync(&dt_, dt_dev_, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    if (dt_ < 0.0) fail("dt negative: " + std::to_string(dt_) + ".");
+    return dt_;
+}
+
+void Solver::take_step() {
+    switch (cfg_.integrator) {
+    case Integrator::FE:
+        stage_pass(dev::kFE, Un_, Ua_);
+        HIP_OK(dev::copy_state(Ua_, Un_, L_.N, stream_));
+        break;
+    case Integrator::SSPRK3:
+        stage_pass(dev::kSSP1, Un_, Ua_);
+        stage_pass(dev::kSSP2, Ua_, Ub_);
+        stage_pass(dev::kSSP3, Ub_, Un_);
+        break;
+    case Integrator::RK4:
+        stage_pass(dev::kRK4_1, Un_, Ua_);
+        stage_pass(dev::kRK4_2, Ua_, Ub_);
+        stage_pass(dev::kRK4_3, Ub_, Ua_);
+        stage_pass(dev::kRK4_4, Ua_, Un_);
+        break;
+    }
+    synchronize();
+    step_++;
+    t_ += dt_;
+}
+
+void Solver::calc_rhs(double* rhs_ref) {
+    if (L_.teno) launch_teno(Un_);
+    launch_flux(dev::kRhsOnly, Un_, nullptr);
+    std::vector<double> r(4 * (size_t)L_.N);
+    HIP_OK(hipMemcpyAsync(r.data(), rhs_, r.size() * 8, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    for (uint32_t j = 0; j < L_.N; ++j) std::memcpy(&rhs_ref[4 * (size_t)L_.perm[j]], &r[4 * (size_t)j], 32);
+}
+
+void Solver::set_state(const double* U, const double* prim5) {
+    const size_t N = L_.N;
+    std::vector<double> u(4 * N), w(4 * N), h(N);
+    for (uint32_t j = 0; j < N; ++j) {
+        const uint32_t c = L_.perm[j];
+        std::memcpy(&u[4 * (size_t)j], &U[4 * (size_t)c], 32);
+        double W5[5];
+        if (prim5) std::memcpy(W5, &prim5[5 * (size_t)c], 40);
+        else host_prims(ph_, &U[4 * (size_t)c], W5);
+        std::memcpy(&w[4 * (size_t)j], W5, 32);
+        h[j] = W5[4];
+    }
+    HIP_OK(hipMemcpy(Un_, u.data(), u.size() * 8, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(prims_, w.data(), w.size() * 8, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h_, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+}
+
+void Solver::get_state(double* U, double* prim5) {
+    const size_t N = L_.N;
+    std::vector<double> u(4 * N), w(4 * N), h(N);
+    synchronize();
+    HIP_OK(hipMemcpy(u.data(), Un_, u.size() * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(w.data(), prims_, w.size() * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(h.data(), h_, h.size() * 8, hipMemcpyDeviceToHost));
+    for (uint32_t j = 0; j < N; ++j) {
+        const uint32_t c = L_.perm[j];
+        if (U) std::memcpy(&U[4 * (size_t)c], &u[4 * (size_t)j], 32);
+        if (prim5) {
+            std::memcpy(&prim5[5 * (size_t)c], &w[4 * (size_t)j], 32);
+            prim5[5 * (size_t)c + 4] = h[j];
+        }
+    }
+}
+
+// solver_initialize.cpp:35-186
+void Solver::init_solution() {
+    const uint32_t N = mesh_.n_cells;
+    std::vector<double> U(4 * (size_t)N), W(5 * (size_t)N);
+    auto put = [&](uint32_t c, double rho, double u0, double u1, double p, double T) {
+        const double e = ph_.cv * T;
+        const double E = e + 0.5 * (u0 * u0 + u1 * u1);
+        const double h = e + p / rho;
+        double* Uc = &U[4 * (size_t)c];
+        Uc[0] = rho; Uc[1] = rho * u0; Uc[2] = rho * u1; Uc[3] = rho * E;
+        double* Wc = &W[5 * (size_t)c];
+        Wc[0] = u0; Wc[1] = u1; Wc[2] = p; Wc[3] = T; Wc[4] = h;
+    };
+    const InitSpec& I = cfg_.init;
+    if (I.kind == InitSpec::Constant) {
+        const double rho = I.p / (ph_.R * I.T);
+        for (uint32_t c = 0; c < N; ++c) put(c, rho, I.u[0], I.u[1], I.p, I.T);
+    } else if (I.kind == InitSpec::Quadrant) {
+        for (uint32_t c = 0; c < N; ++c) {
+            const bool r = mesh_.cc(c, 0) >= I.xs, t = mesh_.cc(c, 1) >= I.ys;
+            const int q = r ? (t ? 0 : 3) : (t ? 1 : 2);
+            const double* s = &I.states[4 * q];
+            const double rho = s[0], p = s[3];
+            put(c, rho, s[1], s[2], p, p / (rho * ph_.R));
+        }
+    } else {
+        Expression ux(I.u_expr[0]), uy(I.u_expr[1]);
+        std::unique_ptr<Expression> er, ep, eT;
+        if (!I.rho_expr.empty()) er.reset(new Expression(I.rho_expr));
+        if (!I.p_expr.empty()) ep.reset(new Expression(I.p_expr));
+        if (!I.T_expr.empty()) eT.reset(new Expression(I.T_expr));
+        for (uint32_t c = 0; c < N; ++c) {
+            const double x = mesh_.cc(c, 0), y = mesh_.cc(c, 1);
+            const double u0 = ux.eval(x, y), u1 = uy.eval(x, y);
+            double rho = 0.0, p = 0.0, T = 0.0;
+            if (er) rho = er->eval(x, y);
+            if (ep) p = ep->eval(x, y);
+            if (eT) T = eT->eval(x, y);
+            if (er && ep) T = p / (rho * ph_.R);
+            else if (er && eT) p = rho * ph_.R * T;
+            else if (ep && eT) rho = p / (ph_.R * T);
+            else fail("Exactly two of rho, p, and T must be specified for initialization: analytical.");
+            put(c, rho, u0, u1, p, T);
+        }
+    }
+    set_state(U.data(), W.data());
+}
+
+void Solver::get_face_values(double* out) {
+    const uint32_t Q = L_.Q, N = L_.N;
+    const size_t nf = mesh_.n_faces;
+    std::fill(out, out + nf * Q * 2 * 4, NAN);
+    std::vector<double> src;
+    synchronize();
+    if (L_.teno) {
+        src.resize(4 * (size_t)L_.S * Q * N);
+        HIP_OK(hipMemcpy(src.data(), fs_, src.size() * 8, hipMemcpyDeviceToHost));
+    } else {
+        src.resize(4 * (size_t)N);
+        HIP_OK(hipMemcpy(src.data(), Un_, src.size() * 8, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t f = 0; f < mesh_.n_faces_real; ++f)
+        for (int side = 0; side < 2; ++side) {
+            const int32_t c = mesh_.cof(f, side);
+            if (c < 0) continue;
+            const uint32_t j = L_.iperm[c];
+            uint32_t l = 0;
+            for (uint32_t k = 0; k < L_.S; ++k)
+                if (mesh_.face_of_cell(c, k) == f) l = k;
+            for (uint32_t q = 0; q < Q; ++q) {
+                const double* s = L_.teno ? &src[4 * ((size_t)(l * Q + q) * N + j)] : &src[4 * (size_t)j];
+                std::memcpy(&out[((f * (size_t)Q + q) * 2 + side) * 4], s, 32);
+            }
+        }
+}
+
+void Solver::get_decisions(uint8_t* out) {
+    std::vector<uint8_t> d(L_.N);
+    synchronize();
+    HIP_OK(hipMemcpy(d.data(), dec_, d.size(), hipMemcpyDeviceToHost));
+    for (uint32_t j = 0; j < L_.N; ++j) out[L_.perm[j]] = d[j];
+}
+
+void Solver::get_cfl_local(double* out) {
+    std::vector<double> l(L_.N);
+    synchronize();
+    HIP_OK(hipMemcpy(l.data(), lam_cell_, l.size() * 8, hipMemcpyDeviceToHost));
+    for (uint32_t j = 0; j < L_.N; ++j) out[L_.perm[j]] = l[j] * dt_;  // KokkosBlas::scal
+}
+
+void Solver::check_fields() {
+    if (!cfg_.check_nan) return;
+    const uint32_t N = mesh_.n_cells;
+    std::vector<double> U(4 * (size_t)N), W(5 * (size_t)N);
+    get_state(U.data(), W.data());
+    for (uint32_t c = 0; c < N; ++c) {
+        bool nan = false;
+        for (int i = 0; i < 4; ++i) nan |= std::isnan(U[4 * (size_t)c + i]);
+        for (int i = 0; i < 5; ++i) nan |= std::isnan(W[5 * (size_t)c + i]);
+        if (nan) {
+            std::ostringstream msg;
+            msg << "NaN found in solution.\nt: " << t_ << "\nstep: " << step_ << "\ni_cell: " << c
+                << "\n> x: " << mesh_.cc(c, 0) << "\n> y: " << mesh_.cc(c, 1) << "\n";
+            fail(msg.str());
+        }
+    }
+}
+
+// solver.cpp:352-403 (checks and VTU output are not part of this engine)
+int Solver::run() {
+    auto done = [&]() {
+        bool d = false;
+        if (cfg_.n_steps > 0) d |= step_ >= cfg_.n_steps;
+        if (cfg_.t_stop > 0) d |= t_ >= cfg_.t_stop;
+        if (cfg_.t_wall_stop > 0) d |= (wall_seconds() - wall0_) >= cfg_.t_wall_stop;
+        return d;
+    };
+    const bool batch = cfg_.t_stop <= 0 && cfg_.t_wall_stop <= 0 && !cfg_.check_nan;
+    if (batch) {
+        const int64_t n = cfg_.n_steps - step_;
+        if (n > 0) run_steps(n, nullptr);
+        return 0;
+    }
+    while (!done()) {
+        run_steps(1, nullptr);
+        check_fields();
+    }
+    return 0;
+}
+
+}  // namespace mlr

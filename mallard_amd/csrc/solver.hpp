@@ -1,0 +1,147 @@
+// Host solver mirroring the reference's Solver (solver.h:30-279): TOML
+// input, mesh / physics / numerics / boundaries / run-parameter init, the run
+// loop, and the residual, dt and primitive updates, all executed by the HIP
+// kernels on one GPU through one stream.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "layout.hpp"
+#include "mesh.hpp"
+#include "toml.hpp"
+
+namespace mlr {
+
+struct InitSpec {
+    enum Kind { Constant, Analytical, Quadrant } kind = Constant;
+    double u[2] = {0.0, 0.0};
+    double p = 0.0, T = 0.0;                          // constant
+    std::string u_expr[2], rho_expr, p_expr, T_expr;  // analytical
+    double xs = 0.5, ys = 0.5, states[16] = {};       // quadrant (rho,u,v,p) UR, UL, LL, LR
+};
+
+struct Config {
+    // [mesh]
+    MeshType mesh_type = MeshType::File;
+    uint32_t nx = 100, ny = 100;
+    double Lx = 1.0, Ly = 1.0, jitter = 0.0;
+    uint64_t seed = 0;
+    // [physics]
+    double gamma = 1.4, p_ref = 101325.0, T_ref = 298.15, rho_ref = 1.225, p_min = -1e20, p_max = 1e20;
+    // [numerics]
+    Recon recon = Recon::FO;
+    TenoSpec teno;
+    Riemann riemann = Riemann::HLLC;
+    Integrator integrator = Integrator::SSPRK3;
+    bool check_nan = false;
+    std::vector<BoundarySpec> bcs;
+    // [run]
+    bool use_cfl = true;
+    double cfl = 0.5, dt = 0.0;
+    int64_t n_steps = -1;
+    double t_stop = -1.0, t_wall_stop = -1.0;
+    // [output]
+    uint32_t check_interval = 1;
+    InitSpec init;
+    int n_threads = 0;  // host precompute threads (0: OpenMP default)
+};
+
+// Parses a Mallard TOML input with the reference's keys and defaults.
+Config config_from_toml(const toml::Value& in);
+
+class Solver {
+public:
+    Solver();
+    ~Solver();
+    Solver(const Solver&) = delete;
+    Solver& operator=(const Solver&) = delete;
+
+    // solver.cpp:39-80
+    int init(const std::string& input_file);
+    void init(const Config& cfg);                 // generated mesh from cfg
+    void init(const Config& cfg, Mesh&& mesh);    // caller mesh (reference numbering)
+    // solver.cpp:352-373 (no VTU output)
+    int run();
+
+    // reference-numbered host state I/O
+    void set_state(const double* U, const double* prim5);
+    void get_state(double* U, double* prim5);
+    void init_solution();
+    void calc_rhs(double* rhs_ref);                 // R(Un) into host rhs (reference order)
+    double calc_dt();                               // solver.cpp:580-590
+    void take_step();                               // one step with the current dt mode
+    // n steps, dt from CFL (or fixed dt) computed on the device; the step is
+    // captured once into a hipGraph and replayed
+    void run_steps(int64_t n, double* dts_out);
+    void get_face_values(double* out);              // [n_faces alloc][Q][2][4], reference numbering
+    void get_decisions(uint8_t* out);               // [N] bit v = central stencil for variable v
+    void get_cfl_local(double* out);
+    void check_fields();                            // solver.cpp:464-498
+    void synchronize();
+    void set_cfl(double cfl);
+    // stencil ids of one reference cell as uploaded (reference numbering, G x M)
+    void device_stencils(uint32_t cell, uint32_t* ids) const;
+
+    const Config& config() const { return cfg_; }
+    const Mesh& mesh() const { return mesh_; }
+    const Layout& layout() const { return L_; }
+    double time() const { return t_; }
+    int64_t step() const { return step_; }
+    double dt() const { return dt_; }
+    hipStream_t stream() const { return stream_; }
+    void set_graphs(bool on) { use_graphs_ = on; }
+
+    // kernel-level entry points used by the benchmark's timing harness
+    void launch_step();                 // one full step, asynchronously
+    void launch_teno(const double* U);  // one TENO face-state pass
+    void launch_flux(int op, const double* Uin, double* Uout);
+    double* d_state() { return Un_; }
+    double* d_fs() { return fs_; }
+
+private:
+    void setup(Mesh&& mesh);
+    void upload();
+    void free_device();
+    void record_step();
+    dev::Geom geom() const;
+    dev::Teno teno() const;
+    dev::StageArgs stage(int op, const double* Uin, double* Uout) const;
+    void stage_pass(int op, const double* Uin, double* Uout);
+    void dt_pass(double cfl, double fixed_dt);
+
+    Config cfg_;
+    Mesh mesh_;
+    Layout L_;
+    dev::Physics ph_{};
+    hipStream_t stream_ = nullptr;
+    bool use_graphs_ = true;
+
+    // device memory
+    std::vector<void*> allocs_;
+    double *volume_ = nullptr, *nx_ = nullptr, *ny_ = nullptr, *area_ = nullptr, *dxn_ = nullptr, *qw_ = nullptr;
+    int32_t* nb_ = nullptr;
+    uint32_t* info_ = nullptr;
+    dev::BCParam* bc_ = nullptr;
+    int32_t* ids_ = nullptr;
+    double *ta_ = nullptr, *A_ = nullptr, *fbasis_ = nullptr;
+    double *Un_ = nullptr, *Ua_ = nullptr, *Ub_ = nullptr, *W_ = nullptr, *prims_ = nullptr, *h_ = nullptr;
+    double *fs_ = nullptr, *rhs_ = nullptr, *lam_cell_ = nullptr, *dt_dev_ = nullptr, *dt_hist_ = nullptr;
+    unsigned long long* lam_max_ = nullptr;
+    int64_t* step_dev_ = nullptr;
+    uint8_t* dec_ = nullptr;
+    int64_t dt_hist_cap_ = 0;
+
+    hipGraphExec_t step_exec_ = nullptr;
+    hipGraph_t step_graph_ = nullptr;
+
+    double t_ = 0.0, dt_ = 0.0;
+    int64_t step_ = 0;
+    double wall0_ = 0.0;
+};
+
+}  // namespace mlr

@@ -1,0 +1,270 @@
+"""ctypes binding of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg. It is the checker, never the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+
+MESH = {"cartesian": 1, "cartesian_tri": 2, "wedge": 3}
+RECON = {"FO": 0, "TENO": 1}
+BASIS = {"monomial": 0, "legendre": 1}
+RIEMANN = {"Rusanov": 0, "HLL": 1, "HLLC": 2, "Roe": 3}
+INTEGRATOR = {"FE": 0, "RK4": 1, "SSPRK3": 2}
+BC = {"symmetry": 0, "extrapolation": 1, "wall_adiabatic": 2, "upt": 3, "p_out": 4}
+TENO_VARIANT = {"reference": 0, "corrected": 1}
+ZONES = ["interior", "right", "top", "left", "bottom"]
+
+
+class OrcBC(C.Structure):
+    _fields_ = [("zone", C.c_int32), ("type", C.c_int32), ("u", C.c_double * 2),
+                ("p", C.c_double), ("T", C.c_double)]
+
+
+class OrcConfig(C.Structure):
+    _fields_ = [("recon", C.c_int32), ("basis", C.c_int32), ("order", C.c_int32),
+                ("stencil_factor", C.c_double), ("quad_order_cell", C.c_int32),
+                ("quad_order_face", C.c_int32), ("riemann", C.c_int32),
+                ("integrator", C.c_int32), ("gamma", C.c_double), ("p_ref", C.c_double),
+                ("T_ref", C.c_double), ("rho_ref", C.c_double), ("p_min", C.c_double),
+                ("p_max", C.c_double), ("n_bc", C.c_int32), ("bc", OrcBC * 8),
+                ("teno_variant", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.orc_mesh_create.restype = P
+        L.orc_mesh_create.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                      C.c_double, C.c_uint64]
+        L.orc_mesh_destroy.argtypes = [P]
+        L.orc_mesh_counts.argtypes = [P, C.c_void_p]
+        L.orc_mesh_arrays.argtypes = [P] + [C.c_void_p] * 12
+        L.orc_mesh_zone.restype = C.c_int64
+        L.orc_mesh_zone.argtypes = [P, C.c_int, C.c_void_p, C.c_char_p]
+        L.orc_solver_create.restype = P
+        L.orc_solver_create.argtypes = [P, C.POINTER(OrcConfig), C.c_int]
+        L.orc_solver_destroy.argtypes = [P]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_teno_sizes.argtypes = [P, C.c_void_p]
+        L.orc_teno_arrays.argtypes = [P] + [C.c_void_p] * 9
+        L.orc_init_quadrant.argtypes = [P, C.c_double, C.c_double, C.c_void_p]
+        L.orc_init_constant.argtypes = [P, C.c_void_p, C.c_double, C.c_double]
+        L.orc_set_state.argtypes = [P, C.c_void_p, C.c_void_p]
+        L.orc_get_state.argtypes = [P, C.c_void_p, C.c_void_p]
+        L.orc_calc_rhs.argtypes = [P, C.c_void_p, C.c_void_p]
+        L.orc_get_face_values.argtypes = [P, C.c_void_p]
+        L.orc_calc_dt.argtypes = [P, C.c_double, dp, C.c_void_p]
+        L.orc_take_step.argtypes = [P, C.c_double]
+        L.orc_run.argtypes = [P, C.c_int, C.c_double, C.c_void_p]
+        L.orc_time.restype = C.c_double
+        L.orc_time.argtypes = [P]
+        L.orc_teno_decisions.argtypes = [P, C.c_void_p]
+        L.orc_riemann_flux.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
+                                       C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.orc_physics_constants.argtypes = [C.c_double] * 4 + [C.c_void_p]
+        L.orc_primitives.argtypes = [C.c_void_p] * 3
+        L.orc_qr_noq.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.orc_forward_substitution.argtypes = [C.c_void_p] * 3 + [C.c_int] * 5
+        L.orc_back_substitution.argtypes = [C.c_void_p] * 3 + [C.c_int] * 5
+        L.orc_gemm.argtypes = [C.c_void_p] * 3 + [C.c_int] * 6
+        L.orc_invert2.argtypes = [C.c_void_p] * 2
+        L.orc_invert3.argtypes = [C.c_void_p] * 2
+        L.orc_triangle_area.restype = C.c_double
+        L.orc_triangle_area.argtypes = [C.c_void_p] * 3
+        L.orc_triangle_J.argtypes = [C.c_void_p] * 4
+        L.orc_basis_1d.restype = C.c_double
+        L.orc_basis_1d.argtypes = [C.c_int, C.c_int, C.c_double]
+        L.orc_basis_deriv_1d.restype = C.c_double
+        L.orc_basis_deriv_1d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double]
+        L.orc_integrator_constant_rhs.argtypes = [C.c_int, C.c_double, C.c_void_p, C.c_int,
+                                                  C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _f64(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+class Mesh:
+    """Reference-numbered mesh from the oracle's generators (mesh.cpp:305-848)."""
+
+    def __init__(self, kind, nx, ny, Lx=1.0, Ly=1.0, jitter=0.0, seed=0):
+        L = lib()
+        self.h = L.orc_mesh_create(MESH[kind], nx, ny, Lx, Ly, jitter, seed)
+        if not self.h:
+            raise RuntimeError(L.orc_last_error().decode())
+        c = np.zeros(6, np.int64)
+        L.orc_mesh_counts(self.h, _p(c))
+        self.n_cells, self.n_nodes, self.n_faces, self.n_faces_real, n_noc, self.n_zones = map(int, c)
+        nc, nf = self.n_cells, self.n_faces
+        self.node_coords = np.zeros((self.n_nodes, 2))
+        self.cell_coords = np.zeros((nc, 2))
+        self.cell_volume = np.zeros(nc)
+        self.face_area = np.zeros(nf)
+        self.face_normals = np.zeros((nf, 2))
+        self.nodes_of_cell = np.zeros(n_noc, np.uint32)
+        self.offsets_nodes_of_cell = np.zeros(nc + 1, np.uint32)
+        self.faces_of_cell = np.zeros(n_noc, np.uint32)
+        self.offsets_faces_of_cell = np.zeros(nc + 1, np.uint32)
+        self.nodes_of_face = np.zeros(2 * nf, np.uint32)
+        self.offsets_nodes_of_face = np.zeros(nf + 1, np.uint32)
+        self.cells_of_face = np.zeros((nf, 2), np.int32)
+        L.orc_mesh_arrays(self.h, _p(self.node_coords), _p(self.cell_coords), _p(self.cell_volume),
+                          _p(self.face_area), _p(self.face_normals), _p(self.nodes_of_cell),
+                          _p(self.offsets_nodes_of_cell), _p(self.faces_of_cell),
+                          _p(self.offsets_faces_of_cell), _p(self.nodes_of_face),
+                          _p(self.offsets_nodes_of_face), _p(self.cells_of_face))
+        self.zones = {}
+        for z in range(self.n_zones):
+            n = L.orc_mesh_zone(self.h, z, None, None)
+            buf = np.zeros(n, np.uint32)
+            name = C.create_string_buffer(32)
+            L.orc_mesh_zone(self.h, z, _p(buf), name)
+            self.zones[name.value.decode()] = buf
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_mesh_destroy(self.h)
+            self.h = None
+
+
+def make_config(recon="TENO", basis="legendre", order=2, factor=2.0, qcell=0, qface=0,
+                riemann="HLLC", integrator="SSPRK3", gamma=1.4, p_ref=101325.0, T_ref=298.15,
+                rho_ref=1.225, p_min=-1e20, p_max=1e20, bcs=None,
+                teno_variant="reference"):
+    cfg = OrcConfig()
+    cfg.recon, cfg.basis, cfg.order = RECON[recon], BASIS[basis], order
+    cfg.stencil_factor, cfg.quad_order_cell, cfg.quad_order_face = factor, qcell, qface
+    cfg.riemann, cfg.integrator = RIEMANN[riemann], INTEGRATOR[integrator]
+    cfg.gamma, cfg.p_ref, cfg.T_ref, cfg.rho_ref = gamma, p_ref, T_ref, rho_ref
+    cfg.p_min, cfg.p_max = p_min, p_max
+    cfg.teno_variant = TENO_VARIANT[teno_variant]
+    bcs = bcs if bcs is not None else [(z, "symmetry", {}) for z in ("left", "right", "top", "bottom")]
+    cfg.n_bc = len(bcs)
+    for i, (zone, kind, par) in enumerate(bcs):
+        b = cfg.bc[i]
+        b.zone, b.type = ZONES.index(zone), BC[kind]
+        u = par.get("u", (0.0, 0.0))
+        b.u[0], b.u[1] = u
+        b.p, b.T = par.get("p", 0.0), par.get("T", 0.0)
+    return cfg
+
+
+class Solver:
+    """Oracle solver: one thread reproduces the canonical one-thread order."""
+
+    def __init__(self, mesh, cfg, n_threads=1):
+        L = lib()
+        self.mesh = mesh
+        self.cfg = cfg
+        self.h = L.orc_solver_create(mesh.h, C.byref(cfg), n_threads)
+        if not self.h:
+            raise RuntimeError(L.orc_last_error().decode())
+        self.nc = mesh.n_cells
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_solver_destroy(self.h)
+            self.h = None
+
+    def teno(self):
+        L = lib()
+        s = np.zeros(5, np.int64)
+        L.orc_teno_sizes(self.h, _p(s))
+        K, M, nst, nos, nrm = map(int, s)
+        out = dict(n_dof=K, max_cells=M,
+                   offsets_stencil_groups=np.zeros(self.nc + 1, np.uint32),
+                   offsets_stencils=np.zeros(nos, np.uint32), stencils=np.zeros(nst, np.uint32),
+                   offsets_recon=np.zeros(nos, np.uint64), recon=np.zeros(nrm),
+                   transformed_areas=np.zeros(nst), integral_psi_target=np.zeros(K),
+                   oscillation_indicator=np.zeros(K * K), poly_indices=np.zeros(2 * K, np.uint8))
+        L.orc_teno_arrays(self.h, *[_p(out[k]) for k in (
+            "offsets_stencil_groups", "offsets_stencils", "stencils", "offsets_recon", "recon",
+            "transformed_areas", "integral_psi_target", "oscillation_indicator", "poly_indices")])
+        return out
+
+    def init_quadrant(self, xs, ys, states):
+        st = _f64(states).reshape(16)
+        lib().orc_init_quadrant(self.h, xs, ys, _p(st))
+
+    def init_constant(self, u, p, T):
+        uu = _f64(u)
+        lib().orc_init_constant(self.h, _p(uu), p, T)
+
+    def set_state(self, U, prim=None):
+        U = _f64(U)
+        prim = _f64(prim) if prim is not None else None
+        lib().orc_set_state(self.h, _p(U), _p(prim))
+
+    def state(self):
+        U = np.zeros((self.nc, 4))
+        W = np.zeros((self.nc, 5))
+        lib().orc_get_state(self.h, _p(U), _p(W))
+        return U, W
+
+    def calc_rhs(self, U):
+        U = _f64(U)
+        r = np.zeros((self.nc, 4))
+        if lib().orc_calc_rhs(self.h, _p(U), _p(r)):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return r
+
+    def face_values(self, nq):
+        out = np.zeros((self.mesh.n_faces, nq, 2, 4))
+        lib().orc_get_face_values(self.h, _p(out))
+        return out
+
+    def calc_dt(self, cfl):
+        dt = C.c_double()
+        cl = np.zeros(self.nc)
+        rc = lib().orc_calc_dt(self.h, cfl, C.byref(dt), _p(cl))
+        if rc:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return dt.value, cl
+
+    def take_step(self, dt):
+        if lib().orc_take_step(self.h, dt):
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def run(self, n, cfl):
+        dts = np.zeros(n)
+        rc = lib().orc_run(self.h, n, cfl, _p(dts))
+        if rc:
+            raise RuntimeError(lib().orc_last_error().decode())
+        return dts
+
+    def time(self):
+        return lib().orc_time(self.h)
+
+    def decisions(self):
+        d = np.zeros((self.nc, 4), np.uint8)
+        lib().orc_teno_decisions(self.h, _p(d))
+        return d
+
+
+def riemann_flux(kind, n, rho_l, u_l, p_l, g_l, h_l, rho_r, u_r, p_r, g_r, h_r):
+    F = np.zeros(4)
+    n, ul, ur = _f64(n), _f64(u_l), _f64(u_r)
+    lib().orc_riemann_flux(RIEMANN[kind], _p(F), _p(n), rho_l, _p(ul), p_l, g_l, h_l, rho_r,
+                           _p(ur), p_r, g_r, h_r)
+    return F
