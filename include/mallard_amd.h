@@ -192,9 +192,12 @@ int mlr_layout_teno_tables(const mlr_layout* lay, double* ipt, double* oi);
 int mlr_layout_slots(const mlr_layout* lay, uint32_t cell, uint32_t* nb, uint32_t* info, double* nx,
                      double* ny, double* area, double* dxn);
 
-/* Device math probes (0 sqrt, 1 divide, 2 pow, 3 x^6) for the parity tests. */
+/* Device math probes for the parity tests: 0 sqrt, 1 divide, 2 the
+ * double-double pow of the two-rarefaction estimate, 3 x^6, 4 the platform
+ * pow. */
 int mlr_probe_math(int which, const double* x, const double* y, double* out, int n);
-/* The host x^6 used by the precompute and the tests. */
+/* Host builds of the same double-double pow and correctly rounded x^6. */
+int mlr_host_pow(const double* x, const double* y, double* out, int n);
 int mlr_host_pow6(const double* x, double* out, int n);
 
 #ifdef __cplusplus

@@ -82,6 +82,7 @@ _SIGS = {
     "mlr_layout_teno_tables": (C.c_int, [_P, _P, _P]),
     "mlr_layout_slots": (C.c_int, [_P, C.c_uint32] + [_P] * 6),
     "mlr_probe_math": (C.c_int, [C.c_int, _P, _P, _P, C.c_int]),
+    "mlr_host_pow": (C.c_int, [_P, _P, _P, C.c_int]),
     "mlr_host_pow6": (C.c_int, [_P, _P, C.c_int]),
 }
 
@@ -327,6 +328,13 @@ def probe_math(which, x, y=None):
     rc = lib().mlr_probe_math(which, _p(x), _p(y), _p(out), x.size)
     if rc:
         raise MallardError(lib().mlr_last_error(None).decode())
+    return out
+
+
+def host_pow(x, y):
+    x, y = _f64(x), _f64(y)
+    out = np.zeros_like(x)
+    lib().mlr_host_pow(_p(x), _p(y), _p(out), x.size)
     return out
 
 

@@ -8,6 +8,7 @@
 #include <memory>
 #include <string>
 
+#include "ddpow.hpp"
 #include "solver.hpp"
 
 struct mlr_ctx {
@@ -486,6 +487,11 @@ int mlr_probe_math(int which, const double* x, const double* y, double* out, int
         (void)hipFree(dy);
         (void)hipFree(dout);
     });
+}
+
+int mlr_host_pow(const double* x, const double* y, double* out, int n) {
+    for (int i = 0; i < n; ++i) out[i] = mlr::ddpow::pow_dd(x[i], y[i]);
+    return 0;
 }
 
 int mlr_host_pow6(const double* x, double* out, int n) {

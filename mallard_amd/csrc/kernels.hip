@@ -12,6 +12,7 @@
 
 #include <cstdint>
 
+#include "ddpow.hpp"
 #include "device.hpp"
 
 namespace mlr::dev {
@@ -79,12 +80,12 @@ __device__ inline void trrs(const double* Wl, const double* Wr, double& ps, int 
     const double rr = Wr[0], ur = Wr[1], pr = Wr[2], gr = Wr[3];
     const double al = sqrt(gl * pl / rl), ar = sqrt(gr * pr / rr);
     const double zl = (gl - 1.0) / (2.0 * gl), zr = (gr - 1.0) / (2.0 * gr);
-    const double Plr = pow((pl / pr), zl);
+    const double Plr = ddpow::pow_dd((pl / pr), zl);
     const double us = (Plr * ul / al + ur / ar + 2.0 * (1.0 - Plr) / (gl - 1.0)) / (Plr / al + 1.0 / ar);
     double bl = 1.0 + (gl - 1.0) / (2.0 * al) * (ul - us);
     double br = 1.0 + (gr - 1.0) / (2.0 * ar) * (us - ur);
     if (guard) { bl = fmax(bl, 0.0); br = fmax(br, 0.0); }
-    ps = 0.5 * (pl * pow(bl, (1.0 / zl)) + pr * pow(br, (1.0 / zr)));
+    ps = 0.5 * (pl * ddpow::pow_dd(bl, (1.0 / zl)) + pr * ddpow::pow_dd(br, (1.0 / zr)));
 }
 // riemann_solver.h:273-306
 __device__ inline void tsrs(const double* Wl, const double* Wr, double& ps) {
@@ -626,7 +627,8 @@ __global__ void probe_kernel(const double* x, const double* y, double* out, int 
     switch (which) {
     case 0: out[i] = sqrt(x[i]); break;
     case 1: out[i] = x[i] / y[i]; break;
-    case 2: out[i] = pow(x[i], y[i]); break;
+    case 2: out[i] = ddpow::pow_dd(x[i], y[i]); break;
+    case 4: out[i] = pow(x[i], y[i]); break;
     case 3: out[i] = pow6_cr(x[i]); break;
     default: out[i] = 0.0;
     }
