@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Throughput of the explicit Euler residual + SSP-RK3 step on one MI355X.
+
+Workload: the 2D four-quadrant Riemann problem on a 2048x2048 quad mesh split
+into 8,388,608 triangles (interior nodes jittered by +/-0.2h from a seed),
+TENO reconstruction (p=2, Legendre basis, stencil factor 2), HLLC, SSP-RK3,
+CFL 0.4, transmissive (extrapolation) boundaries on the four sides, FP64
+throughout. A "step" is one full
+SSP-RK3 step (3 stages, each a TENO face-state pass + a flux/residual/stage
+update pass) plus its CFL time-step reduction; the metric counts
+cell-updates per RK stage: n_cells * 3 * steps / seconds.
+
+The timed region launches every kernel on the solver's stream with a HIP
+event after each launch (mlr_run_steps_profiled), so the same run yields the
+whole-step throughput and the average duration of each kernel. The same
+number of steps replayed from the captured hipGraph is timed afterwards and
+reported as ms_per_step_graph.
+
+cpu_baseline: the CPU oracle (the restatement of the reference's OpenMP path:
+AoS state, per-face atomics) timed on this host on a bounded sample: the same
+numerics on a 256x256 (131,072-triangle) mesh for about --cpu-seconds.
+
+Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 3]
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s per RK stage, learned indicator vs plain TENO at matched L1"
+QUADRANT = np.array([[1.5, 0.0, 0.0, 1.5],        # upper right
+                     [0.5323, 1.206, 0.0, 0.3],   # upper left
+                     [0.138, 1.206, 1.206, 0.029],  # lower left
+                     [0.5323, 0.0, 1.206, 0.3]])  # lower right
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "teno_kernel_riemann2048_traffic.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def numerics_kw(args):
+    bcs = [(z, args.bc, {}) for z in ("left", "right", "top", "bottom")]
+    return dict(recon="TENO", basis="legendre", order=args.order, stencil_factor=2.0,
+                teno_variant=args.variant, riemann="HLLC", integrator=args.integrator, cfl=0.4, bcs=bcs)
+
+
+def teno_bytes(n_cells, n_stencils, K, M, S, Q):
+    """Algorithmic HBM bytes of one TENO face-state launch.
+
+    alg: each compulsory element counted once, per non-empty stencil the
+    reference's K x M pseudo-inverse (8 B), M ids (4 B) and M transformed
+    areas (8 B), plus the cell's own state (32 B).
+    layout: what this build's packed layout must move at least: the
+    (K-1) x (M-1) non-zero block of the pseudo-inverse, M-1 ids and areas,
+    the cell state, its info word, the face-point basis values, the face
+    states written (S*Q points x 4 variables) and the decision byte.
+    """
+    alg = n_stencils * (8 * K * M + 12 * M) + 32 * n_cells
+    layout = (n_stencils * (8 * (K - 1) * (M - 1) + 12 * (M - 1)) +
+              n_cells * (32 + 4 + 8 * S * Q * (K - 1) + 8 * S * Q * 4 + 1))
+    return alg, layout
+
+
+def check_state(dev, what):
+    U, W = dev.state()
+    bad = ~np.isfinite(U).all(axis=1) | (U[:, 0] <= 0) | (W[:, 2] <= 0)
+    if bad.any():
+        raise RuntimeError(f"non-physical state after the {what}: {int(bad.sum())} cells, "
+                           f"min rho {np.nanmin(U[:, 0]):.3g}, min p {np.nanmin(W[:, 2]):.3g}")
+    log(f"[bench] {what}: state physical, min rho {U[:, 0].min():.3g}, min p {W[:, 2].min():.3g}")
+
+
+def cpu_baseline(args):
+    """Oracle (OpenMP, atomics) on a bounded sample of the same workload."""
+    from oracle import oracle_py as orc
+    threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    n = args.cpu_nx
+    m = orc.Mesh("cartesian_tri", n, n, 1.0, 1.0, args.jitter, args.seed)
+    bcs = [(z, args.bc, {}) for z in ("left", "right", "top", "bottom")]
+    s = orc.Solver(m, orc.make_config(order=args.order, teno_variant=args.variant, bcs=bcs), n_threads=threads)
+    s.init_quadrant(0.5, 0.5, QUADRANT)
+    t = time.perf_counter()
+    s.run(1, 0.4)
+    one = time.perf_counter() - t
+    # at most the workload's 100 steps (1 + 99) from the initial state
+    k = min(99, max(2, int(args.cpu_seconds / max(one, 1e-3))))
+    t = time.perf_counter()
+    s.run(k, 0.4)
+    el = time.perf_counter() - t
+    U, _ = s.state()
+    if not np.isfinite(U).all():
+        raise RuntimeError("oracle sample produced non-finite state")
+    return {"value": m.n_cells * 3 * k / el, "unit": "cell-updates/s per RK stage", "cores": threads,
+            "kind": "port",
+            "sample": f"CPU oracle (OpenMP, per-face atomics) on {n}x{n} quads split into {m.n_cells} "
+                      f"jittered triangles, same numerics, {k} SSP-RK3 steps in {el:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=2048, help="quads per side (2 triangles each)")
+    ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--jitter", type=float, default=0.2)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--variant", default="corrected", choices=["corrected", "reference"])
+    ap.add_argument("--bc", default="extrapolation", choices=["extrapolation", "symmetry"],
+                    help="boundary condition on the four sides (transmissive by default)")
+    ap.add_argument("--integrator", default="SSPRK3", choices=["SSPRK3", "FE"],
+                    help="FE adds one plain state copy per step (a known-size kernel for "
+                         "calibrating HBM byte counters); the metric is defined for SSPRK3")
+    ap.add_argument("--cpu-nx", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} (WORLD_SIZE {world}) is not supported: "
+              "the residual path runs on a single MI355X and has no multi-GPU mode.", file=sys.stderr)
+        sys.exit(2)
+
+    import mallard_amd as mlr
+
+    t0 = time.perf_counter()
+    log(f"[bench] building {args.nx}x{args.nx} jittered triangle mesh + TENO p={args.order} layout ...")
+    dev = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", args.nx, args.nx, 1.0, 1.0, args.jitter, args.seed),
+                     num=mlr.numerics(**numerics_kw(args)))
+    dev.init_quadrant(0.5, 0.5, QUADRANT)
+    N = dev.n_cells
+    log(f"[bench] setup {time.perf_counter() - t0:.1f} s: {N} cells, {dev.n_stencils} stencils, "
+        f"device layout {dev.device_bytes / 1e9:.2f} GB")
+
+    if args.warmup > 0:
+        dev.run_steps(args.warmup)
+    log(f"[bench] warmup {args.warmup} steps done; timing {args.steps} steps")
+
+    # every timed block is the workload itself: steps from the initial state
+    dev.init_quadrant(0.5, 0.5, QUADRANT)
+    # timed region: kernel-by-kernel launches with HIP events on the solver's stream
+    t = time.perf_counter()
+    st = dev.run_steps_profiled(args.steps)   # returns after the stream has drained
+    wall = time.perf_counter() - t
+    log(f"[bench] timed: {wall:.3f} s wall, {st['total_ms']:.1f} ms by events")
+    check_state(dev, "timed steps")
+
+    dev.init_quadrant(0.5, 0.5, QUADRANT)
+    t = time.perf_counter()
+    dev.run_steps(args.steps)
+    wall_graph = time.perf_counter() - t
+    check_state(dev, "graph-replayed steps")
+
+    teno_ms = st["teno_ms"] / max(st["teno_n"], 1)
+    flux_ms = st["flux_ms"] / max(st["flux_n"], 1)
+    alg, lay = teno_bytes(N, dev.n_stencils, dev.K, dev.M, dev.S, dev.Q)
+    achieved = alg / (teno_ms * 1e-3) / 1e9
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # summary, used only when it was measured on this workload and on the
+    # kernel source being run
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        tr = json.load(open(TRAFFIC_FILE))
+        src = open(os.path.join(ROOT, "mallard_amd", "csrc", "kernels.hip"), "rb").read()
+        if (tr.get("n_cells") == N and tr.get("order") == args.order and
+                tr.get("kernels_sha256") == hashlib.sha256(src).hexdigest()[:16]):
+            traffic = tr["bytes_per_launch"]
+
+    stages = {"SSPRK3": 3, "FE": 1}[args.integrator]
+    value = N * stages * args.steps / wall
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s per RK stage",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step_graph": wall_graph / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2D four-quadrant Riemann, {args.nx}x{args.nx} quads split into {N} triangles "
+                        f"jittered +/-{args.jitter}h (seed {args.seed}), TENO p={args.order} legendre "
+                        f"({args.variant} weighting) + HLLC + {args.integrator}, CFL 0.4, {args.bc} boundaries",
+            "n_cells": N,
+            "indicator": "none (plain FP64 TENO)",
+        },
+        "roofline": {
+            "kernel": "teno_kernel",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "alg_bytes_per_launch": alg,
+            "layout_bytes_per_launch": lay,
+            "layout_GBs": lay / (teno_ms * 1e-3) / 1e9,
+            "avg_launch_ms": teno_ms,
+        },
+        "kernels_ms": {"teno": teno_ms, "flux_update": flux_ms,
+                       "spectral_dt": st["dt_ms"] / max(st["dt_n"], 1),
+                       "copy": st["copy_ms"] / max(st["copy_n"], 1) if st["copy_n"] else None},
+    }
+    if not args.no_cpu:
+        log("[bench] CPU baseline ...")
+        try:
+            out["cpu_baseline"] = cpu_baseline(args)
+        except Exception as e:  # the GPU figures stand on their own
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

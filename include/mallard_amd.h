@@ -109,9 +109,10 @@ int mlr_ctx_create(const mlr_mesh_desc* mesh, const mlr_numerics_desc* num, mlr_
 int mlr_ctx_destroy(mlr_ctx* ctx);
 const char* mlr_last_error(const mlr_ctx* ctx);
 
-/* sizes[0..11] = n_cells, n_faces (allocated), n_faces_real, face slots per
+/* sizes[0..12] = n_cells, n_faces (allocated), n_faces_real, face slots per
  * cell, face quadrature points, n_dof, cells per stencil, stencils per cell,
- * device bytes of the layout, step, recon, teno_variant */
+ * device bytes allocated by the context, step, recon, teno_variant, non-empty TENO
+ * stencils over all cells */
 int mlr_get_sizes(const mlr_ctx* ctx, int64_t* sizes);
 
 /* Initial states (solver_initialize.cpp:35-186). states[16] = (rho,u,v,p)
@@ -136,6 +137,13 @@ int mlr_step(mlr_ctx* ctx);
  * captured once in a hipGraph and replayed; dt values written to dts if not
  * NULL. */
 int mlr_run_steps(mlr_ctx* ctx, int64_t n, double* dts);
+/* The same n steps launched kernel by kernel (no graph), with a HIP event
+ * recorded on the context's stream after every launch. No reference
+ * counterpart: it is the measurement hook behind the throughput benchmark.
+ * stats[9] = { total event ms,
+ *              TENO face-state ms, launches,  flux+stage-update ms, launches,
+ *              spectral-radius+dt ms, launches, copy ms, launches }. */
+int mlr_run_steps_profiled(mlr_ctx* ctx, int64_t n, double* stats);
 /* Solver::run (solver.cpp:352-373) with the configured stop conditions. */
 int mlr_run(mlr_ctx* ctx);
 int mlr_set_cfl(mlr_ctx* ctx, double cfl);

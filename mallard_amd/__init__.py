@@ -62,6 +62,7 @@ _SIGS = {
     "mlr_compute_dt": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "mlr_step": (C.c_int, [_P]),
     "mlr_run_steps": (C.c_int, [_P, C.c_int64, _P]),
+    "mlr_run_steps_profiled": (C.c_int, [_P, C.c_int64, _P]),
     "mlr_run": (C.c_int, [_P]),
     "mlr_set_cfl": (C.c_int, [_P, C.c_double]),
     "mlr_get_time": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
@@ -230,10 +231,11 @@ class Solver:
         if rc:
             raise MallardError(L.mlr_last_error(None).decode())
         self.h = h
-        s = np.zeros(12, np.int64)
+        s = np.zeros(13, np.int64)
         L.mlr_get_sizes(self.h, _p(s))
         self.n_cells, self.n_faces, self.n_faces_real, self.S, self.Q, self.K, self.M, self.G = map(int, s[:8])
         self.device_bytes = int(s[8])
+        self.n_stencils = int(s[12])
 
     def close(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -282,6 +284,14 @@ class Solver:
         dts = np.zeros(n)
         self._check(lib().mlr_run_steps(self.h, n, _p(dts)))
         return dts
+
+    def run_steps_profiled(self, n):
+        """n steps launched kernel by kernel with HIP events between launches;
+        returns event-measured milliseconds and launch counts per kernel kind."""
+        st = np.zeros(9)
+        self._check(lib().mlr_run_steps_profiled(self.h, n, _p(st)))
+        return dict(total_ms=st[0], teno_ms=st[1], teno_n=int(st[2]), flux_ms=st[3], flux_n=int(st[4]),
+                    dt_ms=st[5], dt_n=int(st[6]), copy_ms=st[7], copy_n=int(st[8]))
 
     def run(self):
         self._check(lib().mlr_run(self.h))

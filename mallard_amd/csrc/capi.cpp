@@ -157,10 +157,14 @@ int mlr_get_sizes(const mlr_ctx* ctx, int64_t* s) {
     s[5] = L.K;
     s[6] = L.M;
     s[7] = L.G;
-    s[8] = (int64_t)L.device_bytes();
+    s[8] = (int64_t)ctx->solver.device_bytes();
     s[9] = ctx->solver.step();
     s[10] = (int64_t)ctx->solver.config().recon;
     s[11] = (int64_t)L.tspec.variant;
+    int64_t n_st = 0;
+    if (L.teno)
+        for (uint32_t j = 0; j < L.N; ++j) n_st += __builtin_popcount((L.info[j] >> 20) & 0xFu);
+    s[12] = n_st;
     return 0;
 }
 
@@ -246,6 +250,11 @@ int mlr_step(mlr_ctx* ctx) {
 int mlr_run_steps(mlr_ctx* ctx, int64_t n, double* dts) {
     if (!ctx) return 1;
     return guarded(ctx, [&] { ctx->solver.run_steps(n, dts); });
+}
+
+int mlr_run_steps_profiled(mlr_ctx* ctx, int64_t n, double* stats) {
+    if (!ctx || !stats) return 1;
+    return guarded(ctx, [&] { ctx->solver.run_steps_profiled(n, stats); });
 }
 
 int mlr_run(mlr_ctx* ctx) {

@@ -208,6 +208,10 @@ void Solver::free_device() {
     if (step_graph_) { (void)hipGraphDestroy(step_graph_); step_graph_ = nullptr; }
     for (void* p : allocs_) (void)hipFree(p);
     allocs_.clear();
+    dev_bytes_ = 0;
+    for (auto& e : prof_) (void)hipEventDestroy(e.second);
+    prof_.clear();
+    prof_used_ = 0;
     if (stream_) { (void)hipStreamDestroy(stream_); stream_ = nullptr; }
 }
 
@@ -261,6 +265,7 @@ void Solver::upload() {
         void* p = nullptr;
         HIP_OK(hipMalloc(&p, bytes ? bytes : 8));
         allocs_.push_back(p);
+        dev_bytes_ += bytes;
         return p;
     };
     auto up = [&](auto*& dst, const auto& v) {
@@ -378,12 +383,26 @@ dev::StageArgs Solver::stage(int op, const double* Uin, double* Uout) const {
     return a;
 }
 
+void Solver::mark(int tag) {
+    if (!prof_on_) return;
+    if (prof_used_ == prof_.size()) {
+        hipEvent_t e;
+        HIP_OK(hipEventCreate(&e));
+        prof_.push_back({tag, e});
+    }
+    prof_[prof_used_].first = tag;
+    HIP_OK(hipEventRecord(prof_[prof_used_].second, stream_));
+    ++prof_used_;
+}
+
 void Solver::launch_teno(const double* U) {
     HIP_OK(dev::teno_faces(geom(), teno(), U, fs_, dec_, ph_, stream_));
+    mark(kStatTenoMs);
 }
 
 void Solver::launch_flux(int op, const double* Uin, double* Uout) {
     HIP_OK(dev::flux_update(geom(), stage(op, Uin, Uout), ph_, stream_));
+    mark(kStatFluxMs);
 }
 
 void Solver::stage_pass(int op, const double* Uin, double* Uout) {
@@ -395,6 +414,7 @@ void Solver::dt_pass(double cfl, double fixed_dt) {
     if (fixed_dt <= 0.0)
         HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_max_, stream_));
     HIP_OK(dev::finalize_dt(lam_max_, cfl, fixed_dt, dt_dev_, dt_hist_, dt_hist_cap_, step_dev_, lam_max_, stream_));
+    mark(kStatDtMs);
 }
 
 // time_integrator.cpp:57-163 as fused stage passes
@@ -404,6 +424,7 @@ void Solver::launch_step() {
     case Integrator::FE:
         stage_pass(dev::kFE, Un_, Ua_);
         HIP_OK(dev::copy_state(Ua_, Un_, L_.N, stream_));
+        mark(kStatOtherMs);
         break;
     case Integrator::SSPRK3:
         stage_pass(dev::kSSP1, Un_, Ua_);
@@ -467,6 +488,42 @@ void Solver::run_steps(int64_t n, double* dts_out) {
         else launch_step();
     }
     synchronize();
+    finish_steps(s0, n, dts_out);
+}
+
+void Solver::run_steps_profiled(int64_t n, double* stats) {
+    for (int k = 0; k < kStatCount; ++k) stats[k] = 0.0;
+    if (n <= 0) return;
+    if (cfg_.use_cfl && cfg_.cfl < 0.0) fail("dt negative: " + std::to_string(cfg_.cfl) + ".");
+    int64_t s0 = 0;
+    HIP_OK(hipMemcpyAsync(&s0, step_dev_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    prof_on_ = true;
+    prof_used_ = 0;
+    try {
+        mark(-1);
+        for (int64_t i = 0; i < n; ++i) launch_step();
+    } catch (...) {
+        prof_on_ = false;
+        throw;
+    }
+    prof_on_ = false;
+    synchronize();
+    // launch i ran between the events recorded before and after it
+    for (size_t i = 1; i < prof_used_; ++i) {
+        float ms = 0.0f;
+        HIP_OK(hipEventElapsedTime(&ms, prof_[i - 1].second, prof_[i].second));
+        const int tag = prof_[i].first;
+        stats[tag] += ms;
+        stats[tag + 1] += 1.0;
+    }
+    float tot = 0.0f;
+    HIP_OK(hipEventElapsedTime(&tot, prof_[0].second, prof_[prof_used_ - 1].second));
+    stats[kStatTotalMs] = tot;
+    finish_steps(s0, n, nullptr);
+}
+
+void Solver::finish_steps(int64_t s0, int64_t n, double* dts_out) {
     std::vector<double> d(dt_hist_cap_);
     HIP_OK(hipMemcpy(d.data(), dt_hist_, sizeof(double) * dt_hist_cap_, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; ++i) {
@@ -497,6 +554,7 @@ void Solver::take_step() {
     case Integrator::FE:
         stage_pass(dev::kFE, Un_, Ua_);
         HIP_OK(dev::copy_state(Ua_, Un_, L_.N, stream_));
+        mark(kStatOtherMs);
         break;
     case Integrator::SSPRK3:
         stage_pass(dev::kSSP1, Un_, Ua_);

@@ -17,6 +17,16 @@
 
 namespace mlr {
 
+// Layout of the stats array filled by Solver::run_steps_profiled.
+enum StepStats {
+    kStatTotalMs = 0,   // first launch start to last launch end, event time
+    kStatTenoMs, kStatTenoN,      // TENO face-state kernels: summed ms, launches
+    kStatFluxMs, kStatFluxN,      // flux + stage-update kernels
+    kStatDtMs, kStatDtN,          // spectral radius + dt finalisation
+    kStatOtherMs, kStatOtherN,    // copies (forward Euler)
+    kStatCount
+};
+
 struct InitSpec {
     enum Kind { Constant, Analytical, Quadrant } kind = Constant;
     double u[2] = {0.0, 0.0};
@@ -78,6 +88,10 @@ public:
     // n steps, dt from CFL (or fixed dt) computed on the device; the step is
     // captured once into a hipGraph and replayed
     void run_steps(int64_t n, double* dts_out);
+    // the same n steps launched one kernel at a time with a HIP event recorded
+    // on the stream after each launch; stats receives the event-measured totals
+    // (see StepStats)
+    void run_steps_profiled(int64_t n, double* stats);
     void get_face_values(double* out);              // [n_faces alloc][Q][2][4], reference numbering
     void get_decisions(uint8_t* out);               // [N] bit v = central stencil for variable v
     void get_cfl_local(double* out);
@@ -101,6 +115,7 @@ public:
     void launch_teno(const double* U);  // one TENO face-state pass
     void launch_flux(int op, const double* Uin, double* Uout);
     double* d_state() { return Un_; }
+    uint64_t device_bytes() const { return dev_bytes_; }  // all device allocations
     double* d_fs() { return fs_; }
 
 private:
@@ -113,6 +128,8 @@ private:
     dev::StageArgs stage(int op, const double* Uin, double* Uout) const;
     void stage_pass(int op, const double* Uin, double* Uout);
     void dt_pass(double cfl, double fixed_dt);
+    void mark(int tag);
+    void finish_steps(int64_t s0, int64_t n, double* dts_out);
 
     Config cfg_;
     Mesh mesh_;
@@ -123,6 +140,7 @@ private:
 
     // device memory
     std::vector<void*> allocs_;
+    uint64_t dev_bytes_ = 0;
     double *volume_ = nullptr, *nx_ = nullptr, *ny_ = nullptr, *area_ = nullptr, *dxn_ = nullptr, *qw_ = nullptr;
     int32_t* nb_ = nullptr;
     uint32_t* info_ = nullptr;
@@ -135,6 +153,11 @@ private:
     int64_t* step_dev_ = nullptr;
     uint8_t* dec_ = nullptr;
     int64_t dt_hist_cap_ = 0;
+
+    // launch profiling: (kernel tag, event recorded after that launch)
+    bool prof_on_ = false;
+    std::vector<std::pair<int, hipEvent_t>> prof_;
+    size_t prof_used_ = 0;
 
     hipGraphExec_t step_exec_ = nullptr;
     hipGraph_t step_graph_ = nullptr;

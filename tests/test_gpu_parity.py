@@ -163,8 +163,9 @@ def test_teno_corrected_100_steps(mlr, oracle):
 
 
 def test_teno_smooth_field_steps(mlr, oracle):
-    """Smooth field: corrected variant for 40 steps; the reference weighting
-    diverges (NaN) within the first step on both sides."""
+    """Smooth field: corrected variant for 40 steps. With the reference
+    weighting the first residual is finite and must match, but the oracle's
+    first SSP-RK3 step already ends non-finite, so no steps are compared."""
     bcs = [(z, "extrapolation", {}) for z in ("left", "right", "top", "bottom")]
     for variant, n in (("corrected", 40), ("reference", 1)):
         dev, orc, m = pair(mlr, oracle, "cartesian_tri", 24, 24, jitter=0.1, seed=5, recon="TENO",
@@ -181,7 +182,6 @@ def test_teno_smooth_field_steps(mlr, oracle):
         if variant == "reference":
             r1, r2 = dev.calc_rhs(), orc.calc_rhs(U)
             assert np.array_equal(np.isnan(r1), np.isnan(r2))
-            assert np.isnan(r2).any()
             fin = np.isfinite(r2)
             assert np.allclose(r1[fin], r2[fin], rtol=1e-12, atol=1e-12 * np.abs(r2[fin]).max())
             continue
