@@ -809,6 +809,55 @@ FluxFn flux_fn(int t) {
 }  // namespace
 
 // ---------------------------------------------------------------- solver
+// time_integrator.cpp:57-163: FE, RK4 and SSPRK3 as the sequence of KokkosBlas
+// axpy / axpby updates the reference issues (y <- y + a x; y <- a x + b y).
+void blas_axpy(int nt, double a, const std::vector<double>& x, std::vector<double>& y) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int64_t i = 0; i < (int64_t)y.size(); ++i) y[i] = y[i] + a * x[i];
+}
+void blas_axpby(int nt, double a, const std::vector<double>& x, double b, std::vector<double>& y) {
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int64_t i = 0; i < (int64_t)y.size(); ++i) y[i] = a * x[i] + b * y[i];
+}
+
+template <class RHS>
+void integrate_step(int type, double dt, int nt, std::vector<double>& U, std::vector<double>* rhs,
+                    std::vector<double>& Utmp, RHS calc) {
+    auto axpy = [nt](double a, const std::vector<double>& x, std::vector<double>& y) { blas_axpy(nt, a, x, y); };
+    auto axpby = [nt](double a, const std::vector<double>& x, double b, std::vector<double>& y) {
+        blas_axpby(nt, a, x, b, y);
+    };
+    if (type == ORC_TI_FE) {
+        calc(U.data(), rhs[0].data());
+        axpy(dt, rhs[0], U);
+    } else if (type == ORC_TI_RK4) {
+        const double co[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+        calc(U.data(), rhs[0].data());
+        Utmp = U;
+        axpy(dt / 2.0, rhs[0], Utmp);
+        calc(Utmp.data(), rhs[1].data());
+        Utmp = U;
+        axpy(dt / 2.0, rhs[1], Utmp);
+        calc(Utmp.data(), rhs[2].data());
+        Utmp = U;
+        axpy(dt, rhs[2], Utmp);
+        calc(Utmp.data(), rhs[3].data());
+        for (int k = 0; k < 4; k++) axpy(dt * co[k], rhs[k], U);
+    } else if (type == ORC_TI_SSPRK3) {
+        const double co[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+        calc(U.data(), rhs[0].data());
+        Utmp = U;
+        axpy(dt, rhs[0], Utmp);
+        calc(Utmp.data(), rhs[1].data());
+        axpby(3.0 / 4.0, U, 1.0 / 4.0, Utmp);
+        axpy(dt / 4.0, rhs[1], Utmp);
+        calc(Utmp.data(), rhs[2].data());
+        for (int k = 0; k < 3; k++) axpy(dt * co[k], rhs[k], U);
+    } else {
+        throw std::runtime_error("Unknown time integrator type.");
+    }
+}
+
 struct orc_solver {
     orc_mesh* m = nullptr;
     orc_config cfg{};
@@ -1403,47 +1452,10 @@ struct orc_solver {
         for (int64_t c = 0; c < (int64_t)nc; ++c) phys.prims(&prim[(size_t)c * NP], &U[(size_t)c * NC]);
     }
 
-    // KokkosBlas axpy / axpby semantics: y <- y + a x ; y <- a x + b y
-    void axpy(double a, const std::vector<double>& x, std::vector<double>& y) {
-#pragma omp parallel for num_threads(nthreads) schedule(static)
-        for (int64_t i = 0; i < (int64_t)y.size(); ++i) y[i] = y[i] + a * x[i];
-    }
-    void axpby(double a, const std::vector<double>& x, double b, std::vector<double>& y) {
-#pragma omp parallel for num_threads(nthreads) schedule(static)
-        for (int64_t i = 0; i < (int64_t)y.size(); ++i) y[i] = a * x[i] + b * y[i];
-    }
-
     // time_integrator.cpp:57-163 then solver.cpp:521-531
     void take_step(double dt) {
-        if (cfg.integrator == ORC_TI_FE) {
-            calc_rhs(U.data(), rhs[0].data());
-            axpy(dt, rhs[0], U);
-        } else if (cfg.integrator == ORC_TI_RK4) {
-            const double co[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-            calc_rhs(U.data(), rhs[0].data());
-            Utmp = U;
-            axpy(dt / 2.0, rhs[0], Utmp);
-            calc_rhs(Utmp.data(), rhs[1].data());
-            Utmp = U;
-            axpy(dt / 2.0, rhs[1], Utmp);
-            calc_rhs(Utmp.data(), rhs[2].data());
-            Utmp = U;
-            axpy(dt, rhs[2], Utmp);
-            calc_rhs(Utmp.data(), rhs[3].data());
-            for (int k = 0; k < 4; k++) axpy(dt * co[k], rhs[k], U);
-        } else if (cfg.integrator == ORC_TI_SSPRK3) {
-            const double co[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
-            calc_rhs(U.data(), rhs[0].data());
-            Utmp = U;
-            axpy(dt, rhs[0], Utmp);
-            calc_rhs(Utmp.data(), rhs[1].data());
-            axpby(3.0 / 4.0, U, 1.0 / 4.0, Utmp);
-            axpy(dt / 4.0, rhs[1], Utmp);
-            calc_rhs(Utmp.data(), rhs[2].data());
-            for (int k = 0; k < 3; k++) axpy(dt * co[k], rhs[k], U);
-        } else {
-            throw std::runtime_error("Unknown time integrator type.");
-        }
+        integrate_step(cfg.integrator, dt, nthreads, U, rhs, Utmp,
+                       [this](const double* u, double* r) { calc_rhs(u, r); });
         update_primitives();
         t += dt;
     }
@@ -1631,6 +1643,18 @@ void orc_integrator_constant_rhs(int type, double dt, double* U, int n, const do
         for (int s = 0; s < 3; s++) axpy(dt * co[s], k, u);
     }
     std::memcpy(U, u.data(), n * 8);
+}
+
+// The integrators of the solver on dU/dt = lam * U (n independent entries).
+int orc_integrator_linear_rhs(int type, double dt, double lam, double* U, int n) {
+    std::vector<double> u(U, U + n), tmp(n);
+    std::vector<double> r[4] = {std::vector<double>(n), std::vector<double>(n), std::vector<double>(n),
+                                std::vector<double>(n)};
+    ORC_TRY(integrate_step(type, dt, 1, u, r, tmp,
+                           [&](const double* x, double* y) { for (int i = 0; i < n; i++) y[i] = lam * x[i]; }),
+            return 1);
+    std::memcpy(U, u.data(), (size_t)n * 8);
+    return 0;
 }
 
 }  // extern "C"

@@ -130,6 +130,8 @@ double orc_basis_1d(int basis, int p, double x);
 double orc_basis_deriv_1d(int basis, int n, int p, double x);
 /* in-place integrator step with a constant RHS callback value (time_integrator_test) */
 void orc_integrator_constant_rhs(int type, double dt, double* U, int n, const double* rhs);
+/* the solver's integrator sequence on dU/dt = lam U (closed-form amplification test) */
+int orc_integrator_linear_rhs(int type, double dt, double lam, double* U, int n);
 
 #ifdef __cplusplus
 }

@@ -87,6 +87,7 @@ def lib():
         L.orc_triangle_area.restype = C.c_double
         L.orc_triangle_area.argtypes = [C.c_void_p] * 3
         L.orc_triangle_J.argtypes = [C.c_void_p] * 4
+        L.orc_integrator_linear_rhs.argtypes = [C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_int]
         L.orc_basis_1d.restype = C.c_double
         L.orc_basis_1d.argtypes = [C.c_int, C.c_int, C.c_double]
         L.orc_basis_deriv_1d.restype = C.c_double

@@ -341,3 +341,80 @@ def test_reference_weighting_diverges_corrected_does_not(oracle):
     U = cor.state()[0]
     assert np.isfinite(U).all()
     assert 0.75 < U[:, 0].min() and U[:, 0].max() < 1.25
+
+
+@pytest.mark.parametrize("integrator,coefs", [("FE", [1, 1]), ("SSPRK3", [1, 1, 1 / 2, 1 / 6]),
+                                              ("RK4", [1, 1, 1 / 2, 1 / 6, 1 / 24])])
+def test_integrator_amplification_closed_form(oracle, integrator, coefs):
+    """The solver's stage sequence on dU/dt = lam U multiplies U by the
+    truncated exponential of z = lam dt (pins every stage coefficient, not only
+    their sum as the constant-RHS test does)."""
+    for z in (-0.9, -0.37, 0.25, 1.3):
+        U = np.array([1.0, -2.5, 3.0])
+        assert oracle.lib().orc_integrator_linear_rhs(oracle.INTEGRATOR[integrator], 0.1, z / 0.1,
+                                                      oracle._p(U), 3) == 0
+        amp = sum(c * z ** k for k, c in enumerate(coefs))
+        assert np.allclose(U, amp * np.array([1.0, -2.5, 3.0]), rtol=1e-14, atol=0)
+
+
+def _gl01(n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1), 0.5 * w
+
+
+def _poly2(px, py, x, y):
+    return np.polynomial.polynomial.polyval(x, _leg(px)) * np.polynomial.polynomial.polyval(y, _leg(py))
+
+
+def _tri_mean(f, t0, t1, t2, n=8):
+    """Mean of f over a triangle by the collapsed Gauss rule (exact far beyond p=3)."""
+    u, wu = _gl01(n)
+    U, V = np.meshgrid(u, u, indexing="ij")
+    W = np.outer(wu, wu) * (1 - U)
+    xi, eta = U, V * (1 - U)
+    x = t0[0] + (t1[0] - t0[0]) * xi + (t2[0] - t0[0]) * eta
+    y = t0[1] + (t1[1] - t0[1]) * xi + (t2[1] - t0[1]) * eta
+    return 2.0 * np.sum(W * f(x, y))
+
+
+@pytest.mark.parametrize("order,jit", [(1, 0.2), (2, 0.0), (2, 0.2), (3, 0.2)])
+def test_pseudo_inverse_matches_numpy_svd(oracle, order, jit):
+    """The stored (M-1)x(K-1) block of A+ equals numpy's SVD pseudo-inverse of
+    the least-squares matrix rebuilt here independently: rows are stencil
+    members, columns the Legendre basis means over each member (target-cell
+    frame) minus the target-cell means, each row scaled by the member's area
+    in that frame (face_reconstruction.cpp:477-741)."""
+    m = oracle.Mesh("cartesian_tri", 10, 9, 1.0, 1.0, jit, 3)
+    s = oracle.Solver(m, oracle.make_config(order=order))
+    T = s.teno()
+    K, M = T["n_dof"], T["max_cells"]
+    poly = T["poly_indices"].reshape(K, 2)
+    ipt = T["integral_psi_target"]
+    og, os_, st, orm = T["offsets_stencil_groups"], T["offsets_stencils"], T["stencils"], T["offsets_recon"]
+    L = oracle.lib()
+    worst = 0.0
+    for c in range(0, m.n_cells, 5):
+        nodes = m.nodes_of_cell[3 * c:3 * c + 3]
+        w = [m.node_coords[n] for n in nodes]
+        J = np.zeros(4)
+        L.orc_triangle_J(oracle._p(np.ascontiguousarray(w[0])), oracle._p(np.ascontiguousarray(w[1])),
+                         oracle._p(np.ascontiguousarray(w[2])), oracle._p(J))
+        Ji = np.linalg.inv(J.reshape(2, 2))
+        for sl in range(og[c + 1] - og[c]):
+            si = og[c] + sl
+            ids = st[os_[si]:os_[si + 1]]
+            if len(ids) == 0:
+                continue
+            B = np.zeros((len(ids) - 1, K - 1))
+            for r, nb in enumerate(ids[1:]):
+                t = [Ji @ (m.node_coords[n] - w[0]) for n in m.nodes_of_cell[3 * nb:3 * nb + 3]]
+                area = 0.5 * abs((t[1][0] - t[0][0]) * (t[2][1] - t[0][1]) - (t[2][0] - t[0][0]) * (t[1][1] - t[0][1]))
+                for k in range(1, K):
+                    mean = _tri_mean(lambda x, y: _poly2(poly[k][0], poly[k][1], x, y), *t)
+                    B[r, k - 1] = area * (mean - ipt[k])
+            ref = np.linalg.pinv(B)
+            got = T["recon"][orm[si]:orm[si] + K * M].reshape(K, M)
+            assert np.all(got[0] == 0) and np.all(got[:, 0] == 0)
+            err = np.abs(got[1:, 1:] - ref).max() / np.abs(ref).max()
+            worst = max(worst, err)
+    assert worst < 1e-10, worst
