@@ -53,19 +53,20 @@ def numerics_kw(args):
                 teno_variant=args.variant, riemann="HLLC", integrator=args.integrator, cfl=0.4, bcs=bcs)
 
 
-def teno_bytes(n_cells, n_stencils, K, M, S, Q):
+def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes):
     """Algorithmic HBM bytes of one TENO face-state launch.
 
     alg: each compulsory element counted once, per non-empty stencil the
     reference's K x M pseudo-inverse (8 B), M ids (4 B) and M transformed
     areas (8 B), plus the cell's own state (32 B).
     layout: what this build's packed layout must move at least: the
-    (K-1) x (M-1) non-zero block of the pseudo-inverse, M-1 ids and areas,
-    the cell state, its info word, the face-point basis values, the face
-    states written (S*Q points x 4 variables) and the decision byte.
+    (K-1) x (M-1) non-zero block of the pseudo-inverse, M-1 areas and in-tile
+    positions, the tile lists (4 B per entry), the cell state, its info word,
+    the face-point basis values, the face states written (S*Q points x 4
+    variables) and the decision byte.
     """
     alg = n_stencils * (8 * K * M + 12 * M) + 32 * n_cells
-    layout = (n_stencils * (8 * (K - 1) * (M - 1) + 12 * (M - 1)) +
+    layout = (n_stencils * (8 * (K - 1) * (M - 1) + (8 + pos_bytes) * (M - 1)) + 4 * tile_entries +
               n_cells * (32 + 4 + 8 * S * Q * (K - 1) + 8 * S * Q * 4 + 1))
     return alg, layout
 
@@ -164,7 +165,7 @@ def main():
 
     teno_ms = st["teno_ms"] / max(st["teno_n"], 1)
     flux_ms = st["flux_ms"] / max(st["flux_n"], 1)
-    alg, lay = teno_bytes(N, dev.n_stencils, dev.K, dev.M, dev.S, dev.Q)
+    alg, lay = teno_bytes(N, dev.n_stencils, dev.K, dev.M, dev.S, dev.Q, dev.tile_entries, dev.tile_pos_bytes)
     achieved = alg / (teno_ms * 1e-3) / 1e9
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # summary, used only when it was measured on this workload and on the
@@ -178,6 +179,11 @@ def main():
             traffic = tr["bytes_per_launch"]
 
     stages = {"SSPRK3": 3, "FE": 1}[args.integrator]
+    # what HBM delivers to plain 16-byte streaming on this GPU (2 GiB buffers)
+    nb = 2 << 30
+    stream_copy = 2 * nb / (mlr.probe_stream(0, nb, 10) * 1e-3) / 1e9
+    stream_read = nb / (mlr.probe_stream(1, nb, 10) * 1e-3) / 1e9
+
     value = N * stages * args.steps / wall
     out = {
         "metric": METRIC,
@@ -212,6 +218,8 @@ def main():
             "layout_bytes_per_launch": lay,
             "layout_GBs": lay / (teno_ms * 1e-3) / 1e9,
             "avg_launch_ms": teno_ms,
+            "stream_copy_GBs": stream_copy,
+            "stream_read_GBs": stream_read,
         },
         "kernels_ms": {"teno": teno_ms, "flux_update": flux_ms,
                        "spectral_dt": st["dt_ms"] / max(st["dt_n"], 1),

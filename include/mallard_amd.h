@@ -109,10 +109,11 @@ int mlr_ctx_create(const mlr_mesh_desc* mesh, const mlr_numerics_desc* num, mlr_
 int mlr_ctx_destroy(mlr_ctx* ctx);
 const char* mlr_last_error(const mlr_ctx* ctx);
 
-/* sizes[0..12] = n_cells, n_faces (allocated), n_faces_real, face slots per
+/* sizes[0..14] = n_cells, n_faces (allocated), n_faces_real, face slots per
  * cell, face quadrature points, n_dof, cells per stencil, stencils per cell,
- * device bytes allocated by the context, step, recon, teno_variant, non-empty TENO
- * stencils over all cells */
+ * device bytes allocated by the context, step, recon, teno_variant, non-empty
+ * TENO stencils over all cells, TENO tile-list entries (all 64-cell blocks),
+ * bytes per in-tile member position */
 int mlr_get_sizes(const mlr_ctx* ctx, int64_t* sizes);
 
 /* Initial states (solver_initialize.cpp:35-186). states[16] = (rho,u,v,p)
@@ -187,6 +188,10 @@ int mlr_get_device_stencils(const mlr_ctx* ctx, uint32_t cell, uint32_t* ids);
 typedef struct mlr_layout mlr_layout;
 int mlr_layout_create(const mlr_mesh_gen* gen, const mlr_numerics_desc* num, mlr_layout** out);
 int mlr_layout_destroy(mlr_layout* lay);
+/* sizes[0..13] = n_cells, n_faces (allocated), n_faces_real, face slots per
+ * cell, face quadrature points, n_dof, cells per stencil, stencils per cell,
+ * device bytes of the packed arrays, n_nodes, nodes_of_cell entries, zones,
+ * longest 64-cell tile list, bytes per in-tile position (1 or 2) */
 int mlr_layout_sizes(const mlr_layout* lay, int64_t* sizes);
 int mlr_layout_permutation(const mlr_layout* lay, uint32_t* perm);
 int mlr_layout_mesh_arrays(const mlr_layout* lay, double* node_coords, double* cell_coords,
@@ -204,6 +209,11 @@ int mlr_layout_slots(const mlr_layout* lay, uint32_t cell, uint32_t* nb, uint32_
  * double-double pow of the two-rarefaction estimate, 3 x^6, 4 the platform
  * pow. */
 int mlr_probe_math(int which, const double* x, const double* y, double* out, int n);
+/* Streaming bandwidth probe (no reference counterpart; context for the
+ * roofline figures): kind 0 copies `bytes`, kind 1 reads them, with 16-byte
+ * loads per lane over a grid-stride loop; ms_avg = mean duration of `reps`
+ * launches after one warm-up, by HIP events. */
+int mlr_probe_stream(int kind, int64_t bytes, int reps, double* ms_avg);
 /* Host builds of the same double-double pow and correctly rounded x^6. */
 int mlr_host_pow(const double* x, const double* y, double* out, int n);
 int mlr_host_pow6(const double* x, double* out, int n);

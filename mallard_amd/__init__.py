@@ -83,6 +83,7 @@ _SIGS = {
     "mlr_layout_teno_tables": (C.c_int, [_P, _P, _P]),
     "mlr_layout_slots": (C.c_int, [_P, C.c_uint32] + [_P] * 6),
     "mlr_probe_math": (C.c_int, [C.c_int, _P, _P, _P, C.c_int]),
+    "mlr_probe_stream": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "mlr_host_pow": (C.c_int, [_P, _P, _P, C.c_int]),
     "mlr_host_pow6": (C.c_int, [_P, _P, C.c_int]),
 }
@@ -163,10 +164,10 @@ class Layout:
         if L.mlr_layout_create(C.byref(gen), C.byref(num), C.byref(h)):
             raise MallardError(L.mlr_last_error(None).decode())
         self.h = h
-        s = np.zeros(12, np.int64)
+        s = np.zeros(14, np.int64)
         L.mlr_layout_sizes(self.h, _p(s))
         (self.n_cells, self.n_faces, self.n_faces_real, self.S, self.Q, self.K, self.M, self.G,
-         self.device_bytes, self.n_nodes, self.n_noc, self.n_zones) = map(int, s)
+         self.device_bytes, self.n_nodes, self.n_noc, self.n_zones, self.tile_max, self.tile_pos_bytes) = map(int, s)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -231,11 +232,12 @@ class Solver:
         if rc:
             raise MallardError(L.mlr_last_error(None).decode())
         self.h = h
-        s = np.zeros(13, np.int64)
+        s = np.zeros(15, np.int64)
         L.mlr_get_sizes(self.h, _p(s))
         self.n_cells, self.n_faces, self.n_faces_real, self.S, self.Q, self.K, self.M, self.G = map(int, s[:8])
         self.device_bytes = int(s[8])
         self.n_stencils = int(s[12])
+        self.tile_entries, self.tile_pos_bytes = int(s[13]), int(s[14])
 
     def close(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -328,6 +330,14 @@ class Solver:
         ids = np.zeros(self.G * self.M, np.uint32)
         self._check(lib().mlr_get_device_stencils(self.h, cell, _p(ids)))
         return ids.reshape(self.G, self.M)
+
+
+def probe_stream(kind, nbytes, reps=10):
+    """Mean ms of a streaming copy (kind 0) or read (kind 1) of nbytes on the GPU."""
+    ms = C.c_double()
+    if lib().mlr_probe_stream(kind, int(nbytes), reps, C.byref(ms)):
+        raise MallardError(lib().mlr_last_error(None).decode())
+    return ms.value
 
 
 def probe_math(which, x, y=None):

@@ -165,6 +165,8 @@ int mlr_get_sizes(const mlr_ctx* ctx, int64_t* s) {
     if (L.teno)
         for (uint32_t j = 0; j < L.N; ++j) n_st += __builtin_popcount((L.info[j] >> 20) & 0xFu);
     s[12] = n_st;
+    s[13] = L.teno ? (int64_t)L.blk_off.back() : 0;       // tile list entries
+    s[14] = L.teno ? (ctx->solver.tile_pos_bytes()) : 0;  // bytes per in-tile position
     return 0;
 }
 
@@ -373,6 +375,8 @@ int mlr_layout_sizes(const mlr_layout* lay, int64_t* s) {
     s[9] = lay->mesh.n_nodes;
     s[10] = (int64_t)lay->mesh.nodes_of_cell.size();
     s[11] = (int64_t)lay->mesh.zones.size();
+    s[12] = L.tile_max;
+    s[13] = L.st_loc16.empty() ? 1 : 2;
     return 0;
 }
 
@@ -401,11 +405,18 @@ int mlr_layout_cell_stencils(const mlr_layout* lay, uint32_t cell, uint32_t* ids
         if (cell >= L.N) mlr::fail("cell out of range");
         const uint32_t j = L.iperm[cell], N = L.N;
         const int Mp = L.M - 1;
+        // read back through the uploaded form: the block's tile list at the
+        // member's stored position
+        const uint32_t b = j / mlr::Layout::kTileCells;
         for (int s = 0; s < L.G; ++s) {
             const bool on = (L.info[j] >> (20 + s)) & 1u;
             ids[(size_t)s * L.M] = on ? cell : 0xFFFFFFFFu;
-            for (int i = 0; i < Mp; ++i)
-                ids[(size_t)s * L.M + 1 + i] = on ? L.perm[L.st_ids[((size_t)s * Mp + i) * N + j]] : 0xFFFFFFFFu;
+            for (int i = 0; i < Mp; ++i) {
+                const size_t o = L.mem_idx(s, i, j);
+                const uint32_t pos = L.st_loc8.empty() ? L.st_loc16[o] : L.st_loc8[o];
+                if (on && L.blk_off[b] + pos >= L.blk_off[b + 1]) mlr::fail("tile position out of range");
+                ids[(size_t)s * L.M + 1 + i] = on ? L.perm[L.blk_ids[L.blk_off[b] + pos]] : 0xFFFFFFFFu;
+            }
         }
     });
 }
@@ -441,18 +452,18 @@ int mlr_layout_cell_matrices(const mlr_layout* lay, uint32_t cell, double* aplus
             for (int k = 0; k < K; ++k)
                 for (int i = 0; i < M; ++i) {
                     double v = 0.0;
-                    if (on && k > 0 && i > 0) v = L.st_A[(((size_t)s * Mp + (i - 1)) * Kp + (k - 1)) * N + j];
+                    if (on && k > 0 && i > 0) v = L.st_A[L.a_idx(s, i - 1, k - 1, j)];
                     if (aplus) aplus[((size_t)s * K + k) * M + i] = v;
                 }
             if (ta)
                 for (int i = 0; i < M; ++i)
-                    ta[(size_t)s * M + i] = !on ? 0.0 : (i == 0 ? ta0 : L.st_ta[((size_t)s * Mp + (i - 1)) * N + j]);
+                    ta[(size_t)s * M + i] = !on ? 0.0 : (i == 0 ? ta0 : L.st_ta[L.mem_idx(s, i - 1, j)]);
         }
         if (fbasis)
             for (uint32_t l = 0; l < L.S; ++l)
                 for (uint32_t q = 0; q < L.Q; ++q)
                     for (int k = 0; k < Kp; ++k)
-                        fbasis[((size_t)l * L.Q + q) * Kp + k] = L.fbasis[(((size_t)l * L.Q + q) * Kp + k) * N + j];
+                        fbasis[((size_t)l * L.Q + q) * Kp + k] = L.fbasis[L.fb_idx(l * L.Q + q, k, j)];
     });
 }
 
@@ -495,6 +506,36 @@ int mlr_probe_math(int which, const double* x, const double* y, double* out, int
         (void)hipFree(dx);
         (void)hipFree(dy);
         (void)hipFree(dout);
+    });
+}
+
+int mlr_probe_stream(int kind, int64_t bytes, int reps, double* ms_avg) {
+    if (!ms_avg || bytes < 4096 || reps < 1 || (kind != 0 && kind != 1)) return 1;
+    return guarded(nullptr, [&] {
+        struct Buf {
+            void* p = nullptr;
+            ~Buf() { if (p) (void)hipFree(p); }
+        } src, dst;
+        struct Ev {
+            hipEvent_t e = nullptr;
+            ~Ev() { if (e) (void)hipEventDestroy(e); }
+        } e0, e1;
+        auto chk = [](hipError_t e) { if (e != hipSuccess) mlr::fail(hipGetErrorString(e)); };
+        const size_t n = (size_t)bytes / 16 * 16;
+        const size_t dbytes = kind == 0 ? n : (size_t)mlr::dev::stream_probe_threads(n) * 8;
+        chk(hipMalloc(&src.p, n));
+        chk(hipMalloc(&dst.p, dbytes));
+        chk(hipMemset(src.p, 0, n));
+        chk(hipEventCreate(&e0.e));
+        chk(hipEventCreate(&e1.e));
+        chk(mlr::dev::stream_probe(kind, src.p, dst.p, n, nullptr));  // warm-up
+        chk(hipEventRecord(e0.e, nullptr));
+        for (int r = 0; r < reps; ++r) chk(mlr::dev::stream_probe(kind, src.p, dst.p, n, nullptr));
+        chk(hipEventRecord(e1.e, nullptr));
+        chk(hipEventSynchronize(e1.e));
+        float ms = 0.0f;
+        chk(hipEventElapsedTime(&ms, e0.e, e1.e));
+        *ms_avg = ms / reps;
     });
 }
 

@@ -34,7 +34,11 @@ struct Geom {
 struct Teno {
     int32_t K, M, variant;
     double basis0;
-    const int32_t* ids;
+    // stencil members as positions in the 64-cell block's tile list
+    const uint8_t* loc8;       // [G][M-1][N] when every tile has <= 256 cells
+    const uint16_t* loc16;     // [G][M-1][N] otherwise (<= 1024 cells)
+    const uint32_t* blk_off;   // [n_blocks + 1]
+    const int32_t* blk_ids;    // tile lists, ascending cell ids
     const double* ta;
     const double* A;
     const double* fbasis;
@@ -81,5 +85,9 @@ hipError_t finalize_dt(const unsigned long long* lam_max_in, double cfl, double 
                        hipStream_t stream);
 hipError_t copy_state(const double* src, double* dst, uint32_t N, hipStream_t stream);
 hipError_t probe_math(const double* x, const double* y, double* out, int n, int which, hipStream_t stream);
+// Streaming bandwidth probes: kind 0 copies `bytes` from src to dst, kind 1
+// reads them (dst receives stream_probe_threads(bytes) doubles).
+uint32_t stream_probe_threads(size_t bytes);
+hipError_t stream_probe(int kind, const void* src, void* dst, size_t bytes, hipStream_t stream);
 
 }  // namespace mlr::dev

@@ -20,7 +20,6 @@ namespace mlr::dev {
 namespace {
 
 constexpr uint32_t kBoundaryBit = 1u << 3;
-constexpr int kMaxKp = 9;
 
 
 struct D4 {
@@ -262,29 +261,84 @@ __device__ inline double pressure_raw(double gamma, const double* u) {
     return (gamma - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0]);
 }
 
+// ------------------------------------------------------------ quad exchange
+// The 4 lanes of a quad serve one cell. DPP quad permutations move values
+// between them in registers (no LDS, so occupancy is set by VGPRs alone).
+template <int CTRL>
+__device__ inline double qperm(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// quad_perm controls: lane j of a quad reads lane (j - r) & 3, r = 1, 2, 3
+constexpr int kRot1 = 0x93, kRot2 = 0x4E, kRot3 = 0x39;
+
+__device__ inline double sel4(const double* x, int i) {
+    return i == 0 ? x[0] : (i == 1 ? x[1] : (i == 2 ? x[2] : x[3]));
+}
+// 4 x 4 transpose across a quad: out[j] = x[q] of quad lane j (q = own lane).
+__device__ inline void quad_transpose(const double* x, double* out, int q) {
+    double r[4];
+    r[0] = sel4(x, q);
+    r[1] = qperm<kRot1>(sel4(x, (q + 1) & 3));
+    r[2] = qperm<kRot2>(sel4(x, (q + 2) & 3));
+    r[3] = qperm<kRot3>(sel4(x, (q + 3) & 3));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[j] = sel4(r, (q - j) & 3);
+}
+// out[j] = f of quad lane j.
+__device__ inline void quad_gather(double f, double* out, int q) {
+    double r[4];
+    r[0] = f;
+    r[1] = qperm<kRot1>(f);
+    r[2] = qperm<kRot2>(f);
+    r[3] = qperm<kRot3>(f);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[j] = sel4(r, (q - j) & 3);
+}
+
 // ------------------------------------------------------------ TENO kernel
-// 16 cells per 64-lane wave. Phase A: lane s of a quad owns stencil s
-// (coalesced stencil-major loads, a = A+ b, smoothness, raw weight).
-// Phase B: after an LDS exchange lane v owns conserved variable v (cut-off,
-// weights, face-point sums in the reference's stencil-then-dof order).
-template <int KP>
-__global__ __launch_bounds__(64) void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
-                                                  double* __restrict__ fs, uint8_t* __restrict__ dec,
-                                                  Physics ph) {
-    __shared__ double sh_a[16][4][KP][4];
-    __shared__ double sh_w[16][4][4];
-    __shared__ double sh_f[16][8][4];
-    __shared__ uint32_t sh_d[16][4];
-    const int lc = threadIdx.x >> 2, s = threadIdx.x & 3;
+// 64 cells per 256-thread block, 4 lanes (a quad) per cell.
+// Phase A: quad lane s owns stencil s: stencil-major loads of ids, areas and
+// A+, gathers of the members' states, a = A+ b, smoothness, raw weight.
+// Quad transposes then hand lane v the 4 stencils' coefficients and weights
+// of conserved variable v. Phase B: cut-off, weights and the face-point sums
+// in the reference's stencil-then-dof order; with the corrected weighting the
+// quad then scales its face states toward the cell mean for positivity.
+// KP = n_dof - 1, MP = members per stencil after the target (0: runtime),
+// FQ = face points per cell; IdxT is the width of the in-tile positions and
+// TILE the LDS capacity in cells. The block first stages the states of every
+// cell its stencils read (the tile list) in LDS, so each neighbour state is
+// fetched once per block instead of once per stencil member. Blocks are
+// mapped so that each of the 8 XCDs walks one contiguous range of the
+// space-filling-curve order.
+template <int KP, int MP, int FQ, typename IdxT, int TILE>
+__global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
+                                                   double* __restrict__ fs, uint8_t* __restrict__ dec,
+                                                   Physics ph, uint32_t blocks_per_xcd) {
+    __shared__ double2 sh_u[2 * TILE];
+    const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
     const uint32_t N = g.N;
-    const uint32_t c = blockIdx.x * 16 + lc;
+    if (lb * 64u >= N) return;  // whole block past the end (uniform)
+    {
+        const uint32_t t0 = t.blk_off[lb], nloc = t.blk_off[lb + 1] - t0;
+        const double2* U2 = reinterpret_cast<const double2*>(U);
+        for (uint32_t j = threadIdx.x; j < 2 * nloc; j += blockDim.x)
+            sh_u[j] = U2[2 * (size_t)t.blk_ids[t0 + (j >> 1)] + (j & 1u)];
+    }
+    __syncthreads();
+    const IdxT* __restrict__ loc = sizeof(IdxT) == 1 ? reinterpret_cast<const IdxT*>(t.loc8)
+                                                     : reinterpret_cast<const IdxT*>(t.loc16);
+    const int q = threadIdx.x & 3;
+    const uint32_t c = lb * 64u + (threadIdx.x >> 2);
     const bool valid = c < N;
     const uint32_t cc = valid ? c : 0;
-    const int Mp = t.M - 1;
+    const int Mp = MP > 0 ? MP : t.M - 1;
     const uint32_t info0 = g.info[cc];
-    const bool active = valid && ((info0 >> (20 + s)) & 1u);
+    const bool active = valid && ((info0 >> (20 + q)) & 1u);
     const D4 uc = ld4(U, cc);
 
+    // ---- phase A: lane q = stencil
     double a[KP][4];
 #pragma unroll
     for (int k = 0; k < KP; ++k)
@@ -292,22 +346,31 @@ __global__ __launch_bounds__(64) void teno_kernel(Geom g, Teno t, const double* 
         for (int v = 0; v < 4; ++v) a[k][v] = 0.0;
     double w[4] = {0.0, 0.0, 0.0, 0.0};
     if (active) {
-        const size_t sN = (size_t)s * Mp;
-        for (int i = 0; i < Mp; ++i) {
-            const size_t o = (sN + i) * N + c;
-            const int32_t id = t.ids[o];
+        // blocked tables: this block's rows start at (lb * 4 + q) * Mp
+        const size_t row0 = ((size_t)lb * 4 + q) * Mp;
+        const uint32_t cl = c & 63u;
+        auto member = [&](int i) {
+            const size_t o = (row0 + i) * 64 + cl;
+            const uint32_t pos = loc[o];
             const double tt = t.ta[o];
-            const D4 un = ld4(U, (size_t)id);
+            const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
+            const D4 un = {{u01.x, u01.y, u23.x, u23.y}};
             double b[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) b[v] = tt * (un.v[v] - uc.v[v]);
-            const double* Ai = t.A + ((sN + i) * KP) * N + c;
+            const double* Ai = t.A + (row0 + i) * (KP * 64) + cl;
 #pragma unroll
             for (int k = 0; k < KP; ++k) {
-                const double Ak = Ai[(size_t)k * N];
+                const double Ak = Ai[k * 64];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) a[k][v] = a[k][v] + Ak * b[v];
             }
+        };
+        if constexpr (MP > 0) {
+#pragma unroll
+            for (int i = 0; i < MP; ++i) member(i);
+        } else {
+            for (int i = 0; i < Mp; ++i) member(i);
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -322,27 +385,29 @@ __global__ __launch_bounds__(64) void teno_kernel(Geom g, Teno t, const double* 
             w[v] = 1.0 / pow6_cr(si + 1.0e-12);
         }
     }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        sh_w[lc][s][v] = w[v];
-#pragma unroll
-        for (int k = 0; k < KP; ++k) sh_a[lc][s][k][v] = a[k][v];
-    }
-    __syncthreads();
 
-    // ---- phase B: lane v = s
-    const int v = s;
-    const int G = 4;
+    // ---- exchange: lane q = variable
     double ws[4];
+    quad_transpose(w, ws, q);
+    double at[4][KP];
 #pragma unroll
-    for (int st = 0; st < G; ++st) ws[st] = sh_w[lc][st][v];
+    for (int k = 0; k < KP; ++k) {
+        double x[4] = {a[k][0], a[k][1], a[k][2], a[k][3]}, y[4];
+        quad_transpose(x, y, q);
+#pragma unroll
+        for (int st = 0; st < 4; ++st) at[st][k] = y[st];
+    }
+
+    // ---- phase B: cut-off (face_reconstruction.cpp:948-981)
+    const int v = q;
+    constexpr int G = 4;
     const double CT1 = 1.0e-7, CT2 = 1.0e-5;
     double sd = 0.0;
 #pragma unroll
     for (int st = 1; st < G; ++st) sd += ws[st];
-    uint32_t central = 0;
+    bool central = false;
     if (ws[0] / (sd + ws[0]) > CT1) {
-        central = 1;
+        central = true;
         ws[0] = 1.0;
 #pragma unroll
         for (int st = 1; st < G; ++st) ws[st] = 0.0;
@@ -359,44 +424,46 @@ __global__ __launch_bounds__(64) void teno_kernel(Geom g, Teno t, const double* 
 #pragma unroll
         for (int st = 1; st < G; ++st) ws[st] /= s2;
     }
-    double pw[4][KP];
+    double pw[G][KP];
 #pragma unroll
     for (int st = 0; st < G; ++st)
 #pragma unroll
-        for (int k = 0; k < KP; ++k) pw[st][k] = ws[st] * sh_a[lc][st][k][v];
+        for (int k = 0; k < KP; ++k) pw[st][k] = ws[st] * at[st][k];
 
-    const uint32_t S = g.S, Q = g.Q;
+    // face points (face_reconstruction.cpp:985-1031)
     const double ucv = uc.v[v];
-    for (uint32_t l = 0; l < S; ++l)
-        for (uint32_t q = 0; q < Q; ++q) {
-            const double* B = t.fbasis + ((size_t)(l * Q + q) * KP) * N + cc;
-            double bk[KP];
+    double f[FQ];
 #pragma unroll
-            for (int k = 0; k < KP; ++k) bk[k] = B[(size_t)k * N];
-            double acc = ucv;
+    for (int fq = 0; fq < FQ; ++fq) {
+        const double* B = t.fbasis + (((size_t)lb * FQ + fq) * KP) * 64 + (c & 63u);
+        double bk[KP];
 #pragma unroll
-            for (int st = 0; st < G; ++st) {
-                if (ws[st] == 0.0) continue;
-                acc = acc + (ws[st] * 0.0) * t.basis0;
+        for (int k = 0; k < KP; ++k) bk[k] = B[k * 64];
+        double acc = ucv;
 #pragma unroll
-                for (int k = 0; k < KP; ++k) acc = acc + pw[st][k] * bk[k];
-            }
-            sh_f[lc][l * Q + q][v] = acc;
+        for (int st = 0; st < G; ++st) {
+            if (ws[st] == 0.0) continue;
+            acc = acc + (ws[st] * 0.0) * t.basis0;
+#pragma unroll
+            for (int k = 0; k < KP; ++k) acc = acc + pw[st][k] * bk[k];
         }
-    sh_d[lc][v] = central;
-    __syncthreads();
+        f[fq] = acc;
+    }
 
     if (t.variant != 0) {
         // positivity scaling of the cell's face states (Zhang & Shu 2010),
-        // computed redundantly by the 4 lanes, same order as the oracle
-        double Uc[4] = {uc.v[0], uc.v[1], uc.v[2], uc.v[3]};
+        // computed redundantly by the 4 lanes, in the oracle's order
+        double uf[FQ][4];
+#pragma unroll
+        for (int fq = 0; fq < FQ; ++fq) quad_gather(f[fq], uf[fq], q);
+        const double Uc[4] = {uc.v[0], uc.v[1], uc.v[2], uc.v[3]};
         const double pc = pressure_raw(ph.gamma, Uc);
         const double eps = fmin(1.0e-13, fmin(Uc[0], pc));
         double theta = 1.0;
-        for (uint32_t fq = 0; fq < S * Q; ++fq) {
-            double u[4], wv[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) u[k] = sh_f[lc][fq][k];
+        for (int fq = 0; fq < FQ; ++fq) {
+            const double* u = uf[fq];
+            double wv[4];
             double tt = 1.0;
             if (u[0] < eps) tt = (Uc[0] - eps) / (Uc[0] - u[0]);
 #pragma unroll
@@ -415,12 +482,14 @@ __global__ __launch_bounds__(64) void teno_kernel(Geom g, Teno t, const double* 
             theta = fmin(theta, tt);
         }
         if (theta < 1.0)
-            for (uint32_t fq = 0; fq < S * Q; ++fq) sh_f[lc][fq][v] = Uc[v] + theta * (sh_f[lc][fq][v] - Uc[v]);
+#pragma unroll
+            for (int fq = 0; fq < FQ; ++fq) f[fq] = Uc[v] + theta * (f[fq] - Uc[v]);
     }
+    const uint64_t cmask = __ballot(central);
     if (valid) {
-        for (uint32_t fq = 0; fq < S * Q; ++fq) fs[((size_t)fq * N + c) * 4 + v] = sh_f[lc][fq][v];
-        if (dec && v == 0)
-            dec[c] = (uint8_t)(sh_d[lc][0] | (sh_d[lc][1] << 1) | (sh_d[lc][2] << 2) | (sh_d[lc][3] << 3));
+#pragma unroll
+        for (int fq = 0; fq < FQ; ++fq) fs[((size_t)fq * N + c) * 4 + v] = f[fq];
+        if (dec && v == 0) dec[c] = (uint8_t)((cmask >> ((threadIdx.x & 63u) & ~3u)) & 0xFu);
     }
 }
 
@@ -620,6 +689,37 @@ __global__ void copy_kernel(const double* __restrict__ s, double* __restrict__ d
     if (i < n) d[i] = s[i];
 }
 
+// Streaming probes: the bandwidth HBM delivers to a plain access pattern
+// (16 B per lane, grid-stride, 4 loads in flight per lane), the practical
+// ceiling the gather kernels are compared with.
+__global__ __launch_bounds__(256) void stream_copy_kernel(const double2* __restrict__ s, double2* __restrict__ d,
+                                                          size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const double2 a = s[i], b = s[i + stride], c = s[i + 2 * stride], e = s[i + 3 * stride];
+        d[i] = a;
+        d[i + stride] = b;
+        d[i + 2 * stride] = c;
+        d[i + 3 * stride] = e;
+    }
+    for (; i < n; i += stride) d[i] = s[i];
+}
+
+__global__ __launch_bounds__(256) void stream_read_kernel(const double2* __restrict__ s, double* __restrict__ out,
+                                                          size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc0 = 0.0, acc1 = 0.0;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const double2 a = s[i], b = s[i + stride], c = s[i + 2 * stride], e = s[i + 3 * stride];
+        acc0 += (a.x + b.x) + (c.x + e.x);
+        acc1 += (a.y + b.y) + (c.y + e.y);
+    }
+    for (; i < n; i += stride) acc0 += s[i].x + s[i].y;
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = acc0 + acc1;
+}
+
 // Device math probes for the parity tests (sqrt, division, pow, pow6).
 __global__ void probe_kernel(const double* x, const double* y, double* out, int n, int which) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -638,15 +738,40 @@ __global__ void probe_kernel(const double* x, const double* y, double* out, int 
 
 hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs, uint8_t* dec,
                       const Physics& ph, hipStream_t stream) {
-    const int Kp = t.K - 1;
-    if (Kp > kMaxKp || g.S != 3 || g.S * g.Q > 8) return hipErrorInvalidValue;
-    const dim3 grid((g.N + 15) / 16), block(64);
-    switch (Kp) {
-    case 2: hipLaunchKernelGGL(teno_kernel<2>, grid, block, 0, stream, g, t, U, fs, dec, ph); break;
-    case 5: hipLaunchKernelGGL(teno_kernel<5>, grid, block, 0, stream, g, t, U, fs, dec, ph); break;
-    case 9: hipLaunchKernelGGL(teno_kernel<9>, grid, block, 0, stream, g, t, U, fs, dec, ph); break;
-    default: return hipErrorInvalidValue;
+    const int Kp = t.K - 1, Mp = t.M - 1;
+    const uint32_t FQ = g.S * g.Q;
+    if (g.S != 3 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
+    const uint32_t nblk = (g.N + 63) / 64, per_xcd = (nblk + 7) / 8;
+    const dim3 grid(8 * per_xcd), block(256);
+    const bool small = t.loc8 != nullptr;
+    if (!small && t.loc16 == nullptr) return hipErrorInvalidValue;
+#define MLR_TENO(KP, MP, FQV)                                                                              \
+    do {                                                                                                   \
+        if (small)                                                                                         \
+            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint8_t, 256>), grid, block, 0, stream, g, t, U, fs, \
+                               dec, ph, per_xcd);                                                          \
+        else                                                                                               \
+            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint16_t, 1024>), grid, block, 0, stream, g, t, U,   \
+                               fs, dec, ph, per_xcd);                                                      \
+    } while (0)
+    // the default stencil size (factor 2: M = 2K) is compiled with the member
+    // loop unrolled; other sizes take the runtime loop
+    if (FQ == 3) {
+        switch (Kp) {
+        case 2: if (Mp == 5) MLR_TENO(2, 5, 3); else MLR_TENO(2, 0, 3); break;
+        case 5: if (Mp == 11) MLR_TENO(5, 11, 3); else MLR_TENO(5, 0, 3); break;
+        case 9: MLR_TENO(9, 0, 3); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (Kp) {
+        case 2: MLR_TENO(2, 0, 6); break;
+        case 5: MLR_TENO(5, 0, 6); break;
+        case 9: if (Mp == 19) MLR_TENO(9, 19, 6); else MLR_TENO(9, 0, 6); break;
+        default: return hipErrorInvalidValue;
+        }
     }
+#undef MLR_TENO
     return hipGetLastError();
 }
 
@@ -674,6 +799,26 @@ hipError_t finalize_dt(const unsigned long long* lam_max_in, double cfl, double 
 hipError_t copy_state(const double* src, double* dst, uint32_t N, hipStream_t stream) {
     const size_t n = 4 * (size_t)N;
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, dst, n);
+    return hipGetLastError();
+}
+
+uint32_t stream_probe_threads(size_t bytes) {
+    const size_t n = bytes / 16;
+    const size_t blocks = (n + 255) / 256;
+    return (uint32_t)(blocks < 16384 ? blocks : 16384) * 256u;
+}
+
+hipError_t stream_probe(int kind, const void* src, void* dst, size_t bytes, hipStream_t stream) {
+    const size_t n = bytes / 16;
+    const uint32_t threads = stream_probe_threads(bytes);
+    if (threads == 0) return hipErrorInvalidValue;
+    const dim3 grid(threads / 256), block(256);
+    if (kind == 0)
+        hipLaunchKernelGGL(stream_copy_kernel, grid, block, 0, stream, static_cast<const double2*>(src),
+                           static_cast<double2*>(dst), n);
+    else
+        hipLaunchKernelGGL(stream_read_kernel, grid, block, 0, stream, static_cast<const double2*>(src),
+                           static_cast<double*>(dst), n);
     return hipGetLastError();
 }
 

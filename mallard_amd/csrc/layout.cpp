@@ -429,8 +429,58 @@ void teno_matrices_of_cell(const Mesh& m, const TenoSpec& ts, uint32_t c,
 // ------------------------------------------------------------ layout
 uint64_t Layout::device_bytes() const {
     auto b = [](const auto& v) { return (uint64_t)v.size() * sizeof(v[0]); };
-    return b(volume) + b(nb) + b(info) + b(nx) + b(ny) + b(area) + b(dxn) + b(st_ids) + b(st_ta) +
+    return b(volume) + b(nb) + b(info) + b(nx) + b(ny) + b(area) + b(dxn) + b(blk_off) + b(blk_ids) +
+           b(st_loc8) + b(st_loc16) + b(st_ta) +
            b(st_A) + b(fbasis);
+}
+
+// Per-block member lists and in-list positions (see Layout::blk_off).
+void build_tiles(Layout& L) {
+    const uint32_t N = L.N, B = Layout::kTileCells;
+    const uint32_t nblk = (N + B - 1) / B;
+    const int G = L.G, Mp = L.M - 1;
+    std::vector<std::vector<int32_t>> lists(nblk);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t b = 0; b < (int64_t)nblk; ++b) {
+        const uint32_t c0 = (uint32_t)b * B, c1 = std::min(N, c0 + B);
+        std::vector<int32_t>& v = lists[b];
+        for (uint32_t c = c0; c < c1; ++c) v.push_back((int32_t)c);
+        for (int s = 0; s < G; ++s)
+            for (int i = 0; i < Mp; ++i)
+                for (uint32_t c = c0; c < c1; ++c)
+                    if ((L.info[c] >> (20 + s)) & 1u) v.push_back(L.st_ids[L.mem_idx(s, i, c)]);
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    L.blk_off.assign(nblk + 1, 0);
+    uint32_t mx = 0;
+    for (uint32_t b = 0; b < nblk; ++b) {
+        L.blk_off[b + 1] = L.blk_off[b] + (uint32_t)lists[b].size();
+        mx = std::max(mx, (uint32_t)lists[b].size());
+    }
+    if (mx > Layout::kTileMax16)
+        fail("TENO tile of " + std::to_string(mx) + " cells exceeds the LDS limit of " +
+             std::to_string(Layout::kTileMax16));
+    L.tile_max = mx;
+    L.blk_ids.resize(L.blk_off[nblk]);
+    const bool small = mx <= Layout::kTileMax8;
+    if (small) L.st_loc8.assign((size_t)G * Mp * L.n_pad(), 0);
+    else L.st_loc16.assign((size_t)G * Mp * L.n_pad(), 0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t b = 0; b < (int64_t)nblk; ++b) {
+        const std::vector<int32_t>& v = lists[b];
+        std::copy(v.begin(), v.end(), L.blk_ids.begin() + L.blk_off[b]);
+        const uint32_t c0 = (uint32_t)b * B, c1 = std::min(N, c0 + B);
+        for (int s = 0; s < G; ++s)
+            for (int i = 0; i < Mp; ++i)
+                for (uint32_t c = c0; c < c1; ++c) {
+                    if (!((L.info[c] >> (20 + s)) & 1u)) continue;
+                    const size_t o = L.mem_idx(s, i, c);
+                    const uint32_t pos = (uint32_t)(std::lower_bound(v.begin(), v.end(), L.st_ids[o]) - v.begin());
+                    if (small) L.st_loc8[o] = (uint8_t)pos;
+                    else L.st_loc16[o] = (uint16_t)pos;
+                }
+    }
 }
 
 Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::vector<BoundarySpec>& bcs,
@@ -563,10 +613,11 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
     L.OI = ctx.oscillation_indicator();
     const int K = L.K, M = L.M, G = L.G, Kp = K - 1, Mp = M - 1, Q = (int)L.Q;
     L.basis0 = ts.variant == TenoVariant::Corrected ? basis_2d(ts.basis, 0, 0, 0.0, 0.0) - L.ipt[0] : 1.0;
-    L.st_ids.assign((size_t)G * Mp * N, 0);
-    L.st_ta.assign((size_t)G * Mp * N, 0.0);
-    L.st_A.assign((size_t)G * Mp * Kp * N, 0.0);
-    L.fbasis.assign((size_t)S * Q * Kp * N, 0.0);
+    const size_t Np = L.n_pad();
+    L.st_ids.assign((size_t)G * Mp * Np, 0);
+    L.st_ta.assign((size_t)G * Mp * Np, 0.0);
+    L.st_A.assign((size_t)G * Mp * Kp * Np, 0.0);
+    L.fbasis.assign((size_t)S * Q * Kp * Np, 0.0);
     int err_code = 0;
     uint32_t err_cell = 0;
 #pragma omp parallel
@@ -607,11 +658,10 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
                     { err_code = 4; err_cell = c; }
                 }
                 for (int i = 1; i < M; ++i) {
-                    const size_t o = ((size_t)s * Mp + (i - 1)) * N + j;
+                    const size_t o = L.mem_idx(s, i - 1, j);
                     L.st_ids[o] = (int32_t)L.iperm[St[s][i]];
                     L.st_ta[o] = ta[i];
-                    for (int k = 1; k < K; ++k)
-                        L.st_A[(((size_t)s * Mp + (i - 1)) * Kp + (k - 1)) * N + j] = aplus[(size_t)k * M + i];
+                    for (int k = 1; k < K; ++k) L.st_A[L.a_idx(s, i - 1, k - 1, j)] = aplus[(size_t)k * M + i];
                 }
             }
             L.info[j] |= mask << 20;
@@ -629,7 +679,7 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
                     for (int k = 1; k < K; ++k) {
                         const double psi = basis_2d(ts.basis, L.poly[2 * k], L.poly[2 * k + 1], xq[0], xq[1]);
                         const double v = ts.variant == TenoVariant::Corrected ? psi - L.ipt[k] : psi + (L.ipt[k] / ta0);
-                        L.fbasis[(((size_t)l * Q + q) * Kp + (k - 1)) * N + j] = v;
+                        L.fbasis[L.fb_idx(l * Q + q, k - 1, j)] = v;
                     }
                 }
             }
@@ -641,6 +691,7 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
         fail("TENO stencil of cell " + std::to_string(err_cell) +
              " needs the full-matrix pseudo-inverse branch, which this layout does not store");
     if (err_code == 4) fail("TENO stencils of cell " + std::to_string(err_cell) + " disagree on the target area");
+    if (L.teno) build_tiles(L);
     return L;
 }
 

@@ -64,14 +64,46 @@ struct Layout {
     std::vector<uint8_t> poly;            // [K][2]
     std::vector<double> ipt;              // [K]
     std::vector<double> OI;               // [K][K]
-    std::vector<int32_t> st_ids;          // [G][M-1][N] neighbours in new numbering
-    std::vector<double> st_ta;            // [G][M-1][N]
-    std::vector<double> st_A;             // [G][M-1][K-1][N] A+ without its zero row/column
-    std::vector<double> fbasis;           // [S][Q][K-1][N] psi_k(x_q) + ipt_k/ta_0 (or - ipt_k)
+    std::vector<int32_t> st_ids;          // neighbours in new numbering (mem_idx)
+    std::vector<double> st_ta;            // transformed member areas (mem_idx)
+    std::vector<double> st_A;             // A+ without its zero row/column (a_idx)
+    std::vector<double> fbasis;           // psi_k(x_q) + ipt_k/ta_0 (or - ipt_k) (fb_idx)
     double basis0 = 1.0;                  // sign carrier of the k = 0 term
 
+    // LDS tiles of the TENO kernel: block b = cells [64b, 64b+64) (new
+    // numbering). blk_ids[blk_off[b] .. blk_off[b+1]) is the sorted list of
+    // every cell the block's stencils read (its own cells included); st_loc
+    // gives each stencil member's position in its block's list.
+    static constexpr uint32_t kTileCells = 64;
+    static constexpr uint32_t kTileMax8 = 256;    // list length that 8-bit positions address
+    static constexpr uint32_t kTileMax16 = 1024;  // LDS cap of the 16-bit form
+    std::vector<uint32_t> blk_off;        // [n_blocks + 1]
+    std::vector<int32_t> blk_ids;         // [blk_off.back()]
+    std::vector<uint8_t> st_loc8;         // (mem_idx), when every list has <= 256 entries
+    std::vector<uint16_t> st_loc16;       // (mem_idx), otherwise
+    uint32_t tile_max = 0;                // longest list
+
     uint64_t device_bytes() const;
+
+    // Blocked (AoSoA) indices of the per-stencil tables: cells in blocks of
+    // 64 (one tile), stencil- then member-major inside a block, so the data
+    // one thread block reads is one contiguous range. i counts the members
+    // after the target (0 .. M-2), fq = slot * Q + point.
+    //   ids / ta / positions: [N/64][G][M-1][64]
+    //   A+:                   [N/64][G][M-1][K-1][64]
+    //   face basis:           [N/64][S*Q][K-1][64]
+    size_t n_pad() const { return ((size_t)N + 63) / 64 * 64; }
+    size_t mem_idx(int s, int i, uint32_t c) const {
+        return (((size_t)(c >> 6) * G + s) * (M - 1) + i) * 64 + (c & 63u);
+    }
+    size_t a_idx(int s, int i, int k, uint32_t c) const {
+        return ((((size_t)(c >> 6) * G + s) * (M - 1) + i) * (K - 1) + k) * 64 + (c & 63u);
+    }
+    size_t fb_idx(uint32_t fq, int k, uint32_t c) const {
+        return (((size_t)(c >> 6) * (S * Q) + fq) * (K - 1) + k) * 64 + (c & 63u);
+    }
 };
+static_assert(Layout::kTileCells == 64, "the blocked tables assume 64-cell blocks");
 
 // Hilbert-curve order of cell centroids (ties broken by reference id).
 std::vector<uint32_t> sfc_order(const Mesh& m);

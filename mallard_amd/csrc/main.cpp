@@ -21,7 +21,7 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "init failed: %s\n", mlr_last_error(nullptr));
         return 1;
     }
-    int64_t sz[13];
+    int64_t sz[15];
     mlr_get_sizes(ctx, sz);
     std::printf("cells %lld, faces %lld, device layout %.3f GB\n", (long long)sz[0], (long long)sz[2], sz[8] / 1e9);
     const auto t0 = std::chrono::steady_clock::now();

@@ -299,12 +299,18 @@ void Solver::upload() {
     }
     up(bc_, bct);
     if (L_.teno) {
-        up(ids_, L_.st_ids);
+        if (!L_.st_loc8.empty()) up(loc8_, L_.st_loc8);
+        else up(loc16_, L_.st_loc16);
+        up(blk_off_, L_.blk_off);
+        up(blk_ids_, L_.blk_ids);
         up(ta_, L_.st_ta);
         up(A_, L_.st_A);
         up(fbasis_, L_.fbasis);
         // host copies of the big tables are not needed any more
         std::vector<int32_t>().swap(L_.st_ids);
+        std::vector<uint8_t>().swap(L_.st_loc8);
+        std::vector<uint16_t>().swap(L_.st_loc16);
+        std::vector<int32_t>().swap(L_.blk_ids);
         std::vector<double>().swap(L_.st_ta);
         std::vector<double>().swap(L_.st_A);
         std::vector<double>().swap(L_.fbasis);
@@ -355,7 +361,10 @@ dev::Teno Solver::teno() const {
     t.M = L_.M;
     t.variant = (int32_t)L_.tspec.variant;
     t.basis0 = L_.basis0;
-    t.ids = ids_;
+    t.loc8 = loc8_;
+    t.loc16 = loc16_;
+    t.blk_off = blk_off_;
+    t.blk_ids = blk_ids_;
     t.ta = ta_;
     t.A = A_;
     t.fbasis = fbasis_;
@@ -468,9 +477,22 @@ void Solver::device_stencils(uint32_t cell, uint32_t* out) const {
     for (int s = 0; s < L_.G; ++s) {
         const bool on = (info0 >> (20 + s)) & 1u;
         out[(size_t)s * L_.M] = on ? cell : 0xFFFFFFFFu;
+        // member = tile list of the cell's block at the member's stored position
+        const uint32_t b = j / Layout::kTileCells;
         for (int i = 0; i < Mp; ++i) {
+            const size_t o = L_.mem_idx(s, i, j);
+            uint32_t pos = 0;
+            if (loc8_) {
+                uint8_t p8 = 0;
+                HIP_OK(hipMemcpy(&p8, loc8_ + o, 1, hipMemcpyDeviceToHost));
+                pos = p8;
+            } else {
+                uint16_t p16 = 0;
+                HIP_OK(hipMemcpy(&p16, loc16_ + o, 2, hipMemcpyDeviceToHost));
+                pos = p16;
+            }
             int32_t id = -1;
-            HIP_OK(hipMemcpy(&id, ids_ + ((size_t)s * Mp + i) * L_.N + j, 4, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(&id, blk_ids_ + L_.blk_off[b] + pos, 4, hipMemcpyDeviceToHost));
             out[(size_t)s * L_.M + 1 + i] = on ? L_.perm[id] : 0xFFFFFFFFu;
         }
     }

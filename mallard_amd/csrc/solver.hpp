@@ -116,6 +116,7 @@ public:
     void launch_flux(int op, const double* Uin, double* Uout);
     double* d_state() { return Un_; }
     uint64_t device_bytes() const { return dev_bytes_; }  // all device allocations
+    int tile_pos_bytes() const { return loc8_ ? 1 : 2; }
     double* d_fs() { return fs_; }
 
 private:
@@ -145,7 +146,10 @@ private:
     int32_t* nb_ = nullptr;
     uint32_t* info_ = nullptr;
     dev::BCParam* bc_ = nullptr;
-    int32_t* ids_ = nullptr;
+    uint8_t* loc8_ = nullptr;
+    uint16_t* loc16_ = nullptr;
+    uint32_t* blk_off_ = nullptr;
+    int32_t* blk_ids_ = nullptr;
     double *ta_ = nullptr, *A_ = nullptr, *fbasis_ = nullptr;
     double *Un_ = nullptr, *Ua_ = nullptr, *Ub_ = nullptr, *W_ = nullptr, *prims_ = nullptr, *h_ = nullptr;
     double *fs_ = nullptr, *rhs_ = nullptr, *lam_cell_ = nullptr, *dt_dev_ = nullptr, *dt_hist_ = nullptr;

@@ -135,14 +135,33 @@ def test_teno_reference_variant_residual(mlr, oracle, order):
     assert np.allclose(r_dev[fin], r_orc[fin], rtol=1e-12, atol=1e-12 * scale)
 
 
-def test_teno_corrected_residual_bitexact(mlr, oracle):
-    dev, orc, m = pair(mlr, oracle, "cartesian_tri", 16, 16, jitter=0.2, seed=7, recon="TENO",
-                       order=2, teno_variant="corrected")
+@pytest.mark.parametrize("order,factor", [(1, 2.0), (2, 2.0), (2, 3.0), (3, 2.0)])
+def test_teno_corrected_residual_bitexact(mlr, oracle, order, factor):
+    """p=2 with factor 2 runs the unrolled kernel with 8-bit tile positions;
+    factor 3 and p=3 take the 16-bit tile form, p=3 also two face points."""
+    dev, orc, m = pair(mlr, oracle, "cartesian_tri", 20, 20, jitter=0.2, seed=7, recon="TENO",
+                       order=order, teno_variant="corrected", stencil_factor=factor)
     dev.init_quadrant(0.5, 0.5, RIEMANN2D)
     orc.init_quadrant(0.5, 0.5, RIEMANN2D)
     r_dev = dev.calc_rhs()
     r_orc = orc.calc_rhs(orc.state()[0])
     assert np.array_equal(r_dev, r_orc)
+
+
+def test_device_stencils_match_oracle(mlr, oracle):
+    """Stencil ids read back from the uploaded tile layout (positions in each
+    64-cell block's list) equal the oracle's stencils, in reference numbering."""
+    dev, orc, m = pair(mlr, oracle, "cartesian_tri", 24, 24, jitter=0.2, seed=11, recon="TENO",
+                       order=2, teno_variant="corrected")
+    T = orc.teno()
+    og, os_, st = T["offsets_stencil_groups"], T["offsets_stencils"], T["stencils"]
+    for c in range(0, m.n_cells, 13):
+        ids = dev.device_stencils(c)
+        for sl in range(4):
+            si = og[c] + sl
+            ref = st[os_[si]:os_[si + 1]]
+            got = ids[sl][ids[sl] != 0xFFFFFFFF]
+            assert np.array_equal(got, ref), (c, sl)
 
 
 def test_teno_corrected_100_steps(mlr, oracle):
