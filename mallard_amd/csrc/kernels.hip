@@ -496,9 +496,15 @@ __global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double*
 // ------------------------------------------------------------ flux + update
 // One thread per cell: the residual of the cell in the canonical face order,
 // divided by the volume, then the stage's SSP-RK update (no atomics).
-__global__ __launch_bounds__(256) void flux_update_kernel(Geom g, StageArgs a, Physics ph) {
+// RS: Riemann solver (compile-time, so each instantiation carries one
+// solver's code and registers). Blocks follow the XCD-aware order of the
+// TENO kernel, so the neighbour face states a block gathers were written by
+// the same XCD.
+template <int RS>
+__global__ __launch_bounds__(256) void flux_update_kernel(Geom g, StageArgs a, Physics ph, uint32_t blocks_per_xcd) {
     const uint32_t N = g.N;
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
+    const uint32_t c = lb * blockDim.x + threadIdx.x;
     if (c >= N) return;
     const uint32_t Q = g.Q;
     const uint32_t info0 = g.info[c];
@@ -526,9 +532,9 @@ __global__ __launch_bounds__(256) void flux_update_kernel(Geom g, StageArgs a, P
                 oth = a.fo ? ld4(a.Uin, nbc) : ld4(a.fs, (size_t)(nbs * Q + q) * N + nbc);
                 const double* L = side ? oth.v : own.v;
                 const double* R = side ? own.v : oth.v;
-                point_flux(ph, g.bc, a.riemann, a.vacuum_guard, false, 0, n0, n1, L, R, Ft);
+                point_flux(ph, g.bc, RS, a.vacuum_guard, false, 0, n0, n1, L, R, Ft);
             } else {
-                point_flux(ph, g.bc, a.riemann, a.vacuum_guard, true, (inf >> 4) & 0xFu, n0, n1, own.v, own.v, Ft);
+                point_flux(ph, g.bc, RS, a.vacuum_guard, true, (inf >> 4) & 0xFu, n0, n1, own.v, own.v, Ft);
             }
             const double wq = g.qw[q];
 #pragma unroll
@@ -776,8 +782,15 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
 }
 
 hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, hipStream_t stream) {
-    const dim3 grid((g.N + 255) / 256), block(256);
-    hipLaunchKernelGGL(flux_update_kernel, grid, block, 0, stream, g, a, ph);
+    const uint32_t nblk = (g.N + 255) / 256, per_xcd = (nblk + 7) / 8;
+    const dim3 grid(8 * per_xcd), block(256);
+    switch (a.riemann) {
+    case 0: hipLaunchKernelGGL((flux_update_kernel<0>), grid, block, 0, stream, g, a, ph, per_xcd); break;
+    case 1: hipLaunchKernelGGL((flux_update_kernel<1>), grid, block, 0, stream, g, a, ph, per_xcd); break;
+    case 2: hipLaunchKernelGGL((flux_update_kernel<2>), grid, block, 0, stream, g, a, ph, per_xcd); break;
+    case 3: hipLaunchKernelGGL((flux_update_kernel<3>), grid, block, 0, stream, g, a, ph, per_xcd); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
