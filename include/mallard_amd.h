@@ -205,6 +205,53 @@ int mlr_layout_teno_tables(const mlr_layout* lay, double* ipt, double* oi);
 int mlr_layout_slots(const mlr_layout* lay, uint32_t cell, uint32_t* nb, uint32_t* info, double* nx,
                      double* ny, double* area, double* dxn);
 
+/* Host-only input parsing (no GPU), for the input-compatibility tests.
+ * Solver::init's TOML reading (solver.cpp:82-273, mesh.cpp:32-66,
+ * face_reconstruction.cpp:109-152): mesh and numerics as the descriptors
+ * above; run[7] = cfl (0 when dt is fixed), dt, n_steps (-1: none), t_stop,
+ * t_wall_stop, check_interval, check_nan; init_type 0 constant,
+ * 1 analytical, 2 quadrant. */
+int mlr_config_parse(const char* toml_path, mlr_mesh_gen* mesh, mlr_numerics_desc* num, double* run,
+                     int32_t* init_type);
+/* The analytical-initialisation expression evaluator (the exprtk subset of
+ * solver_initialize.cpp:101-186) at n points (x, y). */
+int mlr_expr_eval(const char* expr, int64_t n, const double* x, const double* y, double* out);
+
+/* ---- learned troubled-cell indicator (no reference counterpart; the
+ * BF16-MFMA MLP that lets cells judged smooth skip the directional TENO
+ * stencils). Parameters are the FP32 master vector of 7361 values:
+ * W1[48][64], b1[64], W2[64][64], b2[64], w3[64], b3. Features are 48 bf16
+ * per cell (12 central-stencil members x 4), labels 1 where plain TENO
+ * selected the central stencil for all four variables. */
+typedef struct mlr_dataset mlr_dataset;
+/* device-resident rows of (features, label) */
+int mlr_dataset_create(int64_t capacity, mlr_dataset** out);
+int mlr_dataset_destroy(mlr_dataset* ds);
+int mlr_dataset_size(const mlr_dataset* ds, int64_t* n);
+int mlr_dataset_get(const mlr_dataset* ds, int64_t start, int64_t n, uint16_t* X, float* y);
+/* plain TENO decisions of the context's current state, then every cell's
+ * features and label appended (new-numbering order) */
+int mlr_collect_features(mlr_ctx* ctx, mlr_dataset* ds);
+/* the same for the current state, in reference numbering: X[N][48], y[N] */
+int mlr_indicator_features(mlr_ctx* ctx, uint16_t* X, float* y);
+/* logits of n host rows on the GPU (the inference kernel) */
+int mlr_mlp_forward(const float* params, const uint16_t* X, int64_t n, float* z);
+/* gradient and loss of the weighted mean BCE over dataset rows idx[0..B):
+ * label-1 rows weigh pos_w, label-0 rows neg_w (the training kernels) */
+int mlr_mlp_grad(const float* params, const mlr_dataset* ds, const uint32_t* idx, int64_t B, float pos_w,
+                 float neg_w, float* grad, float* loss);
+/* Adam on the GPU: `steps` batches of `batch` rows drawn from pool by a hash
+ * of (seed, step, i); params updated in place; loss_hist[steps] may be NULL */
+int mlr_mlp_train(float* params, const mlr_dataset* ds, const uint32_t* pool, int64_t n_pool, int steps, int batch,
+                  float lr, uint64_t seed, float pos_w, float neg_w, float* loss_hist);
+/* gate the context's TENO with the network (params NULL: off); cells with
+ * sigmoid(z) >= threshold take the central stencil alone */
+int mlr_set_indicator(mlr_ctx* ctx, const float* params, double threshold);
+/* cells judged smooth (summed over every gated TENO pass) since the last reset */
+int mlr_indicator_gated(mlr_ctx* ctx, int reset, int64_t* gated);
+/* cell volumes in reference numbering */
+int mlr_get_volumes(const mlr_ctx* ctx, double* vol);
+
 /* Device math probes for the parity tests: 0 sqrt, 1 divide, 2 the
  * double-double pow of the two-rarefaction estimate, 3 x^6, 4 the platform
  * pow. */

@@ -84,6 +84,21 @@ _SIGS = {
     "mlr_layout_slots": (C.c_int, [_P, C.c_uint32] + [_P] * 6),
     "mlr_probe_math": (C.c_int, [C.c_int, _P, _P, _P, C.c_int]),
     "mlr_probe_stream": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+    "mlr_config_parse": (C.c_int, [C.c_char_p, C.POINTER(MlrMeshGen), C.POINTER(MlrNumerics), _P, _P]),
+    "mlr_expr_eval": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, _P]),
+    "mlr_dataset_create": (C.c_int, [C.c_int64, C.POINTER(_P)]),
+    "mlr_dataset_destroy": (C.c_int, [_P]),
+    "mlr_dataset_size": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "mlr_dataset_get": (C.c_int, [_P, C.c_int64, C.c_int64, _P, _P]),
+    "mlr_collect_features": (C.c_int, [_P, _P]),
+    "mlr_indicator_features": (C.c_int, [_P, _P, _P]),
+    "mlr_mlp_forward": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "mlr_mlp_grad": (C.c_int, [_P, _P, _P, C.c_int64, C.c_float, C.c_float, _P, _P]),
+    "mlr_mlp_train": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int, C.c_int, C.c_float, C.c_uint64, C.c_float,
+                                C.c_float, _P]),
+    "mlr_set_indicator": (C.c_int, [_P, _P, C.c_double]),
+    "mlr_indicator_gated": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
+    "mlr_get_volumes": (C.c_int, [_P, _P]),
     "mlr_host_pow": (C.c_int, [_P, _P, _P, C.c_int]),
     "mlr_host_pow6": (C.c_int, [_P, _P, C.c_int]),
 }
@@ -326,10 +341,130 @@ class Solver:
         self._check(lib().mlr_get_permutation(self.h, _p(p)))
         return p
 
+    # ---- learned troubled-cell indicator
+    def set_indicator(self, params, threshold=0.9):
+        """Gate TENO with the network (params: 7361 FP32 values; None: off)."""
+        if params is None:
+            self._check(lib().mlr_set_indicator(self.h, None, threshold))
+        else:
+            p = np.ascontiguousarray(params, dtype=np.float32)
+            if p.size != N_PARAMS:
+                raise ValueError("indicator parameters must have %d values" % N_PARAMS)
+            self._check(lib().mlr_set_indicator(self.h, _p(p), threshold))
+
+    def gated(self, reset=False):
+        n = C.c_int64()
+        self._check(lib().mlr_indicator_gated(self.h, int(reset), C.byref(n)))
+        return n.value
+
+    def indicator_features(self):
+        """bf16 features (uint16 [N][48]) and TENO labels of the current state."""
+        X = np.zeros((self.n_cells, N_FEATURES), np.uint16)
+        y = np.zeros(self.n_cells, np.float32)
+        self._check(lib().mlr_indicator_features(self.h, _p(X), _p(y)))
+        return X, y
+
+    def collect_features(self, ds):
+        self._check(lib().mlr_collect_features(self.h, ds.h))
+
+    def volumes(self):
+        v = np.zeros(self.n_cells)
+        self._check(lib().mlr_get_volumes(self.h, _p(v)))
+        return v
+
     def device_stencils(self, cell):
         ids = np.zeros(self.G * self.M, np.uint32)
         self._check(lib().mlr_get_device_stencils(self.h, cell, _p(ids)))
         return ids.reshape(self.G, self.M)
+
+
+def parse_config(path):
+    """Host-only parse of a Mallard TOML input: (mesh_gen, numerics, run dict, init type)."""
+    g, d = MlrMeshGen(), MlrNumerics()
+    run = np.zeros(7)
+    it = np.zeros(1, np.int32)
+    if lib().mlr_config_parse(os.fspath(path).encode(), C.byref(g), C.byref(d), _p(run), _p(it)):
+        raise MallardError(lib().mlr_last_error(None).decode())
+    keys = ("cfl", "dt", "n_steps", "t_stop", "t_wall_stop", "check_interval", "check_nan")
+    return g, d, dict(zip(keys, run.tolist())), ("constant", "analytical", "quadrant")[int(it[0])]
+
+
+def eval_expr(expr, x, y):
+    """The analytical-initialisation expression evaluator at points (x, y)."""
+    x, y = _f64(x).ravel(), _f64(y).ravel()
+    out = np.zeros(len(x))
+    if lib().mlr_expr_eval(expr.encode(), len(x), _p(x), _p(y), _p(out)):
+        raise MallardError(lib().mlr_last_error(None).decode())
+    return out
+
+
+N_FEATURES = 48
+N_PARAMS = 7361
+
+
+class Dataset:
+    """Device-resident training rows (bf16 features, TENO label)."""
+
+    def __init__(self, capacity):
+        h = C.c_void_p()
+        if lib().mlr_dataset_create(int(capacity), C.byref(h)):
+            raise MallardError(lib().mlr_last_error(None).decode())
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mlr_dataset_destroy(self.h)
+            self.h = None
+
+    def __len__(self):
+        n = C.c_int64()
+        lib().mlr_dataset_size(self.h, C.byref(n))
+        return n.value
+
+    def get(self, start=0, n=None, features=True):
+        """(X, y) of rows start..start+n; X is None when features=False."""
+        n = len(self) - start if n is None else n
+        X = np.zeros((n, N_FEATURES), np.uint16) if features else None
+        y = np.zeros(n, np.float32)
+        if lib().mlr_dataset_get(self.h, start, n, _p(X), _p(y)):
+            raise MallardError(lib().mlr_last_error(None).decode())
+        return X, y
+
+
+def _params(p):
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    if p.size != N_PARAMS:
+        raise ValueError("indicator parameters must have %d values" % N_PARAMS)
+    return p
+
+
+def mlp_forward(params, X):
+    """Logits of bf16 feature rows (uint16 [n][48]) by the GPU inference kernel."""
+    X = np.ascontiguousarray(X, dtype=np.uint16)
+    z = np.zeros(len(X), np.float32)
+    if lib().mlr_mlp_forward(_p(_params(params)), _p(X), len(X), _p(z)):
+        raise MallardError(lib().mlr_last_error(None).decode())
+    return z
+
+
+def mlp_grad(params, ds, idx, pos_w=1.0, neg_w=1.0):
+    """Gradient and loss of the weighted mean BCE over dataset rows idx (GPU kernels)."""
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    g = np.zeros(N_PARAMS, np.float32)
+    loss = np.zeros(1, np.float32)
+    if lib().mlr_mlp_grad(_p(_params(params)), ds.h, _p(idx), len(idx), pos_w, neg_w, _p(g), _p(loss)):
+        raise MallardError(lib().mlr_last_error(None).decode())
+    return g, float(loss[0])
+
+
+def mlp_train(params, ds, pool, steps, batch, lr, seed=0, pos_w=1.0, neg_w=1.0):
+    """Adam on the GPU; returns (trained parameters, loss per step)."""
+    p = _params(params).copy()
+    pool = np.ascontiguousarray(pool, dtype=np.uint32)
+    hist = np.zeros(steps, np.float32)
+    if lib().mlr_mlp_train(_p(p), ds.h, _p(pool), len(pool), steps, batch, lr, seed, pos_w, neg_w, _p(hist)):
+        raise MallardError(lib().mlr_last_error(None).decode())
+    return p, hist
 
 
 def probe_stream(kind, nbytes, reps=10):

@@ -9,11 +9,18 @@
 #include <string>
 
 #include "ddpow.hpp"
+#include "expr.hpp"
+#include "learn.hpp"
 #include "solver.hpp"
 
 struct mlr_ctx {
     mlr::Solver solver;
     std::string err;
+};
+
+struct mlr_dataset {
+    explicit mlr_dataset(uint64_t cap) : ds(cap) {}
+    mlr::Dataset ds;
 };
 
 struct mlr_layout {
@@ -83,6 +90,52 @@ mlr::Config config_from_desc(const mlr_numerics_desc* d) {
     c.n_threads = d->n_threads;
     c.n_steps = 0;
     return c;
+}
+
+void desc_from_config(const mlr::Config& c, mlr_mesh_gen* mesh, mlr_numerics_desc* d) {
+    using namespace mlr;
+    if (mesh) {
+        std::memset(mesh, 0, sizeof(*mesh));
+        mesh->type = (int32_t)c.mesh_type;
+        mesh->nx = c.nx;
+        mesh->ny = c.ny;
+        mesh->Lx = c.Lx;
+        mesh->Ly = c.Ly;
+        mesh->jitter = c.jitter;
+        mesh->seed = c.seed;
+    }
+    if (d) {
+        std::memset(d, 0, sizeof(*d));
+        d->recon = c.recon == Recon::TENO ? MLR_RECON_TENO : MLR_RECON_FO;
+        d->basis = c.teno.basis == BasisType::Legendre ? MLR_BASIS_LEGENDRE : MLR_BASIS_MONOMIAL;
+        d->order = c.teno.order;
+        d->stencil_factor = c.teno.factor;
+        d->quad_order_cell = c.teno.quad_order_cell;
+        d->quad_order_face = c.teno.quad_order_face;
+        d->teno_variant = (int32_t)c.teno.variant;
+        d->riemann = (int32_t)c.riemann;
+        d->integrator = c.integrator == Integrator::FE ? MLR_TI_FE
+                                                       : (c.integrator == Integrator::RK4 ? MLR_TI_RK4 : MLR_TI_SSPRK3);
+        d->gamma = c.gamma;
+        d->p_ref = c.p_ref;
+        d->T_ref = c.T_ref;
+        d->rho_ref = c.rho_ref;
+        d->p_min = c.p_min;
+        d->p_max = c.p_max;
+        d->cfl = c.use_cfl ? c.cfl : 0.0;
+        d->dt = c.dt;
+        if (c.bcs.size() > 14) fail("more than 14 boundary conditions");
+        d->n_bc = (int32_t)c.bcs.size();
+        for (size_t b = 0; b < c.bcs.size(); ++b) {
+            std::strncpy(d->bc[b].zone, c.bcs[b].zone.c_str(), sizeof(d->bc[b].zone) - 1);
+            d->bc[b].type = (int32_t)c.bcs[b].type;
+            d->bc[b].u[0] = c.bcs[b].u[0];
+            d->bc[b].u[1] = c.bcs[b].u[1];
+            d->bc[b].p = c.bcs[b].p;
+            d->bc[b].T = c.bcs[b].T;
+        }
+        d->n_threads = c.n_threads;
+    }
 }
 
 }  // namespace
@@ -537,6 +590,102 @@ int mlr_probe_stream(int kind, int64_t bytes, int reps, double* ms_avg) {
         chk(hipEventElapsedTime(&ms, e0.e, e1.e));
         *ms_avg = ms / reps;
     });
+}
+
+// ---------------------------------------------------------------- input parsing (host only)
+int mlr_config_parse(const char* toml_path, mlr_mesh_gen* mesh, mlr_numerics_desc* num, double* run,
+                     int32_t* init_type) {
+    return guarded(nullptr, [&] {
+        const mlr::Config c = mlr::config_from_toml(mlr::toml::parse_file(toml_path ? toml_path : ""));
+        desc_from_config(c, mesh, num);
+        if (run) {
+            run[0] = c.use_cfl ? c.cfl : 0.0;
+            run[1] = c.dt;
+            run[2] = (double)c.n_steps;
+            run[3] = c.t_stop;
+            run[4] = c.t_wall_stop;
+            run[5] = (double)c.check_interval;
+            run[6] = c.check_nan ? 1.0 : 0.0;
+        }
+        if (init_type) *init_type = (int32_t)c.init.kind;
+    });
+}
+
+int mlr_expr_eval(const char* expr, int64_t n, const double* x, const double* y, double* out) {
+    if (!expr || n < 0 || (n > 0 && (!x || !y || !out))) return 1;
+    return guarded(nullptr, [&] {
+        const mlr::Expression e(expr);
+        for (int64_t i = 0; i < n; ++i) out[i] = e.eval(x[i], y[i]);
+    });
+}
+
+// ---------------------------------------------------------------- learned indicator
+int mlr_dataset_create(int64_t capacity, mlr_dataset** out) {
+    if (!out || capacity <= 0) return 1;
+    *out = nullptr;
+    return guarded(nullptr, [&] { *out = new mlr_dataset((uint64_t)capacity); });
+}
+
+int mlr_dataset_destroy(mlr_dataset* ds) {
+    delete ds;
+    return 0;
+}
+
+int mlr_dataset_size(const mlr_dataset* ds, int64_t* n) {
+    if (!ds || !n) return 1;
+    *n = (int64_t)ds->ds.n;
+    return 0;
+}
+
+int mlr_dataset_get(const mlr_dataset* ds, int64_t start, int64_t n, uint16_t* X, float* y) {
+    if (!ds || start < 0 || n < 0) return 1;
+    return guarded(nullptr, [&] { ds->ds.get((uint64_t)start, (uint64_t)n, X, y); });
+}
+
+int mlr_collect_features(mlr_ctx* ctx, mlr_dataset* ds) {
+    if (!ctx || !ds) return 1;
+    return guarded(ctx, [&] { ctx->solver.collect_features(ds->ds); });
+}
+
+int mlr_indicator_features(mlr_ctx* ctx, uint16_t* X, float* y) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.indicator_features(X, y); });
+}
+
+int mlr_mlp_forward(const float* params, const uint16_t* X, int64_t n, float* z) {
+    if (!params || (n > 0 && (!X || !z)) || n < 0) return 1;
+    return guarded(nullptr, [&] { mlr::mlp_forward(params, X, (uint64_t)n, z); });
+}
+
+int mlr_mlp_grad(const float* params, const mlr_dataset* ds, const uint32_t* idx, int64_t B, float pos_w,
+                 float neg_w, float* grad, float* loss) {
+    if (!params || !ds || !idx || !grad || !loss || B <= 0 || B > 0xFFFFFFFFll) return 1;
+    return guarded(nullptr, [&] { mlr::mlp_grad(params, ds->ds, idx, (uint32_t)B, pos_w, neg_w, grad, loss); });
+}
+
+int mlr_mlp_train(float* params, const mlr_dataset* ds, const uint32_t* pool, int64_t n_pool, int steps, int batch,
+                  float lr, uint64_t seed, float pos_w, float neg_w, float* loss_hist) {
+    if (!params || !ds || !pool || n_pool <= 0 || steps <= 0 || batch <= 0) return 1;
+    return guarded(nullptr, [&] {
+        mlr::mlp_train(params, ds->ds, pool, (uint64_t)n_pool, steps, (uint32_t)batch, lr, seed, pos_w, neg_w,
+                       loss_hist);
+    });
+}
+
+int mlr_set_indicator(mlr_ctx* ctx, const float* params, double threshold) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.set_indicator(params, threshold); });
+}
+
+int mlr_indicator_gated(mlr_ctx* ctx, int reset, int64_t* gated) {
+    if (!ctx || !gated) return 1;
+    return guarded(ctx, [&] { *gated = (int64_t)ctx->solver.gated_cells(reset != 0); });
+}
+
+int mlr_get_volumes(const mlr_ctx* ctx, double* vol) {
+    if (!ctx || !vol) return 1;
+    ctx->solver.get_volumes(vol);
+    return 0;
 }
 
 int mlr_host_pow(const double* x, const double* y, double* out, int n) {

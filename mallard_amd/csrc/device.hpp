@@ -31,6 +31,21 @@ struct Geom {
     const BCParam* bc;    // boundary conditions in input order (device copy)
 };
 
+namespace ind {
+constexpr int kNParams = 7361;  // 48*64 + 64 + 64*64 + 64 + 64 + 1
+// Device copy of the indicator network as the kernels read it: W1p / W2p are
+// the bf16 MFMA operand layouts, the rest FP32.
+struct Net {
+    const void* W1p;
+    const void* W2p;
+    const float* b1;
+    const float* b2;
+    const float* w3;
+    const float* b3;
+    float threshold;  // probability at and above which a cell is judged smooth
+};
+}  // namespace ind
+
 struct Teno {
     int32_t K, M, variant;
     double basis0;
@@ -43,6 +58,12 @@ struct Teno {
     const double* A;
     const double* fbasis;
     double OI[81];      // (K-1) x (K-1) block of the oscillation indicator, row-major
+    // learned indicator gate: when gate != 0 a cell the network judges smooth
+    // takes the central stencil alone; gated counts smooth cells
+    int32_t gate;
+    ind::Net net;
+    unsigned long long* gated;
+    double gamma;
 };
 
 // Stage programs of the fused residual + SSP-RK update.
@@ -85,6 +106,24 @@ hipError_t finalize_dt(const unsigned long long* lam_max_in, double cfl, double 
                        hipStream_t stream);
 hipError_t copy_state(const double* src, double* dst, uint32_t N, hipStream_t stream);
 hipError_t probe_math(const double* x, const double* y, double* out, int n, int which, hipStream_t stream);
+
+// Learned indicator (indicator.hip). The network lives in one device buffer of
+// net_bytes() that net_pack fills from the FP32 master parameters.
+size_t net_bytes();
+ind::Net net_view(void* buf, float threshold, const float** W2f);
+hipError_t net_pack(const float* theta, void* buf, hipStream_t stream);
+// features (bf16 [N][48]) and TENO labels of every cell at rows base..base+N-1
+hipError_t ind_features(const Geom& g, const Teno& t, const double* U, const uint8_t* dec, double gamma,
+                        uint16_t* X, float* y, uint64_t base, hipStream_t stream);
+hipError_t ind_infer(const ind::Net& net, const uint16_t* X, uint64_t n, float* z, hipStream_t stream);
+int ind_train_blocks(uint32_t B);
+// gradient of the weighted mean BCE over rows idx[0..B) (part: ind_train_blocks(B) x kNParams)
+hipError_t ind_grad(const ind::Net& net, const float* W2f, const uint16_t* X, const float* y, const uint32_t* idx,
+                    uint32_t B, float pos_w, float neg_w, float* part, float* loss_part, float* grad, float* loss,
+                    hipStream_t stream);
+hipError_t ind_adam(float* theta, float* m, float* v, const float* grad, float lr, int step, hipStream_t stream);
+hipError_t ind_batch(const uint32_t* pool, uint64_t n_pool, uint64_t seed, uint64_t step, uint32_t B,
+                     uint32_t* idx, hipStream_t stream);
 // Streaming bandwidth probes: kind 0 copies `bytes` from src to dst, kind 1
 // reads them (dst receives stream_probe_threads(bytes) doubles).
 uint32_t stream_probe_threads(size_t bytes);

@@ -11,12 +11,16 @@ namespace mlr {
 namespace {
 
 struct Node {
-    enum Op { Const, Var, Neg, Not, Add, Sub, Mul, Div, Pow, Lt, Le, Gt, Ge, Eq, Ne, And, Or, Assign };
+    enum Op { Const, Var, Neg, Not, Add, Sub, Mul, Div, Pow, Lt, Le, Gt, Ge, Eq, Ne, And, Or, Assign, Fn1, Fn2 };
     Op op;
     double c = 0.0;
-    int slot = -1;
+    int slot = -1;  // variable slot, or the function id of Fn1 / Fn2
     std::unique_ptr<Node> a, b;
 };
+
+// exprtk's common functions of one and two arguments
+const char* const kFn1[] = {"sin", "cos", "tan", "exp", "log", "sqrt", "abs", "tanh", "floor", "ceil"};
+const char* const kFn2[] = {"min", "max", "pow"};
 
 using P = std::unique_ptr<Node>;
 
@@ -99,6 +103,28 @@ struct Expression::Impl {
         }
         if (std::isalpha((unsigned char)c) || c == '_') {
             std::string id = ident();
+            ws();
+            if (p < s.size() && s[p] == '(') {
+                for (int f = 0; f < (int)(sizeof(kFn1) / sizeof(kFn1[0])); ++f)
+                    if (id == kFn1[f]) {
+                        ++p;
+                        P n = mk(Node::Fn1, logical_or());
+                        n->slot = f;
+                        if (!eat(")")) err("expected ')'");
+                        return n;
+                    }
+                for (int f = 0; f < (int)(sizeof(kFn2) / sizeof(kFn2[0])); ++f)
+                    if (id == kFn2[f]) {
+                        ++p;
+                        P a = logical_or();
+                        if (!eat(",")) err("expected ','");
+                        P n = mk(Node::Fn2, std::move(a), logical_or());
+                        n->slot = f;
+                        if (!eat(")")) err("expected ')'");
+                        return n;
+                    }
+                err("unknown function '" + id + "'");
+            }
             if (id == "pi") { P n = mk(Node::Const); n->c = M_PI; return n; }
             if (id == "e" && slot_of_opt(id) < 0) { P n = mk(Node::Const); n->c = M_E; return n; }
             if (id == "true") { P n = mk(Node::Const); n->c = 1.0; return n; }
@@ -206,6 +232,25 @@ struct Expression::Impl {
         case Node::And: return (ev(n->a.get(), vars) != 0.0 && ev(n->b.get(), vars) != 0.0) ? 1.0 : 0.0;
         case Node::Or: return (ev(n->a.get(), vars) != 0.0 || ev(n->b.get(), vars) != 0.0) ? 1.0 : 0.0;
         case Node::Assign: return vars[n->slot] = ev(n->a.get(), vars);
+        case Node::Fn1: {
+            const double v = ev(n->a.get(), vars);
+            switch (n->slot) {
+            case 0: return std::sin(v);
+            case 1: return std::cos(v);
+            case 2: return std::tan(v);
+            case 3: return std::exp(v);
+            case 4: return std::log(v);
+            case 5: return std::sqrt(v);
+            case 6: return std::fabs(v);
+            case 7: return std::tanh(v);
+            case 8: return std::floor(v);
+            default: return std::ceil(v);
+            }
+        }
+        case Node::Fn2: {
+            const double u = ev(n->a.get(), vars), v = ev(n->b.get(), vars);
+            return n->slot == 0 ? std::fmin(u, v) : (n->slot == 1 ? std::fmax(u, v) : std::pow(u, v));
+        }
         }
         return 0.0;
     }

@@ -2,7 +2,8 @@
 // (solver_initialize.cpp:101-186). It covers the exprtk subset Mallard
 // inputs use: ';'-separated statements, `var name := expr`, arithmetic
 // + - * / ^, comparisons returning 1/0, and/or/not (&&, ||, !), unary minus,
-// parentheses, the variables x and y, and the constants pi and e. The value
+// parentheses, the functions sin cos tan exp log sqrt abs tanh floor ceil
+// min max pow, the variables x and y, and the constants pi and e. The value
 // of the last statement is the result.
 #pragma once
 

@@ -14,6 +14,7 @@
 
 #include "ddpow.hpp"
 #include "device.hpp"
+#include "indicator.hpp"
 
 namespace mlr::dev {
 
@@ -334,8 +335,44 @@ __global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double*
     const bool valid = c < N;
     const uint32_t cc = valid ? c : 0;
     const int Mp = MP > 0 ? MP : t.M - 1;
+
+    // ---- learned gate: the MLP judges each cell from its central stencil
+    // (features from the LDS tile, one MFMA column per cell); a smooth cell
+    // skips its directional stencils and takes the central one.
+    bool smooth = false;
+    if (t.gate) {
+        const int lane = threadIdx.x & 63, gq = lane >> 4;
+        const uint32_t cm = lb * 64u + (threadIdx.x >> 6) * 16u + (lane & 15);
+        const bool vm = cm < N;
+        const D4 Uo = ld4(U, vm ? cm : 0);
+        const ind::CellRef r = ind::cell_ref(t.gamma, Uo.v);
+        const size_t crow = (size_t)lb * 4 * Mp;  // stencil 0 rows of this block
+        ind::v4s xb[3];
+#pragma unroll
+        for (int kc = 0; kc < 3; ++kc) {
+            const int m = 4 * kc + gq;
+            uint16_t f[4];
+            if (m == 0) {
+                ind::member_features(t.gamma, r, Uo.v, true, f);
+            } else {
+                const uint32_t pos = loc[(crow + (m - 1)) * 64 + (cm & 63u)];
+                const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
+                const double Um[4] = {u01.x, u01.y, u23.x, u23.y};
+                ind::member_features(t.gamma, r, Um, false, f);
+            }
+            if (!r.ok) f[0] = f[1] = f[2] = f[3] = 0;
+            xb[kc] = ind::pack4(f);
+        }
+        const float z = ind::mlp_wave(t.net, xb, lane);
+        const float prob = 1.0f / (1.0f + expf(-z));
+        const int sm = (vm && r.ok && prob >= t.net.threshold) ? 1 : 0;
+        const uint64_t bal = __ballot(sm && gq == 0);
+        if (lane == 0 && bal) atomicAdd(t.gated, (unsigned long long)__popcll(bal));
+        smooth = __shfl(sm, lane >> 2) != 0;
+    }
+
     const uint32_t info0 = g.info[cc];
-    const bool active = valid && ((info0 >> (20 + q)) & 1u);
+    const bool active = valid && ((info0 >> (20 + q)) & 1u) && !(smooth && q != 0);
     const D4 uc = ld4(U, cc);
 
     // ---- phase A: lane q = stencil
@@ -406,7 +443,12 @@ __global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double*
 #pragma unroll
     for (int st = 1; st < G; ++st) sd += ws[st];
     bool central = false;
-    if (ws[0] / (sd + ws[0]) > CT1) {
+    if (smooth) {
+        central = true;
+        ws[0] = 1.0;
+#pragma unroll
+        for (int st = 1; st < G; ++st) ws[st] = 0.0;
+    } else if (ws[0] / (sd + ws[0]) > CT1) {
         central = true;
         ws[0] = 1.0;
 #pragma unroll

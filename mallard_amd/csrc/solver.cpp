@@ -9,6 +9,7 @@
 #include <sstream>
 
 #include "expr.hpp"
+#include "learn.hpp"
 
 namespace mlr {
 
@@ -209,6 +210,9 @@ void Solver::free_device() {
     for (void* p : allocs_) (void)hipFree(p);
     allocs_.clear();
     dev_bytes_ = 0;
+    net_buf_ = nullptr;
+    theta_ = nullptr;
+    gate_ = false;
     for (auto& e : prof_) (void)hipEventDestroy(e.second);
     prof_.clear();
     prof_used_ = 0;
@@ -334,6 +338,7 @@ void Solver::upload() {
     dt_hist_cap_ = kDtHistCap;
     lam_max_ = (unsigned long long*)dz(1);
     step_dev_ = (int64_t*)dz(1);
+    gated_ = (unsigned long long*)dz(1);
     dec_ = (uint8_t*)dalloc(N);
     HIP_OK(hipMemset(dec_, 0, N));
 }
@@ -371,7 +376,89 @@ dev::Teno Solver::teno() const {
     const int Kp = L_.K - 1;
     for (int i = 0; i < Kp; ++i)
         for (int j = 0; j < Kp; ++j) t.OI[i * Kp + j] = L_.OI[(size_t)(i + 1) * L_.K + (j + 1)];
+    t.gamma = ph_.gamma;
+    t.gated = gated_;
+    t.gate = gate_ ? 1 : 0;
+    if (gate_) t.net = dev::net_view(net_buf_, gate_thr_, nullptr);
     return t;
+}
+
+void Solver::drop_graph() {
+    if (step_exec_) { HIP_OK(hipGraphExecDestroy(step_exec_)); step_exec_ = nullptr; }
+    if (step_graph_) { HIP_OK(hipGraphDestroy(step_graph_)); step_graph_ = nullptr; }
+}
+
+void Solver::set_indicator(const float* params, double threshold) {
+    if (!params) {
+        gate_ = false;
+        drop_graph();
+        return;
+    }
+    if (!L_.teno) fail("the learned indicator gates TENO reconstruction only");
+    if (L_.M < kIndFeatures / 4) fail("the learned indicator needs central stencils of at least 12 cells");
+    if (!(threshold > 0.0 && threshold <= 1.0)) fail("indicator threshold must lie in (0, 1]");
+    if (!net_buf_) {
+        HIP_OK(hipMalloc(&net_buf_, dev::net_bytes()));
+        allocs_.push_back(net_buf_);
+        void* th = nullptr;
+        HIP_OK(hipMalloc(&th, kIndParams * sizeof(float)));
+        allocs_.push_back(th);
+        theta_ = static_cast<float*>(th);
+    }
+    HIP_OK(hipMemcpyAsync(theta_, params, kIndParams * sizeof(float), hipMemcpyHostToDevice, stream_));
+    HIP_OK(dev::net_pack(theta_, net_buf_, stream_));
+    synchronize();
+    gate_thr_ = (float)threshold;
+    gate_ = true;
+    drop_graph();  // the captured step carries the gate in its kernel arguments
+}
+
+uint64_t Solver::gated_cells(bool reset) {
+    unsigned long long v = 0;
+    HIP_OK(hipMemcpyAsync(&v, gated_, sizeof(v), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    if (reset) {
+        const unsigned long long z = 0;
+        HIP_OK(hipMemcpyAsync(gated_, &z, sizeof(z), hipMemcpyHostToDevice, stream_));
+        synchronize();
+    }
+    return v;
+}
+
+// TENO decisions of the current state with the plain cut-off (the gate off),
+// then the features and labels of every cell appended to the set.
+void Solver::collect_features(Dataset& ds) {
+    if (!L_.teno || L_.M < kIndFeatures / 4) fail("features need TENO central stencils of at least 12 cells");
+    if (ds.n + L_.N > ds.cap) fail("dataset full");
+    const bool g = gate_;
+    gate_ = false;
+    try {
+        launch_teno(Un_);
+        HIP_OK(dev::ind_features(geom(), teno(), Un_, dec_, ph_.gamma, ds.X, ds.y, ds.n, stream_));
+        synchronize();
+    } catch (...) {
+        gate_ = g;
+        throw;
+    }
+    gate_ = g;
+    ds.n += L_.N;
+}
+
+void Solver::indicator_features(uint16_t* X_ref, float* y_ref) {
+    Dataset tmp(L_.N);
+    collect_features(tmp);
+    std::vector<uint16_t> X((size_t)L_.N * kIndFeatures);
+    std::vector<float> y(L_.N);
+    tmp.get(0, L_.N, X.data(), y.data());
+    for (uint32_t j = 0; j < L_.N; ++j) {
+        const uint32_t c = L_.perm[j];
+        if (X_ref) std::memcpy(X_ref + (size_t)c * kIndFeatures, &X[(size_t)j * kIndFeatures], kIndFeatures * 2);
+        if (y_ref) y_ref[c] = y[j];
+    }
+}
+
+void Solver::get_volumes(double* out) const {
+    for (uint32_t j = 0; j < L_.N; ++j) out[L_.perm[j]] = L_.volume[j];
 }
 
 dev::StageArgs Solver::stage(int op, const double* Uin, double* Uout) const {
@@ -462,9 +549,7 @@ void Solver::synchronize() { HIP_OK(hipStreamSynchronize(stream_)); }
 void Solver::set_cfl(double cfl) {
     cfg_.cfl = cfl;
     cfg_.use_cfl = true;
-    // the captured step carries cfl as a kernel argument
-    if (step_exec_) { HIP_OK(hipGraphExecDestroy(step_exec_)); step_exec_ = nullptr; }
-    if (step_graph_) { HIP_OK(hipGraphDestroy(step_graph_)); step_graph_ = nullptr; }
+    drop_graph();  // the captured step carries cfl as a kernel argument
 }
 
 void Solver::device_stencils(uint32_t cell, uint32_t* out) const {

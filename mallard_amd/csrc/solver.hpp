@@ -17,6 +17,8 @@
 
 namespace mlr {
 
+struct Dataset;
+
 // Layout of the stats array filled by Solver::run_steps_profiled.
 enum StepStats {
     kStatTotalMs = 0,   // first launch start to last launch end, event time
@@ -116,6 +118,15 @@ public:
     void launch_flux(int op, const double* Uin, double* Uout);
     double* d_state() { return Un_; }
     uint64_t device_bytes() const { return dev_bytes_; }  // all device allocations
+
+    // learned troubled-cell indicator: FP32 master parameters (7361) and the
+    // smooth-probability threshold; NULL turns the gate off
+    void set_indicator(const float* params, double threshold);
+    bool indicator_on() const { return gate_; }
+    uint64_t gated_cells(bool reset);      // cells judged smooth since the last reset
+    void collect_features(Dataset& ds);    // features + TENO labels of the current state
+    void indicator_features(uint16_t* X_ref, float* y_ref);  // same, reference numbering
+    void get_volumes(double* out) const;   // cell volumes, reference numbering
     int tile_pos_bytes() const { return loc8_ ? 1 : 2; }
     double* d_fs() { return fs_; }
 
@@ -130,6 +141,7 @@ private:
     void stage_pass(int op, const double* Uin, double* Uout);
     void dt_pass(double cfl, double fixed_dt);
     void mark(int tag);
+    void drop_graph();
     void finish_steps(int64_t s0, int64_t n, double* dts_out);
 
     Config cfg_;
@@ -155,6 +167,11 @@ private:
     double *fs_ = nullptr, *rhs_ = nullptr, *lam_cell_ = nullptr, *dt_dev_ = nullptr, *dt_hist_ = nullptr;
     unsigned long long* lam_max_ = nullptr;
     int64_t* step_dev_ = nullptr;
+    unsigned long long* gated_ = nullptr;
+    void* net_buf_ = nullptr;
+    float* theta_ = nullptr;
+    bool gate_ = false;
+    float gate_thr_ = 0.9f;
     uint8_t* dec_ = nullptr;
     int64_t dt_hist_cap_ = 0;
 
