@@ -217,6 +217,16 @@ int mlr_config_parse(const char* toml_path, mlr_mesh_gen* mesh, mlr_numerics_des
  * solver_initialize.cpp:101-186) at n points (x, y). */
 int mlr_expr_eval(const char* expr, int64_t n, const double* x, const double* y, double* out);
 
+/* DataWriter::write_vtu (data_writer.cpp:93-244): the context's state as one
+ * VTU file (Float64 cell arrays named RHO RHOU_X RHOU_Y RHOE U_X U_Y P T H
+ * CFL, in the order given). The [[write_data]] tables of a TOML input drive
+ * the same writer from mlr_run. */
+int mlr_write_vtu(mlr_ctx* ctx, const char* path, const char* const* variables, int n_vars);
+/* Host-only variant on a layout's mesh with caller arrays (reference
+ * numbering: U[N][4], prim[N][5], cfl[N]). */
+int mlr_layout_write_vtu(const mlr_layout* lay, const char* path, const char* const* variables, int n_vars,
+                         const double* U, const double* prim, const double* cfl);
+
 /* ---- learned troubled-cell indicator (no reference counterpart; the
  * BF16-MFMA MLP that lets cells judged smooth skip the directional TENO
  * stencils). Parameters are the FP32 master vector of 7361 values:

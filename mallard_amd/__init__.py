@@ -85,6 +85,8 @@ _SIGS = {
     "mlr_probe_math": (C.c_int, [C.c_int, _P, _P, _P, C.c_int]),
     "mlr_probe_stream": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "mlr_config_parse": (C.c_int, [C.c_char_p, C.POINTER(MlrMeshGen), C.POINTER(MlrNumerics), _P, _P]),
+    "mlr_write_vtu": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_char_p), C.c_int]),
+    "mlr_layout_write_vtu": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_char_p), C.c_int, _P, _P, _P]),
     "mlr_expr_eval": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, _P]),
     "mlr_dataset_create": (C.c_int, [C.c_int64, C.POINTER(_P)]),
     "mlr_dataset_destroy": (C.c_int, [_P]),
@@ -225,6 +227,13 @@ class Layout:
         lib().mlr_layout_teno_tables(self.h, _p(ipt), _p(oi))
         return ipt, oi.reshape(self.K, self.K)
 
+    def write_vtu(self, path, variables, U, prim, cfl):
+        """The VTU writer on this layout's mesh with caller arrays (host only)."""
+        names = (C.c_char_p * len(variables))(*[v.encode() for v in variables])
+        if lib().mlr_layout_write_vtu(self.h, os.fspath(path).encode(), names, len(variables), _p(_f64(U)),
+                                      _p(_f64(prim)), _p(_f64(cfl))):
+            raise MallardError(lib().mlr_last_error(None).decode())
+
     def slots(self, cell):
         S = self.S
         nb = np.zeros(S, np.uint32)
@@ -340,6 +349,10 @@ class Solver:
         p = np.zeros(self.n_cells, np.uint32)
         self._check(lib().mlr_get_permutation(self.h, _p(p)))
         return p
+
+    def write_vtu(self, path, variables=("RHO", "RHOU_X", "RHOU_Y", "RHOE", "U_X", "U_Y", "P", "T", "H", "CFL")):
+        names = (C.c_char_p * len(variables))(*[v.encode() for v in variables])
+        self._check(lib().mlr_write_vtu(self.h, os.fspath(path).encode(), names, len(variables)))
 
     # ---- learned troubled-cell indicator
     def set_indicator(self, params, threshold=0.9):

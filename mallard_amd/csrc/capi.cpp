@@ -619,6 +619,29 @@ int mlr_expr_eval(const char* expr, int64_t n, const double* x, const double* y,
     });
 }
 
+// ---------------------------------------------------------------- output
+int mlr_write_vtu(mlr_ctx* ctx, const char* path, const char* const* variables, int n_vars) {
+    if (!ctx || !path || !variables || n_vars <= 0) return 1;
+    return guarded(ctx, [&] {
+        const mlr::Mesh& m = ctx->solver.mesh();
+        const size_t N = m.n_cells;
+        std::vector<double> U(4 * N), W(5 * N), cfl(N);
+        ctx->solver.get_state(U.data(), W.data());
+        ctx->solver.get_cfl_local(cfl.data());
+        std::vector<std::string> v(variables, variables + n_vars);
+        mlr::write_vtu(path, m, v, U.data(), W.data(), cfl.data());
+    });
+}
+
+int mlr_layout_write_vtu(const mlr_layout* lay, const char* path, const char* const* variables, int n_vars,
+                         const double* U, const double* prim, const double* cfl) {
+    if (!lay || !path || !variables || n_vars <= 0 || !U || !prim || !cfl) return 1;
+    return guarded(nullptr, [&] {
+        std::vector<std::string> v(variables, variables + n_vars);
+        mlr::write_vtu(path, lay->mesh, v, U, prim, cfl);
+    });
+}
+
 // ---------------------------------------------------------------- learned indicator
 int mlr_dataset_create(int64_t capacity, mlr_dataset** out) {
     if (!out || capacity <= 0) return 1;

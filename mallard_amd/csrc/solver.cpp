@@ -9,6 +9,7 @@
 #include <sstream>
 
 #include "expr.hpp"
+#include "io.hpp"
 #include "learn.hpp"
 
 namespace mlr {
@@ -154,6 +155,29 @@ Config config_from_toml(const toml::Value& in) {
     c.t_stop = find_or(in, {"run", "t_stop"}, -1.0);
     c.t_wall_stop = find_or(in, {"run", "t_wall_stop"}, -1.0);
     c.check_interval = (uint32_t)find_or_int(in, {"output", "check_interval"}, 1);
+
+    // [[write_data]] (solver.cpp:283-295, data_writer.cpp:29-77)
+    if (const Value* wd = in.get("write_data")) {
+        if (wd->kind != Value::Array) fail("write_data must be an array of tables.");
+        for (const Value& o : wd->arr) {
+            for (const char* k : {"prefix", "format", "interval", "variables"})
+                if (!o.contains(k)) fail(std::string("DataWriter: ") + k + " not specified.");
+            WriteSpec w;
+            w.prefix = find(o, "prefix").as_string();
+            w.interval = (uint32_t)find(o, "interval").as_int();
+            const std::string fmt = find(o, "format").as_string();
+            if (fmt == "tecplot") fail("DataWriter::write_tecplot not implemented.");
+            if (fmt != "vtu") fail("DataWriter: Unknown format type: " + fmt + ".");
+            const Value& vars = find(o, "variables");
+            if (vars.kind != Value::Array || vars.arr.empty()) fail("DataWriter: No variables specified.");
+            for (const Value& v : vars.arr) {
+                if (!is_output_variable(v.as_string())) fail("DataWriter: Unknown variable: " + v.as_string() + ".");
+                w.variables.push_back(v.as_string());
+            }
+            if (w.interval == 0) fail("DataWriter: interval must be positive.");
+            c.writers.push_back(w);
+        }
+    }
 
     // [initialize] (solver_initialize.cpp:35-186; "quadrant" is this engine's
     // shorthand for the four-state Riemann problems)
@@ -832,6 +856,30 @@ void Solver::check_fields() {
 }
 
 // solver.cpp:352-403 (checks and VTU output are not part of this engine)
+// solver.cpp:421-425: the step-info print every check_interval steps
+void Solver::print_step_info() const {
+    std::printf("Starting step: %lld time: %g dt: %g\n", (long long)step_, t_, dt_);
+}
+
+// solver.cpp:500-504 and data_writer.cpp:78-92: every writer whose interval
+// divides the step (or all of them when forced) writes the current state
+void Solver::write_data(bool force) {
+    bool any = false;
+    for (const WriteSpec& w : cfg_.writers) any |= force || step_ % w.interval == 0;
+    if (!any) return;
+    const size_t N = mesh_.n_cells;
+    std::vector<double> U(4 * N), W(5 * N), cfl(N);
+    get_state(U.data(), W.data());
+    get_cfl_local(cfl.data());
+    for (const WriteSpec& w : cfg_.writers)
+        if (force || step_ % w.interval == 0) write_vtu(vtu_name(w.prefix, (uint64_t)step_), mesh_, w.variables,
+                                                      U.data(), W.data(), cfl.data());
+}
+
+// solver.cpp:352-403. With no time or wall-clock stop, no NaN check and no
+// output, the steps run as one graph-replayed batch; otherwise step by step
+// (each step still one graph launch) so the stop conditions, checks and
+// writers see every step.
 int Solver::run() {
     auto done = [&]() {
         bool d = false;
@@ -840,16 +888,20 @@ int Solver::run() {
         if (cfg_.t_wall_stop > 0) d |= (wall_seconds() - wall0_) >= cfg_.t_wall_stop;
         return d;
     };
-    const bool batch = cfg_.t_stop <= 0 && cfg_.t_wall_stop <= 0 && !cfg_.check_nan;
+    const bool batch = cfg_.t_stop <= 0 && cfg_.t_wall_stop <= 0 && !cfg_.check_nan && cfg_.writers.empty();
     if (batch) {
         const int64_t n = cfg_.n_steps - step_;
         if (n > 0) run_steps(n, nullptr);
         return 0;
     }
+    write_data(true);
     while (!done()) {
+        if (cfg_.check_interval > 0 && step_ % cfg_.check_interval == 0) print_step_info();
         run_steps(1, nullptr);
         check_fields();
+        write_data(false);
     }
+    write_data(true);
     return 0;
 }
 

@@ -11,6 +11,7 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "io.hpp"
 #include "layout.hpp"
 #include "mesh.hpp"
 #include "toml.hpp"
@@ -57,8 +58,9 @@ struct Config {
     double cfl = 0.5, dt = 0.0;
     int64_t n_steps = -1;
     double t_stop = -1.0, t_wall_stop = -1.0;
-    // [output]
+    // [output], [[write_data]]
     uint32_t check_interval = 1;
+    std::vector<WriteSpec> writers;
     InitSpec init;
     int n_threads = 0;  // host precompute threads (0: OpenMP default)
 };
@@ -98,6 +100,8 @@ public:
     void get_decisions(uint8_t* out);               // [N] bit v = central stencil for variable v
     void get_cfl_local(double* out);
     void check_fields();                            // solver.cpp:464-498
+    void write_data(bool force);                    // solver.cpp:500-504 (VTU writers)
+    void print_step_info() const;
     void synchronize();
     void set_cfl(double cfl);
     // stencil ids of one reference cell as uploaded (reference numbering, G x M)
