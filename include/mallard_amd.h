@@ -170,6 +170,10 @@ int mlr_get_teno_matrices(const mlr_ctx* ctx, uint32_t cell, double* aplus, doub
 /* Device-side stencil ids of one cell mapped back to reference numbering
  * (the uploaded layout, target cell first), ids[G*M]. */
 int mlr_get_device_stencils(const mlr_ctx* ctx, uint32_t cell, uint32_t* ids);
+/* The uploaded pseudo-inverses of one cell (aplus[G*K*M], K x M per stencil
+ * with the zero first row and column) and member areas (ta[G*M], entry 0 of
+ * each stencil left 0): for checking the GPU precompute against the host's. */
+int mlr_get_device_matrices(const mlr_ctx* ctx, uint32_t cell, double* aplus, double* ta);
 
 /* Host-only precompute (no GPU needed): the mesh generator, the SFC
  * renumbering and the packed TENO layout exactly as a context would upload

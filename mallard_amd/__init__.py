@@ -73,6 +73,7 @@ _SIGS = {
     "mlr_get_stencils": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "mlr_get_teno_matrices": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
     "mlr_get_device_stencils": (C.c_int, [_P, C.c_uint32, _P]),
+    "mlr_get_device_matrices": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "mlr_layout_create": (C.c_int, [C.POINTER(MlrMeshGen), C.POINTER(MlrNumerics), C.POINTER(_P)]),
     "mlr_layout_destroy": (C.c_int, [_P]),
     "mlr_layout_sizes": (C.c_int, [_P, _P]),
@@ -384,6 +385,12 @@ class Solver:
         v = np.zeros(self.n_cells)
         self._check(lib().mlr_get_volumes(self.h, _p(v)))
         return v
+
+    def device_matrices(self, cell):
+        a = np.zeros((self.G, self.K, self.M))
+        t = np.zeros((self.G, self.M))
+        self._check(lib().mlr_get_device_matrices(self.h, cell, _p(a), _p(t)))
+        return a, t
 
     def device_stencils(self, cell):
         ids = np.zeros(self.G * self.M, np.uint32)

@@ -386,6 +386,11 @@ int mlr_get_teno_matrices(const mlr_ctx* ctx, uint32_t cell, double* aplus, doub
     });
 }
 
+int mlr_get_device_matrices(const mlr_ctx* ctx, uint32_t cell, double* aplus, double* ta) {
+    if (!ctx || !aplus || !ta) return 1;
+    return guarded(const_cast<mlr_ctx*>(ctx), [&] { ctx->solver.device_matrices(cell, aplus, ta); });
+}
+
 int mlr_get_device_stencils(const mlr_ctx* ctx, uint32_t cell, uint32_t* ids) {
     if (!ctx) return 1;
     return guarded(const_cast<mlr_ctx*>(ctx), [&] { ctx->solver.device_stencils(cell, ids); });
@@ -456,7 +461,7 @@ int mlr_layout_cell_stencils(const mlr_layout* lay, uint32_t cell, uint32_t* ids
         const mlr::Layout& L = lay->L;
         if (!L.teno) mlr::fail("stencils exist only for TENO");
         if (cell >= L.N) mlr::fail("cell out of range");
-        const uint32_t j = L.iperm[cell], N = L.N;
+        const uint32_t j = L.iperm[cell];
         const int Mp = L.M - 1;
         // read back through the uploaded form: the block's tile list at the
         // member's stored position
@@ -480,8 +485,8 @@ int mlr_layout_cell_matrices(const mlr_layout* lay, uint32_t cell, double* aplus
         const mlr::Layout& L = lay->L;
         if (!L.teno) mlr::fail("matrices exist only for TENO");
         if (cell >= L.N) mlr::fail("cell out of range");
-        const uint32_t j = L.iperm[cell], N = L.N;
-        const int K = L.K, M = L.M, Kp = K - 1, Mp = M - 1;
+        const uint32_t j = L.iperm[cell];
+        const int K = L.K, M = L.M, Kp = K - 1;
         // the target's transformed area is identical in all stencils; the
         // reference variant folds it into fbasis, so it is recovered from
         // the cell geometry here

@@ -107,6 +107,24 @@ hipError_t finalize_dt(const unsigned long long* lam_max_in, double cfl, double 
 hipError_t copy_state(const double* src, double* dst, uint32_t N, hipStream_t stream);
 hipError_t probe_math(const double* x, const double* y, double* out, int n, int which, hipStream_t stream);
 
+// TENO pseudo-inverses on the GPU (precompute.hip), bit-identical to the host
+// precompute for the Legendre basis.
+struct PreInput {
+    uint32_t N;
+    const double* xy;      // node coordinates [n_nodes][2]
+    const uint32_t* tri;   // nodes of each cell, new numbering, reference node order [N][3]
+    const int32_t* ids;    // stencil members (new numbering), blocked like the A+ table
+    const uint32_t* info;  // slot-0 info words (stencil mask in bits 20-23)
+    int32_t nq;            // cell quadrature points (<= 8)
+    double qp[16], qw[8];
+    uint8_t poly[20];      // [K][2] Legendre degrees
+    double ipt[10];
+};
+bool aplus_supported(int K, int M);
+// A: [N/64][4][M-1][K-1][64], ta: [N/64][4][M-1][64]; err |= 1 singular
+// frame, |= 2 a stencil needing the full-matrix branch
+hipError_t teno_aplus(const PreInput& in, int K, int M, double* A, double* ta, int* err, hipStream_t stream);
+
 // Learned indicator (indicator.hip). The network lives in one device buffer of
 // net_bytes() that net_pack fills from the FP32 master parameters.
 size_t net_bytes();

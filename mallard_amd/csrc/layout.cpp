@@ -1,5 +1,7 @@
 #include "layout.hpp"
 
+#include "device.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -323,6 +325,16 @@ struct TenoCtx {
         }
     }
 
+    // the member area member_row computes for the target itself
+    double target_area(uint32_t c, const double* w0, const double* Ji) const {
+        double t[3][2];
+        for (int l = 0; l < 3; ++l) {
+            for (int i = 0; i < 2; i++) t[l][i] = m.nc(m.node_of_cell(c, l), i) - w0[i];
+            gemv2(Ji, t[l], t[l]);
+        }
+        return tri_area(t[0], t[1], t[2]);
+    }
+
     void compute_ipt() {
         RingSearch rs(m);
         std::vector<uint32_t> cen;
@@ -484,7 +496,7 @@ void build_tiles(Layout& L) {
 }
 
 Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::vector<BoundarySpec>& bcs,
-                    int n_threads) {
+                    int n_threads, bool allow_device) {
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
 #endif
@@ -614,9 +626,15 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
     const int K = L.K, M = L.M, G = L.G, Kp = K - 1, Mp = M - 1, Q = (int)L.Q;
     L.basis0 = ts.variant == TenoVariant::Corrected ? basis_2d(ts.basis, 0, 0, 0.0, 0.0) - L.ipt[0] : 1.0;
     const size_t Np = L.n_pad();
+    L.device_aplus = allow_device && ts.basis == BasisType::Legendre && dev::aplus_supported(K, M) &&
+                     ctx.qc.n <= 8 && K <= 10;
+    L.qcell_pts = ctx.qc.pts;
+    L.qcell_w = ctx.qc.w;
     L.st_ids.assign((size_t)G * Mp * Np, 0);
-    L.st_ta.assign((size_t)G * Mp * Np, 0.0);
-    L.st_A.assign((size_t)G * Mp * Kp * Np, 0.0);
+    if (!L.device_aplus) {
+        L.st_ta.assign((size_t)G * Mp * Np, 0.0);
+        L.st_A.assign((size_t)G * Mp * Kp * Np, 0.0);
+    }
     L.fbasis.assign((size_t)S * Q * Kp * Np, 0.0);
     int err_code = 0;
     uint32_t err_cell = 0;
@@ -644,7 +662,16 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
             ctx.target_frame(c, w0, Ji);
             double ta0 = NAN;
             uint32_t mask = 0;
-            for (int s = 0; s < G; ++s) {
+            if (L.device_aplus) {
+                // stencil ids only; the GPU builds A+ and the areas at upload
+                ta0 = ctx.target_area(c, w0, Ji);
+                for (int s = 0; s < G; ++s) {
+                    if (St[s].empty()) continue;
+                    mask |= 1u << s;
+                    for (int i = 1; i < M; ++i) L.st_ids[L.mem_idx(s, i - 1, j)] = (int32_t)L.iperm[St[s][i]];
+                }
+            }
+            for (int s = 0; s < G && !L.device_aplus; ++s) {
                 if (St[s].empty()) continue;
                 mask |= 1u << s;
                 const bool col0 = ctx.stencil_matrix(c, St[s], w0, Ji, aplus.data(), ta.data(), work);

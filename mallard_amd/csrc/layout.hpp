@@ -82,6 +82,9 @@ struct Layout {
     std::vector<uint8_t> st_loc8;         // (mem_idx), when every list has <= 256 entries
     std::vector<uint16_t> st_loc16;       // (mem_idx), otherwise
     uint32_t tile_max = 0;                // longest list
+    // true: st_A / st_ta are left empty and computed on the GPU at upload
+    bool device_aplus = false;
+    std::vector<double> qcell_pts, qcell_w;  // cell quadrature (for the device precompute)
 
     uint64_t device_bytes() const;
 
@@ -109,8 +112,10 @@ static_assert(Layout::kTileCells == 64, "the blocked tables assume 64-cell block
 std::vector<uint32_t> sfc_order(const Mesh& m);
 
 // Builds the layout. bcs lists the boundary conditions in input order.
+// allow_device: leave the pseudo-inverses to the GPU precompute when it
+// supports the basis and stencil size (the host still searches stencils).
 Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts,
-                    const std::vector<BoundarySpec>& bcs, int n_threads = 0);
+                    const std::vector<BoundarySpec>& bcs, int n_threads = 0, bool allow_device = false);
 
 // Reference-numbered stencil lists recomputed for one cell (used by the
 // index-parity tests through the C-ABI): out = G stencils of M ids each (or

@@ -279,13 +279,59 @@ void Solver::setup(Mesh&& mesh) {
     if (cfg_.integrator != Integrator::FE && cfg_.integrator != Integrator::RK4 &&
         cfg_.integrator != Integrator::SSPRK3)
         fail("Unknown time integrator type.");
-    L_ = build_layout(mesh_, cfg_.recon, cfg_.teno, cfg_.bcs, cfg_.n_threads);
+    L_ = build_layout(mesh_, cfg_.recon, cfg_.teno, cfg_.bcs, cfg_.n_threads, /*allow_device=*/true);
     HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     upload();
     t_ = 0.0;
     step_ = 0;
     wall0_ = wall_seconds();
     init_solution();
+}
+
+void Solver::precompute_aplus() {
+    struct Tmp {
+        std::vector<void*> p;
+        ~Tmp() { for (void* q : p) (void)hipFree(q); }
+        void* get(size_t bytes) {
+            void* q = nullptr;
+            HIP_OK(hipMalloc(&q, bytes ? bytes : 8));
+            p.push_back(q);
+            return q;
+        }
+    } tmp;
+    const uint32_t N = L_.N;
+    std::vector<uint32_t> tri(3 * (size_t)N);
+    for (uint32_t j = 0; j < N; ++j)
+        for (int l = 0; l < 3; ++l) tri[3 * (size_t)j + l] = mesh_.node_of_cell(L_.perm[j], l);
+    dev::PreInput in{};
+    in.N = N;
+    void* xy = tmp.get(mesh_.node_coords.size() * sizeof(double));
+    void* tr = tmp.get(tri.size() * sizeof(uint32_t));
+    void* ids = tmp.get(L_.st_ids.size() * sizeof(int32_t));
+    int* err = (int*)tmp.get(sizeof(int));
+    HIP_OK(hipMemcpy(xy, mesh_.node_coords.data(), mesh_.node_coords.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(tr, tri.data(), tri.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ids, L_.st_ids.data(), L_.st_ids.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(err, 0, sizeof(int)));
+    in.xy = (const double*)xy;
+    in.tri = (const uint32_t*)tr;
+    in.ids = (const int32_t*)ids;
+    in.info = info_;
+    in.nq = (int32_t)L_.qcell_w.size();
+    if (in.nq > 8 || L_.K > 10) fail("device precompute: quadrature or basis too large");
+    for (int q = 0; q < in.nq; ++q) {
+        in.qp[2 * q] = L_.qcell_pts[2 * (size_t)q];
+        in.qp[2 * q + 1] = L_.qcell_pts[2 * (size_t)q + 1];
+        in.qw[q] = L_.qcell_w[q];
+    }
+    for (int k = 0; k < 2 * L_.K; ++k) in.poly[k] = L_.poly[k];
+    for (int k = 0; k < L_.K; ++k) in.ipt[k] = L_.ipt[k];
+    HIP_OK(dev::teno_aplus(in, L_.K, L_.M, A_, ta_, err, stream_));
+    HIP_OK(hipStreamSynchronize(stream_));
+    int e = 0;
+    HIP_OK(hipMemcpy(&e, err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e & 1) fail("Matrix is singular.");
+    if (e & 2) fail("a TENO stencil needs the full-matrix pseudo-inverse branch, which this layout does not store");
 }
 
 void Solver::upload() {
@@ -331,8 +377,17 @@ void Solver::upload() {
         else up(loc16_, L_.st_loc16);
         up(blk_off_, L_.blk_off);
         up(blk_ids_, L_.blk_ids);
-        up(ta_, L_.st_ta);
-        up(A_, L_.st_A);
+        if (L_.device_aplus) {
+            // pseudo-inverses and member areas built on the GPU from the
+            // stencil ids, the node coordinates and the cells' nodes
+            const size_t Np = L_.n_pad(), Mp = (size_t)L_.M - 1, Kp = (size_t)L_.K - 1;
+            A_ = (double*)dalloc((size_t)L_.G * Mp * Kp * Np * sizeof(double));
+            ta_ = (double*)dalloc((size_t)L_.G * Mp * Np * sizeof(double));
+            precompute_aplus();
+        } else {
+            up(ta_, L_.st_ta);
+            up(A_, L_.st_A);
+        }
         up(fbasis_, L_.fbasis);
         // host copies of the big tables are not needed any more
         std::vector<int32_t>().swap(L_.st_ids);
@@ -603,6 +658,26 @@ void Solver::device_stencils(uint32_t cell, uint32_t* out) const {
             int32_t id = -1;
             HIP_OK(hipMemcpy(&id, blk_ids_ + L_.blk_off[b] + pos, 4, hipMemcpyDeviceToHost));
             out[(size_t)s * L_.M + 1 + i] = on ? L_.perm[id] : 0xFFFFFFFFu;
+        }
+    }
+}
+
+void Solver::device_matrices(uint32_t cell, double* aplus, double* ta) const {
+    if (!L_.teno) fail("matrices exist only for TENO");
+    if (cell >= L_.N) fail("cell out of range");
+    const uint32_t j = L_.iperm[cell];
+    const int K = L_.K, M = L_.M, G = L_.G;
+    uint32_t info0 = 0;
+    HIP_OK(hipMemcpy(&info0, info_ + j, 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < (size_t)G * K * M; ++i) aplus[i] = 0.0;
+    for (size_t i = 0; i < (size_t)G * M; ++i) ta[i] = 0.0;
+    for (int s = 0; s < G; ++s) {
+        if (!((info0 >> (20 + s)) & 1u)) continue;
+        for (int i = 1; i < M; ++i) {
+            HIP_OK(hipMemcpy(&ta[(size_t)s * M + i], ta_ + L_.mem_idx(s, i - 1, j), 8, hipMemcpyDeviceToHost));
+            for (int k = 1; k < K; ++k)
+                HIP_OK(hipMemcpy(&aplus[((size_t)s * K + k) * M + i], A_ + L_.a_idx(s, i - 1, k - 1, j), 8,
+                                 hipMemcpyDeviceToHost));
         }
     }
 }

@@ -106,6 +106,9 @@ public:
     void set_cfl(double cfl);
     // stencil ids of one reference cell as uploaded (reference numbering, G x M)
     void device_stencils(uint32_t cell, uint32_t* ids) const;
+    // uploaded A+ (G x K x M, zero first row / column restored) and member
+    // areas (G x M, the target's area first) of one reference cell
+    void device_matrices(uint32_t cell, double* aplus, double* ta) const;
 
     const Config& config() const { return cfg_; }
     const Mesh& mesh() const { return mesh_; }
@@ -137,6 +140,7 @@ public:
 private:
     void setup(Mesh&& mesh);
     void upload();
+    void precompute_aplus();
     void free_device();
     void record_step();
     dev::Geom geom() const;

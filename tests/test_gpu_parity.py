@@ -238,3 +238,18 @@ def test_determinism(mlr):
         s.run_steps(20)
         out.append(s.state()[0])
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_gpu_precompute_matches_host(mlr, order):
+    """The pseudo-inverses and member areas the GPU builds at upload equal
+    the host precompute's bit for bit (Legendre basis, factor 2)."""
+    gen = mlr.mesh_gen("cartesian_tri", 18, 14, 1.0, 1.0, 0.2, 21)
+    num = mlr.numerics(recon="TENO", order=order, teno_variant="corrected")
+    dev = mlr.Solver(gen=gen, num=num)
+    L = mlr.Layout(gen, num)
+    for c in range(0, L.n_cells, 3):
+        a_d, t_d = dev.device_matrices(c)
+        a_h, t_h, _ = L.cell_matrices(c)
+        assert np.array_equal(a_d, a_h), c
+        assert np.array_equal(t_d[:, 1:], t_h[:, 1:]), c
