@@ -209,6 +209,25 @@ int mlr_layout_teno_tables(const mlr_layout* lay, double* ipt, double* oi);
 int mlr_layout_slots(const mlr_layout* lay, uint32_t cell, uint32_t* nb, uint32_t* info, double* nx,
                      double* ny, double* area, double* dxn);
 
+/* ---- batched ensembles (no reference counterpart): n independent cases with
+ * their own meshes, states and CFL time steps, every kernel phase of a step
+ * one launch over all cases driven by a device table of per-case
+ * descriptors and block offsets, the step captured in a hipGraph. Each case's
+ * results equal its standalone run bit for bit. The cases share the numerics
+ * (num) and differ in mesh (gens[n_cases]); FE is not supported. */
+typedef struct mlr_ensemble mlr_ensemble;
+int mlr_ensemble_create(const mlr_mesh_gen* gens, int n_cases, const mlr_numerics_desc* num, mlr_ensemble** out);
+int mlr_ensemble_destroy(mlr_ensemble* ens);
+int mlr_ensemble_sizes(const mlr_ensemble* ens, int64_t* n_cells);  /* [n_cases] */
+int mlr_ensemble_init_quadrant(mlr_ensemble* ens, int k, double xs, double ys, const double* states);
+int mlr_ensemble_get_state(mlr_ensemble* ens, int k, double* U, double* prim);
+int mlr_ensemble_get_time(mlr_ensemble* ens, int k, double* t, int64_t* step);
+int mlr_ensemble_run_steps(mlr_ensemble* ens, int64_t n);
+/* kernel-by-kernel with events (the stats layout of mlr_run_steps_profiled) */
+int mlr_ensemble_run_steps_profiled(mlr_ensemble* ens, int64_t n, double* stats);
+int mlr_ensemble_set_indicator(mlr_ensemble* ens, const float* params, double threshold);
+int mlr_ensemble_gated(mlr_ensemble* ens, int reset, int64_t* gated);
+
 /* Host-only input parsing (no GPU), for the input-compatibility tests.
  * Solver::init's TOML reading (solver.cpp:82-273, mesh.cpp:32-66,
  * face_reconstruction.cpp:109-152): mesh and numerics as the descriptors

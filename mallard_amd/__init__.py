@@ -86,6 +86,16 @@ _SIGS = {
     "mlr_probe_math": (C.c_int, [C.c_int, _P, _P, _P, C.c_int]),
     "mlr_probe_stream": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "mlr_config_parse": (C.c_int, [C.c_char_p, C.POINTER(MlrMeshGen), C.POINTER(MlrNumerics), _P, _P]),
+    "mlr_ensemble_create": (C.c_int, [_P, C.c_int, C.POINTER(MlrNumerics), C.POINTER(_P)]),
+    "mlr_ensemble_destroy": (C.c_int, [_P]),
+    "mlr_ensemble_sizes": (C.c_int, [_P, _P]),
+    "mlr_ensemble_init_quadrant": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, _P]),
+    "mlr_ensemble_get_state": (C.c_int, [_P, C.c_int, _P, _P]),
+    "mlr_ensemble_get_time": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "mlr_ensemble_run_steps": (C.c_int, [_P, C.c_int64]),
+    "mlr_ensemble_run_steps_profiled": (C.c_int, [_P, C.c_int64, _P]),
+    "mlr_ensemble_set_indicator": (C.c_int, [_P, _P, C.c_double]),
+    "mlr_ensemble_gated": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
     "mlr_write_vtu": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_char_p), C.c_int]),
     "mlr_layout_write_vtu": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_char_p), C.c_int, _P, _P, _P]),
     "mlr_expr_eval": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, _P]),
@@ -416,6 +426,65 @@ def eval_expr(expr, x, y):
     if lib().mlr_expr_eval(expr.encode(), len(x), _p(x), _p(y), _p(out)):
         raise MallardError(lib().mlr_last_error(None).decode())
     return out
+
+
+class Ensemble:
+    """Independent cases advanced together, one launch per kernel phase."""
+
+    def __init__(self, gens, num):
+        arr = (MlrMeshGen * len(gens))(*gens)
+        h = C.c_void_p()
+        if lib().mlr_ensemble_create(C.cast(arr, C.c_void_p), len(gens), C.byref(num), C.byref(h)):
+            raise MallardError(lib().mlr_last_error(None).decode())
+        self.h = h
+        self.n = len(gens)
+        s = np.zeros(self.n, np.int64)
+        lib().mlr_ensemble_sizes(self.h, _p(s))
+        self.n_cells = s
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.mlr_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise MallardError(lib().mlr_last_error(None).decode())
+
+    def init_quadrant(self, k, xs, ys, states):
+        self._check(lib().mlr_ensemble_init_quadrant(self.h, k, xs, ys, _p(_f64(states).reshape(16))))
+
+    def state(self, k):
+        U = np.zeros((int(self.n_cells[k]), 4))
+        W = np.zeros((int(self.n_cells[k]), 5))
+        self._check(lib().mlr_ensemble_get_state(self.h, k, _p(U), _p(W)))
+        return U, W
+
+    def time(self, k):
+        t, s = C.c_double(), C.c_int64()
+        lib().mlr_ensemble_get_time(self.h, k, C.byref(t), C.byref(s))
+        return t.value, s.value
+
+    def run_steps(self, n):
+        self._check(lib().mlr_ensemble_run_steps(self.h, n))
+
+    def run_steps_profiled(self, n):
+        st = np.zeros(9)
+        self._check(lib().mlr_ensemble_run_steps_profiled(self.h, n, _p(st)))
+        return dict(total_ms=st[0], teno_ms=st[1], teno_n=int(st[2]), flux_ms=st[3], flux_n=int(st[4]),
+                    dt_ms=st[5], dt_n=int(st[6]))
+
+    def set_indicator(self, params, threshold=0.9):
+        p = None if params is None else _params(params)
+        self._check(lib().mlr_ensemble_set_indicator(self.h, _p(p), threshold))
+
+    def gated(self, reset=False):
+        n = C.c_int64()
+        self._check(lib().mlr_ensemble_gated(self.h, int(reset), C.byref(n)))
+        return n.value
 
 
 N_FEATURES = 48

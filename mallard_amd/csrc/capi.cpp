@@ -9,6 +9,7 @@
 #include <string>
 
 #include "ddpow.hpp"
+#include "ensemble.hpp"
 #include "expr.hpp"
 #include "learn.hpp"
 #include "solver.hpp"
@@ -16,6 +17,10 @@
 struct mlr_ctx {
     mlr::Solver solver;
     std::string err;
+};
+
+struct mlr_ensemble {
+    std::unique_ptr<mlr::Ensemble> e;
 };
 
 struct mlr_dataset {
@@ -223,32 +228,32 @@ int mlr_get_sizes(const mlr_ctx* ctx, int64_t* s) {
     return 0;
 }
 
+static void quadrant_state(mlr::Solver& solver, double xs, double ys, const double* states) {
+    const mlr::Mesh& m = solver.mesh();
+    const auto& ph_cfg = solver.config();
+    const double R = ph_cfg.p_ref / (ph_cfg.T_ref * ph_cfg.rho_ref);
+    const double cp = R * ph_cfg.gamma / (ph_cfg.gamma - 1.0);
+    const double cv = cp / ph_cfg.gamma;
+    std::vector<double> U(4 * (size_t)m.n_cells), W(5 * (size_t)m.n_cells);
+    for (uint32_t c2 = 0; c2 < m.n_cells; ++c2) {
+        const bool r = m.cc(c2, 0) >= xs, t = m.cc(c2, 1) >= ys;
+        const int q = r ? (t ? 0 : 3) : (t ? 1 : 2);
+        const double rho = states[4 * q], u0 = states[4 * q + 1], u1 = states[4 * q + 2], p = states[4 * q + 3];
+        const double T = p / (rho * R);
+        const double e = cv * T;
+        const double E = e + 0.5 * (u0 * u0 + u1 * u1);
+        const double h = e + p / rho;
+        double* Uc = &U[4 * (size_t)c2];
+        Uc[0] = rho; Uc[1] = rho * u0; Uc[2] = rho * u1; Uc[3] = rho * E;
+        double* Wc = &W[5 * (size_t)c2];
+        Wc[0] = u0; Wc[1] = u1; Wc[2] = p; Wc[3] = T; Wc[4] = h;
+    }
+    solver.set_state(U.data(), W.data());
+}
+
 int mlr_init_quadrant(mlr_ctx* ctx, double xs, double ys, const double* states) {
-    if (!ctx) return 1;
-    return guarded(ctx, [&] {
-        mlr::Config c = ctx->solver.config();
-        (void)c;
-        const mlr::Mesh& m = ctx->solver.mesh();
-        const auto& ph_cfg = ctx->solver.config();
-        const double R = ph_cfg.p_ref / (ph_cfg.T_ref * ph_cfg.rho_ref);
-        const double cp = R * ph_cfg.gamma / (ph_cfg.gamma - 1.0);
-        const double cv = cp / ph_cfg.gamma;
-        std::vector<double> U(4 * (size_t)m.n_cells), W(5 * (size_t)m.n_cells);
-        for (uint32_t c2 = 0; c2 < m.n_cells; ++c2) {
-            const bool r = m.cc(c2, 0) >= xs, t = m.cc(c2, 1) >= ys;
-            const int q = r ? (t ? 0 : 3) : (t ? 1 : 2);
-            const double rho = states[4 * q], u0 = states[4 * q + 1], u1 = states[4 * q + 2], p = states[4 * q + 3];
-            const double T = p / (rho * R);
-            const double e = cv * T;
-            const double E = e + 0.5 * (u0 * u0 + u1 * u1);
-            const double h = e + p / rho;
-            double* Uc = &U[4 * (size_t)c2];
-            Uc[0] = rho; Uc[1] = rho * u0; Uc[2] = rho * u1; Uc[3] = rho * E;
-            double* Wc = &W[5 * (size_t)c2];
-            Wc[0] = u0; Wc[1] = u1; Wc[2] = p; Wc[3] = T; Wc[4] = h;
-        }
-        ctx->solver.set_state(U.data(), W.data());
-    });
+    if (!ctx || !states) return 1;
+    return guarded(ctx, [&] { quadrant_state(ctx->solver, xs, ys, states); });
 }
 
 int mlr_init_constant(mlr_ctx* ctx, const double* u, double p, double T) {
@@ -621,6 +626,85 @@ int mlr_expr_eval(const char* expr, int64_t n, const double* x, const double* y,
     return guarded(nullptr, [&] {
         const mlr::Expression e(expr);
         for (int64_t i = 0; i < n; ++i) out[i] = e.eval(x[i], y[i]);
+    });
+}
+
+// ---------------------------------------------------------------- ensembles
+int mlr_ensemble_create(const mlr_mesh_gen* gens, int n_cases, const mlr_numerics_desc* num, mlr_ensemble** out) {
+    if (!out || !gens || n_cases <= 0) return 1;
+    *out = nullptr;
+    auto ens = std::make_unique<mlr_ensemble>();
+    const int rc = guarded(nullptr, [&] {
+        const mlr::Config c = config_from_desc(num);
+        std::vector<mlr::MeshSpec> ms((size_t)n_cases);
+        for (int k = 0; k < n_cases; ++k) {
+            const mlr_mesh_gen& g = gens[k];
+            if (g.type == MLR_MESH_CARTESIAN) ms[k].type = mlr::MeshType::Cartesian;
+            else if (g.type == MLR_MESH_CARTESIAN_TRI) ms[k].type = mlr::MeshType::CartesianTri;
+            else if (g.type == MLR_MESH_WEDGE) ms[k].type = mlr::MeshType::Wedge;
+            else mlr::fail("MeshType::FILE not implemented.");
+            ms[k].nx = g.nx;
+            ms[k].ny = g.ny;
+            ms[k].Lx = g.Lx;
+            ms[k].Ly = g.Ly;
+            ms[k].jitter = g.jitter;
+            ms[k].seed = g.seed;
+        }
+        ens->e = std::make_unique<mlr::Ensemble>(c, ms);
+    });
+    if (rc == 0) *out = ens.release();
+    return rc;
+}
+
+int mlr_ensemble_destroy(mlr_ensemble* ens) {
+    delete ens;
+    return 0;
+}
+
+int mlr_ensemble_sizes(const mlr_ensemble* ens, int64_t* n_cells) {
+    if (!ens || !n_cells) return 1;
+    for (int k = 0; k < ens->e->size(); ++k) n_cells[k] = ens->e->solver(k).mesh().n_cells;
+    return 0;
+}
+
+int mlr_ensemble_init_quadrant(mlr_ensemble* ens, int k, double xs, double ys, const double* states) {
+    if (!ens || !states || k < 0 || k >= ens->e->size()) return 1;
+    return guarded(nullptr, [&] { quadrant_state(ens->e->solver(k), xs, ys, states); });
+}
+
+int mlr_ensemble_get_state(mlr_ensemble* ens, int k, double* U, double* prim) {
+    if (!ens || k < 0 || k >= ens->e->size()) return 1;
+    return guarded(nullptr, [&] { ens->e->solver(k).get_state(U, prim); });
+}
+
+int mlr_ensemble_get_time(mlr_ensemble* ens, int k, double* t, int64_t* step) {
+    if (!ens || k < 0 || k >= ens->e->size()) return 1;
+    if (t) *t = ens->e->solver(k).time();
+    if (step) *step = ens->e->solver(k).step();
+    return 0;
+}
+
+int mlr_ensemble_run_steps(mlr_ensemble* ens, int64_t n) {
+    if (!ens) return 1;
+    return guarded(nullptr, [&] { ens->e->run_steps(n); });
+}
+
+int mlr_ensemble_run_steps_profiled(mlr_ensemble* ens, int64_t n, double* stats) {
+    if (!ens || !stats) return 1;
+    return guarded(nullptr, [&] { ens->e->run_steps_profiled(n, stats); });
+}
+
+int mlr_ensemble_set_indicator(mlr_ensemble* ens, const float* params, double threshold) {
+    if (!ens) return 1;
+    return guarded(nullptr, [&] { ens->e->set_indicator(params, threshold); });
+}
+
+int mlr_ensemble_gated(mlr_ensemble* ens, int reset, int64_t* gated) {
+    if (!ens || !gated) return 1;
+    return guarded(nullptr, [&] {
+        int64_t g = 0;
+        for (int k = 0; k < ens->e->size(); ++k) g += (int64_t)ens->e->solver(k).gated_cells(reset != 0);
+        *gated = g;
     });
 }
 

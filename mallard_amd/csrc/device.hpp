@@ -95,6 +95,24 @@ struct StageArgs {
     const double* dt;     // device scalar
 };
 
+// One case of a batched ensemble: its geometry, stencils and buffers, and the
+// first block of the case in the batched TENO grid (64-cell blocks) and in the
+// flux / spectral grids (256-cell blocks).
+struct CaseBufs {
+    Geom g;
+    Teno t;
+    double *Un, *Ua, *Ub, *W, *prims, *h, *fs, *rhs, *lam_cell, *dt, *dt_hist;
+    unsigned long long* lam_max;
+    int64_t* step;
+    uint8_t* dec;
+    uint32_t blk_teno0, blk_flux0;
+};
+enum BufRole : int32_t { kBufUn = 0, kBufUa = 1, kBufUb = 2, kBufNone = 3 };
+struct BatchStage {
+    int32_t op, fo, vacuum_guard;
+    int32_t in, out;  // BufRole of the stage input and output
+};
+
 // Launchers (asynchronous on `stream`). Each returns a hipError_t.
 hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs, uint8_t* decisions,
                       const Physics& ph, hipStream_t stream);
@@ -105,6 +123,17 @@ hipError_t finalize_dt(const unsigned long long* lam_max_in, double cfl, double 
                        double* dt_hist, int64_t hist_cap, int64_t* step, unsigned long long* lam_max_reset,
                        hipStream_t stream);
 hipError_t copy_state(const double* src, double* dst, uint32_t N, hipStream_t stream);
+// Ensemble launchers: `cases` is a device array of n_cases descriptors whose
+// blk_* offsets are prefix sums; every case has the same K, M, face points,
+// tile-position width (small = 8-bit) and Riemann solver.
+hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
+                      bool small, const Physics& ph, hipStream_t stream);
+hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const BatchStage& st, int riemann,
+                      const Physics& ph, hipStream_t stream);
+hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const Physics& ph,
+                          hipStream_t stream);
+hipError_t finalize_batch(const CaseBufs* cases, int n_cases, double cfl, double fixed_dt, int64_t hist_cap,
+                          hipStream_t stream);
 hipError_t probe_math(const double* x, const double* y, double* out, int n, int which, hipStream_t stream);
 
 // TENO pseudo-inverses on the GPU (precompute.hip), bit-identical to the host

@@ -1,0 +1,187 @@
+#include "ensemble.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+namespace mlr {
+
+namespace {
+
+void hip_ok(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+Ensemble::Ensemble(const Config& base, const std::vector<MeshSpec>& meshes) : cfg_(base) {
+    if (meshes.empty()) fail("an ensemble needs at least one case");
+    if (cfg_.integrator == Integrator::FE) fail("ensembles run the RK4 and SSPRK3 integrators");
+    cases_.reserve(meshes.size());
+    for (const MeshSpec& ms : meshes) {
+        Config c = cfg_;
+        c.mesh_type = ms.type;
+        c.nx = ms.nx;
+        c.ny = ms.ny;
+        c.Lx = ms.Lx;
+        c.Ly = ms.Ly;
+        c.jitter = ms.jitter;
+        c.seed = ms.seed;
+        cases_.emplace_back(new Solver());
+        cases_.back()->init(c);
+    }
+    const Solver& s0 = *cases_[0];
+    teno_ = s0.L_.teno;
+    ph_ = s0.ph_;
+    if (teno_) {
+        K_ = s0.L_.K;
+        M_ = s0.L_.M;
+        FQ_ = (int)(s0.L_.S * s0.L_.Q);
+        small_ = s0.loc8_ != nullptr;
+    }
+    for (const auto& c : cases_) {
+        if (c->L_.teno != teno_ || (teno_ && (c->L_.K != K_ || c->L_.M != M_ ||
+                                              (int)(c->L_.S * c->L_.Q) != FQ_ || (c->loc8_ != nullptr) != small_)))
+            fail("ensemble cases must share the reconstruction layout (order, stencil size, tile width)");
+    }
+    hip_ok(hipStreamCreate(&stream_), "hipStreamCreate");
+    hip_ok(hipMalloc(&d_cases_, cases_.size() * sizeof(dev::CaseBufs)), "hipMalloc");
+    upload_table();
+}
+
+Ensemble::~Ensemble() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    for (auto& e : prof_) (void)hipEventDestroy(e.second);
+    if (d_cases_) (void)hipFree(d_cases_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+uint64_t Ensemble::n_cells() const {
+    uint64_t n = 0;
+    for (const auto& c : cases_) n += c->L_.N;
+    return n;
+}
+
+void Ensemble::upload_table() {
+    std::vector<dev::CaseBufs> tab;
+    tab.reserve(cases_.size());
+    uint32_t bt = 0, bf = 0;
+    for (const auto& c : cases_) {
+        dev::CaseBufs b = c->case_bufs();
+        b.blk_teno0 = bt;
+        b.blk_flux0 = bf;
+        bt += (c->L_.N + 63) / 64;
+        bf += (c->L_.N + 255) / 256;
+        tab.push_back(b);
+    }
+    teno_blocks_ = bt;
+    flux_blocks_ = bf;
+    hip_ok(hipMemcpy(d_cases_, tab.data(), tab.size() * sizeof(dev::CaseBufs), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+void Ensemble::drop_graph() {
+    if (exec_) { hip_ok(hipGraphExecDestroy(exec_), "hipGraphExecDestroy"); exec_ = nullptr; }
+    if (graph_) { hip_ok(hipGraphDestroy(graph_), "hipGraphDestroy"); graph_ = nullptr; }
+}
+
+void Ensemble::set_indicator(const float* params, double threshold) {
+    for (auto& c : cases_) c->set_indicator(params, threshold);
+    upload_table();
+    drop_graph();
+}
+
+void Ensemble::mark(int tag) {
+    if (!prof_on_) return;
+    if (prof_used_ == prof_.size()) {
+        hipEvent_t e;
+        hip_ok(hipEventCreate(&e), "hipEventCreate");
+        prof_.push_back({tag, e});
+    }
+    prof_[prof_used_].first = tag;
+    hip_ok(hipEventRecord(prof_[prof_used_].second, stream_), "hipEventRecord");
+    ++prof_used_;
+}
+
+void Ensemble::launch_step(bool prof) {
+    (void)prof;
+    const int n = size();
+    const int64_t cap = cases_[0]->dt_hist_cap_;
+    if (cfg_.use_cfl) hip_ok(dev::spectral_batch(d_cases_, n, flux_blocks_, ph_, stream_), "spectral_batch");
+    hip_ok(dev::finalize_batch(d_cases_, n, cfg_.cfl, cfg_.use_cfl ? 0.0 : cfg_.dt, cap, stream_), "finalize_batch");
+    mark(kStatDtMs);
+    const int vac = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
+    auto stage = [&](int op, int in, int out) {
+        if (teno_) {
+            hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, ph_, stream_), "teno_batch");
+            mark(kStatTenoMs);
+        }
+        const dev::BatchStage st{op, teno_ ? 0 : 1, vac, in, out};
+        hip_ok(dev::flux_batch(d_cases_, n, flux_blocks_, st, (int)cfg_.riemann, ph_, stream_), "flux_batch");
+        mark(kStatFluxMs);
+    };
+    if (cfg_.integrator == Integrator::SSPRK3) {
+        stage(dev::kSSP1, dev::kBufUn, dev::kBufUa);
+        stage(dev::kSSP2, dev::kBufUa, dev::kBufUb);
+        stage(dev::kSSP3, dev::kBufUb, dev::kBufUn);
+    } else {
+        stage(dev::kRK4_1, dev::kBufUn, dev::kBufUa);
+        stage(dev::kRK4_2, dev::kBufUa, dev::kBufUb);
+        stage(dev::kRK4_3, dev::kBufUb, dev::kBufUa);
+        stage(dev::kRK4_4, dev::kBufUa, dev::kBufUn);
+    }
+}
+
+void Ensemble::finish(int64_t s0, int64_t n) {
+    for (auto& c : cases_) c->finish_steps(s0, n, nullptr);
+}
+
+void Ensemble::run_steps(int64_t n) {
+    if (n <= 0) return;
+    const int64_t s0 = cases_[0]->device_step();
+    if (!exec_) {
+        hip_ok(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            launch_step(false);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_ok(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
+        hip_ok(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    for (int64_t i = 0; i < n; ++i) hip_ok(hipGraphLaunch(exec_, stream_), "hipGraphLaunch");
+    hip_ok(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    finish(s0, n);
+}
+
+void Ensemble::run_steps_profiled(int64_t n, double* stats) {
+    for (int k = 0; k < kStatCount; ++k) stats[k] = 0.0;
+    if (n <= 0) return;
+    const int64_t s0 = cases_[0]->device_step();
+    prof_on_ = true;
+    prof_used_ = 0;
+    try {
+        mark(-1);
+        for (int64_t i = 0; i < n; ++i) launch_step(true);
+    } catch (...) {
+        prof_on_ = false;
+        throw;
+    }
+    prof_on_ = false;
+    hip_ok(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    for (size_t i = 1; i < prof_used_; ++i) {
+        float ms = 0.0f;
+        hip_ok(hipEventElapsedTime(&ms, prof_[i - 1].second, prof_[i].second), "hipEventElapsedTime");
+        stats[prof_[i].first] += ms;
+        stats[prof_[i].first + 1] += 1.0;
+    }
+    float tot = 0.0f;
+    hip_ok(hipEventElapsedTime(&tot, prof_[0].second, prof_[prof_used_ - 1].second), "hipEventElapsedTime");
+    stats[kStatTotalMs] = tot;
+    finish(s0, n);
+}
+
+}  // namespace mlr

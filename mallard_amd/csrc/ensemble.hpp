@@ -1,0 +1,65 @@
+// Batched ensembles: many independent cases (own meshes, states and CFL
+// time steps) advanced together, every kernel phase of a step one launch over
+// all cases' blocks driven by a device table of per-case descriptors and
+// block offsets, and the whole step captured in one hipGraph. Each case's
+// arithmetic is exactly its standalone solver's, so results are bit-identical
+// to running the cases one by one.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "solver.hpp"
+
+namespace mlr {
+
+struct MeshSpec {
+    MeshType type = MeshType::CartesianTri;
+    uint32_t nx = 64, ny = 64;
+    double Lx = 1.0, Ly = 1.0, jitter = 0.0;
+    uint64_t seed = 0;
+};
+
+class Ensemble {
+public:
+    // Builds one solver per mesh with the shared numerics of `base`.
+    Ensemble(const Config& base, const std::vector<MeshSpec>& meshes);
+    ~Ensemble();
+    Ensemble(const Ensemble&) = delete;
+    Ensemble& operator=(const Ensemble&) = delete;
+
+    int size() const { return (int)cases_.size(); }
+    Solver& solver(int k) { return *cases_.at((size_t)k); }
+    uint64_t n_cells() const;
+
+    // the learned indicator on every case (params NULL: off)
+    void set_indicator(const float* params, double threshold);
+    // n steps of every case, graph-replayed
+    void run_steps(int64_t n);
+    // the same launches one at a time with an event after each (Solver's StepStats layout)
+    void run_steps_profiled(int64_t n, double* stats);
+
+private:
+    void upload_table();
+    void launch_step(bool prof);
+    void mark(int tag);
+    void drop_graph();
+    void finish(int64_t s0, int64_t n);
+
+    Config cfg_;
+    std::vector<std::unique_ptr<Solver>> cases_;
+    dev::CaseBufs* d_cases_ = nullptr;
+    uint32_t teno_blocks_ = 0, flux_blocks_ = 0;
+    int K_ = 0, M_ = 0, FQ_ = 0;
+    bool small_ = true, teno_ = false;
+    dev::Physics ph_{};
+    hipStream_t stream_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    bool prof_on_ = false;
+    std::vector<std::pair<int, hipEvent_t>> prof_;
+    size_t prof_used_ = 0;
+};
+
+}  // namespace mlr

@@ -314,11 +314,9 @@ __device__ inline void quad_gather(double f, double* out, int q) {
 // mapped so that each of the 8 XCDs walks one contiguous range of the
 // space-filling-curve order.
 template <int KP, int MP, int FQ, typename IdxT, int TILE>
-__global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
-                                                   double* __restrict__ fs, uint8_t* __restrict__ dec,
-                                                   Physics ph, uint32_t blocks_per_xcd) {
-    __shared__ double2 sh_u[2 * TILE];
-    const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
+__device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const double* __restrict__ U,
+                                           double* __restrict__ fs, uint8_t* __restrict__ dec, const Physics& ph,
+                                           uint32_t lb, double2* sh_u) {
     const uint32_t N = g.N;
     if (lb * 64u >= N) return;  // whole block past the end (uniform)
     {
@@ -535,6 +533,47 @@ __global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double*
     }
 }
 
+template <int KP, int MP, int FQ, typename IdxT, int TILE>
+__global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
+                                                   double* __restrict__ fs, uint8_t* __restrict__ dec,
+                                                   Physics ph, uint32_t blocks_per_xcd) {
+    __shared__ double2 sh_u[2 * TILE];
+    const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
+    teno_block<KP, MP, FQ, IdxT, TILE>(g, t, U, fs, dec, ph, lb, sh_u);
+}
+
+// case of global block gb: the last case whose first block is <= gb
+__device__ inline int case_of_block(const CaseBufs* __restrict__ cs, int n, uint32_t gb, bool flux) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((flux ? cs[mid].blk_flux0 : cs[mid].blk_teno0) <= gb) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline const double* case_buf(const CaseBufs& c, int role) {
+    return role == kBufUn ? c.Un : (role == kBufUa ? c.Ua : c.Ub);
+}
+__device__ inline double* case_buf_w(const CaseBufs& c, int role) {
+    return role == kBufUn ? c.Un : (role == kBufUa ? c.Ua : (role == kBufUb ? c.Ub : nullptr));
+}
+
+// The TENO pass of every case of an ensemble in one launch: the grid covers
+// all cases' 64-cell blocks (XCD-aware order over the whole grid), each
+// block finds its case in the offset table and runs that case's block.
+template <int KP, int MP, int FQ, typename IdxT, int TILE>
+__global__ __launch_bounds__(256) void teno_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, int in_role,
+                                                         uint32_t total, Physics ph, uint32_t blocks_per_xcd) {
+    __shared__ double2 sh_u[2 * TILE];
+    const uint32_t gb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
+    if (gb >= total) return;
+    const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, false)];
+    teno_block<KP, MP, FQ, IdxT, TILE>(cs.g, cs.t, case_buf(cs, in_role), cs.fs, cs.dec, ph, gb - cs.blk_teno0,
+                                       sh_u);
+}
+
 // ------------------------------------------------------------ flux + update
 // One thread per cell: the residual of the cell in the canonical face order,
 // divided by the volume, then the stage's SSP-RK update (no atomics).
@@ -543,10 +582,8 @@ __global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double*
 // TENO kernel, so the neighbour face states a block gathers were written by
 // the same XCD.
 template <int RS>
-__global__ __launch_bounds__(256) void flux_update_kernel(Geom g, StageArgs a, Physics ph, uint32_t blocks_per_xcd) {
+__device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, const Physics& ph, uint32_t c) {
     const uint32_t N = g.N;
-    const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
-    const uint32_t c = lb * blockDim.x + threadIdx.x;
     if (c >= N) return;
     const uint32_t Q = g.Q;
     const uint32_t info0 = g.info[c];
@@ -662,15 +699,46 @@ __global__ __launch_bounds__(256) void flux_update_kernel(Geom g, StageArgs a, P
     }
 }
 
+template <int RS>
+__global__ __launch_bounds__(256) void flux_update_kernel(Geom g, StageArgs a, Physics ph, uint32_t blocks_per_xcd) {
+    const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
+    flux_cell<RS>(g, a, ph, lb * blockDim.x + threadIdx.x);
+}
+
+// The flux + stage update of every case in one launch (256-cell blocks).
+template <int RS>
+__global__ __launch_bounds__(256) void flux_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases,
+                                                         BatchStage st, uint32_t total, Physics ph,
+                                                         uint32_t blocks_per_xcd) {
+    const uint32_t gb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
+    if (gb >= total) return;
+    const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, true)];
+    StageArgs a{};
+    a.op = st.op;
+    a.riemann = RS;
+    a.fo = st.fo;
+    a.vacuum_guard = st.vacuum_guard;
+    a.Uin = case_buf(cs, st.in);
+    a.Un = cs.Un;
+    a.Uout = case_buf_w(cs, st.out);
+    a.W = cs.W;
+    a.prims = cs.prims;
+    a.h = cs.h;
+    a.rhs = cs.rhs;
+    a.fs = cs.fs;
+    a.dt = cs.dt;
+    flux_cell<RS>(cs.g, a, ph, (gb - cs.blk_flux0) * blockDim.x + threadIdx.x);
+}
+
 // ------------------------------------------------------------ dt
 // solver.cpp:592-706: per-cell spectral radius over the cell's faces in
 // local order, max over the grid through a block max and one atomic.
-__global__ __launch_bounds__(256) void spectral_kernel(Geom g, const double* __restrict__ Un,
-                                                       const double* __restrict__ pr, Physics ph,
-                                                       double* __restrict__ lam_cell,
-                                                       unsigned long long* lam_max) {
+__device__ __forceinline__ void spectral_block(const Geom& g, const double* __restrict__ Un,
+                                               const double* __restrict__ pr, const Physics& ph,
+                                               double* __restrict__ lam_cell, unsigned long long* lam_max,
+                                               uint32_t lb) {
     const uint32_t N = g.N;
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = lb * blockDim.x + threadIdx.x;
     double lam = 0.0;
     bool ok = false;
     if (c < N) {
@@ -716,6 +784,23 @@ __global__ __launch_bounds__(256) void spectral_kernel(Geom g, const double* __r
     }
 }
 
+__global__ __launch_bounds__(256) void spectral_kernel(Geom g, const double* __restrict__ Un,
+                                                       const double* __restrict__ pr, Physics ph,
+                                                       double* __restrict__ lam_cell,
+                                                       unsigned long long* lam_max) {
+    spectral_block(g, Un, pr, ph, lam_cell, lam_max, blockIdx.x);
+}
+
+// spectral radii of every case in one launch (each block inside one case, so
+// the block max goes to that case's maximum)
+__global__ __launch_bounds__(256) void spectral_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases,
+                                                             uint32_t total, Physics ph) {
+    const uint32_t gb = blockIdx.x;
+    if (gb >= total) return;
+    const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, true)];
+    spectral_block(cs.g, cs.Un, cs.prims, ph, cs.lam_cell, cs.lam_max, gb - cs.blk_flux0);
+}
+
 __global__ void finalize_dt_kernel(const unsigned long long* lam_max_in, double cfl, double fixed_dt,
                                    double* dt, double* dt_hist, int64_t hist_cap, int64_t* step,
                                    unsigned long long* lam_max_reset) {
@@ -730,6 +815,25 @@ __global__ void finalize_dt_kernel(const unsigned long long* lam_max_in, double 
     if (dt_hist && step) dt_hist[*step % hist_cap] = d;
     if (step) *step += 1;
     if (lam_max_reset) *lam_max_reset = 0ull;
+}
+
+// per-case dt of an ensemble: thread k finalises case k
+__global__ void finalize_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, double cfl, double fixed_dt,
+                                      int64_t hist_cap) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_cases) return;
+    const CaseBufs& cs = cases[k];
+    double d;
+    if (fixed_dt > 0.0) {
+        d = fixed_dt;
+    } else {
+        const double lm = __longlong_as_double((long long)*cs.lam_max);
+        d = cfl / lm;
+    }
+    *cs.dt = d;
+    cs.dt_hist[*cs.step % hist_cap] = d;
+    *cs.step += 1;
+    *cs.lam_max = 0ull;
 }
 
 __global__ void copy_kernel(const double* __restrict__ s, double* __restrict__ d, size_t n) {
@@ -848,6 +952,58 @@ hipError_t finalize_dt(const unsigned long long* lam_max_in, double cfl, double 
                        hipStream_t stream) {
     hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1), 0, stream, lam_max_in, cfl, fixed_dt, dt, dt_hist,
                        hist_cap, step, lam_max_reset);
+    return hipGetLastError();
+}
+
+hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
+                      bool small, const Physics& ph, hipStream_t stream) {
+    const int Kp = K - 1, Mp = M - 1;
+    if (n_cases < 1 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
+    const uint32_t per_xcd = (total_blocks + 7) / 8;
+    const dim3 grid(8 * per_xcd), block(256);
+#define MLR_TB(KP, MP, FQV)                                                                                    \
+    do {                                                                                                       \
+        if (small)                                                                                             \
+            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint8_t, 256>), grid, block, 0, stream, cases,      \
+                               n_cases, in_role, total_blocks, ph, per_xcd);                                    \
+        else                                                                                                   \
+            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint16_t, 1024>), grid, block, 0, stream, cases,    \
+                               n_cases, in_role, total_blocks, ph, per_xcd);                                    \
+    } while (0)
+    if (FQ == 3 && Kp == 5 && Mp == 11) MLR_TB(5, 11, 3);
+    else if (FQ == 3 && Kp == 5) MLR_TB(5, 0, 3);
+    else if (FQ == 3 && Kp == 2) MLR_TB(2, 0, 3);
+    else if (FQ == 6 && Kp == 9) MLR_TB(9, 0, 6);
+    else return hipErrorInvalidValue;
+#undef MLR_TB
+    return hipGetLastError();
+}
+
+hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const BatchStage& st, int riemann,
+                      const Physics& ph, hipStream_t stream) {
+    const uint32_t per_xcd = (total_blocks + 7) / 8;
+    const dim3 grid(8 * per_xcd), block(256);
+    switch (riemann) {
+    case 0: hipLaunchKernelGGL((flux_batch_kernel<0>), grid, block, 0, stream, cases, n_cases, st, total_blocks, ph, per_xcd); break;
+    case 1: hipLaunchKernelGGL((flux_batch_kernel<1>), grid, block, 0, stream, cases, n_cases, st, total_blocks, ph, per_xcd); break;
+    case 2: hipLaunchKernelGGL((flux_batch_kernel<2>), grid, block, 0, stream, cases, n_cases, st, total_blocks, ph, per_xcd); break;
+    case 3: hipLaunchKernelGGL((flux_batch_kernel<3>), grid, block, 0, stream, cases, n_cases, st, total_blocks, ph, per_xcd); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const Physics& ph,
+                          hipStream_t stream) {
+    hipLaunchKernelGGL(spectral_batch_kernel, dim3(total_blocks), dim3(256), 0, stream, cases, n_cases, total_blocks,
+                       ph);
+    return hipGetLastError();
+}
+
+hipError_t finalize_batch(const CaseBufs* cases, int n_cases, double cfl, double fixed_dt, int64_t hist_cap,
+                          hipStream_t stream) {
+    hipLaunchKernelGGL(finalize_batch_kernel, dim3((n_cases + 63) / 64), dim3(64), 0, stream, cases, n_cases, cfl,
+                       fixed_dt, hist_cap);
     return hipGetLastError();
 }
 

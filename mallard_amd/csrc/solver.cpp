@@ -2,6 +2,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -540,6 +541,33 @@ void Solver::get_volumes(double* out) const {
     for (uint32_t j = 0; j < L_.N; ++j) out[L_.perm[j]] = L_.volume[j];
 }
 
+dev::CaseBufs Solver::case_bufs() const {
+    dev::CaseBufs c{};
+    c.g = geom();
+    if (L_.teno) c.t = teno();
+    c.Un = Un_;
+    c.Ua = Ua_;
+    c.Ub = Ub_;
+    c.W = W_;
+    c.prims = prims_;
+    c.h = h_;
+    c.fs = fs_;
+    c.rhs = rhs_;
+    c.lam_cell = lam_cell_;
+    c.dt = dt_dev_;
+    c.dt_hist = dt_hist_;
+    c.lam_max = lam_max_;
+    c.step = step_dev_;
+    c.dec = dec_;
+    return c;
+}
+
+int64_t Solver::device_step() {
+    int64_t s0 = 0;
+    HIP_OK(hipMemcpy(&s0, step_dev_, sizeof(int64_t), hipMemcpyDeviceToHost));
+    return s0;
+}
+
 dev::StageArgs Solver::stage(int op, const double* Uin, double* Uout) const {
     dev::StageArgs a{};
     a.op = op;
@@ -730,10 +758,16 @@ void Solver::run_steps_profiled(int64_t n, double* stats) {
 }
 
 void Solver::finish_steps(int64_t s0, int64_t n, double* dts_out) {
-    std::vector<double> d(dt_hist_cap_);
-    HIP_OK(hipMemcpy(d.data(), dt_hist_, sizeof(double) * dt_hist_cap_, hipMemcpyDeviceToHost));
+    // the n dt values of these steps (a ring of dt_hist_cap_ entries)
+    std::vector<double> d((size_t)std::min<int64_t>(n, dt_hist_cap_));
+    const int64_t first = s0 % dt_hist_cap_, cnt = (int64_t)d.size();
+    const int64_t part = std::min<int64_t>(cnt, dt_hist_cap_ - first);
+    HIP_OK(hipMemcpy(d.data(), dt_hist_ + first, sizeof(double) * part, hipMemcpyDeviceToHost));
+    if (cnt > part) HIP_OK(hipMemcpy(d.data() + part, dt_hist_, sizeof(double) * (cnt - part), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; ++i) {
-        const double dt = d[(s0 + i) % dt_hist_cap_];
+        // beyond the ring only the last dt_hist_cap_ values remain
+        const int64_t k = i - (n - cnt);
+        const double dt = k >= 0 ? d[(size_t)k] : d[0];
         if (dts_out) dts_out[i] = dt;
         if (dt < 0.0) fail("dt negative: " + std::to_string(dt) + ".");
         t_ += dt;

@@ -68,7 +68,11 @@ struct Config {
 // Parses a Mallard TOML input with the reference's keys and defaults.
 Config config_from_toml(const toml::Value& in);
 
+class Ensemble;
+
 class Solver {
+    friend class Ensemble;
+
 public:
     Solver();
     ~Solver();
@@ -150,6 +154,8 @@ private:
     void dt_pass(double cfl, double fixed_dt);
     void mark(int tag);
     void drop_graph();
+    dev::CaseBufs case_bufs() const;   // this solver as one case of an ensemble
+    int64_t device_step();
     void finish_steps(int64_t s0, int64_t n, double* dts_out);
 
     Config cfg_;
