@@ -25,24 +25,26 @@ __device__ inline int32_t central_member(const Teno& t, uint32_t c, int i) {
 }
 
 // One thread per cell: the 48 features (bf16) and the TENO label (1 when the
-// central stencil was selected for all 4 variables) at row base + c.
+// central stencil was selected for all 4 variables, -1 for a cell without
+// features) at row base + c.
 __global__ __launch_bounds__(256) void features_kernel(Geom g, Teno t, const double* __restrict__ U,
                                                        const uint8_t* __restrict__ dec, double gamma,
                                                        uint16_t* __restrict__ X, float* __restrict__ y,
                                                        uint64_t base) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
-    const double* Uc = U + 4 * (size_t)c;
-    const CellRef r = cell_ref(gamma, Uc);
-    uint16_t* row = X + (base + c) * kFeat;
-    for (int m = 0; m < kMembers; ++m) {
-        uint16_t f[4];
-        if (m == 0) member_features(gamma, r, Uc, true, f);
-        else member_features(gamma, r, U + 4 * (size_t)central_member(t, c, m - 1), false, f);
-        if (!r.ok) f[0] = f[1] = f[2] = f[3] = 0;
-        for (int j = 0; j < 4; ++j) row[4 * m + j] = f[j];
+    const CellRef r = cell_ref(gamma, U + 4 * (size_t)c);
+    double d[kMembers][4], smax[4] = {0.0, 0.0, 0.0, 0.0};
+    bool ok = r.ok;
+    for (int m = 1; m < kMembers && ok; ++m) {
+        ok = member_diff(gamma, r, U + 4 * (size_t)central_member(t, c, m - 1), d[m]);
+        for (int v = 0; v < 4; ++v) smax[v] = fmax(smax[v], fabs(d[m][v]));
     }
-    y[base + c] = (r.ok && dec[c] == 0xFu) ? 1.0f : 0.0f;
+    uint16_t* row = X + (base + c) * kFeat;
+    for (int v = 0; v < 4; ++v) row[v] = ok ? feat_bf16(scale_feature(smax[v])) : 0;
+    for (int m = 1; m < kMembers; ++m)
+        for (int v = 0; v < 4; ++v) row[4 * m + v] = ok ? feat_bf16(pattern_feature(d[m][v], smax[v])) : 0;
+    y[base + c] = !ok ? -1.0f : (dec[c] == 0xFu ? 1.0f : 0.0f);
 }
 
 // Logits of n feature rows, 16 rows per wave.

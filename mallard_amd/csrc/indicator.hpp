@@ -2,13 +2,15 @@
 // 48-64-64-1 MLP (GELU, GELU, sigmoid) with BF16 weights and activations and
 // FP32 accumulation on the CDNA matrix cores (v_mfma_f32_16x16x16_bf16).
 //
-// Features of cell c (12 members of its central stencil, target first), each
-// member m contributing 4 values computed in FP64 from conserved states and
-// rounded to FP32 then BF16 (round to nearest even):
-//   m = 0:  0, u_c / a_c, v_c / a_c, 0
-//   m > 0:  rho_m / rho_c - 1, (u_m - u_c) / a_c, (v_m - v_c) / a_c, p_m / p_c - 1
-// clipped to [-8, 8]; a_c = sqrt(gamma p_c / rho_c). A cell whose own density
-// or pressure is not positive is never judged smooth.
+// Features of cell c, from the 12 members of its central stencil (target
+// first). For members m = 1..11 the raw differences, in FP64,
+//   d = (rho_m / rho_c - 1, (u_m - u_c) / a_c, (v_m - v_c) / a_c, p_m / p_c - 1),
+// a_c = sqrt(gamma p_c / rho_c), are divided per variable by their largest
+// magnitude s_v over the stencil (0 when s_v = 0), so the pattern is scale
+// free; member 0's four slots carry the scale, ilogb(s_v) / 32 clipped below
+// at -2 (-2.5 when s_v = 0). Values are rounded to FP32 then BF16 (nearest
+// even). A cell with a non-physical member (or itself) has no features and
+// is never judged smooth.
 //
 // MFMA mapping (one wave = 16 cells, MFMA column n = lane % 16, row group
 // g = lane / 16): every layer is computed transposed, D = W^T X^T, so the
@@ -81,28 +83,25 @@ __device__ inline CellRef cell_ref(double gamma, const double* U) {
     return r;
 }
 
-__device__ inline uint16_t feat_bf16(double x) {
-    x = fmin(8.0, fmax(-8.0, x));
-    return f32_to_bf16((float)x);
-}
+__device__ inline uint16_t feat_bf16(double x) { return f32_to_bf16((float)x); }
 
-// the 4 features of member state Um (self: the cell itself) as bf16
-__device__ inline void member_features(double gamma, const CellRef& r, const double* Um, bool self, uint16_t* f) {
-    if (self) {
-        f[0] = feat_bf16(0.0);
-        f[1] = feat_bf16(r.u / r.a);
-        f[2] = feat_bf16(r.v / r.a);
-        f[3] = feat_bf16(0.0);
-        return;
-    }
+// raw differences of member state Um against the cell; false when Um is not physical
+__device__ inline bool member_diff(double gamma, const CellRef& r, const double* Um, double* d) {
     const double rho = Um[0];
     const double u = Um[1] / rho, v = Um[2] / rho;
     const double p = pressure_of(gamma, Um);
-    f[0] = feat_bf16(rho / r.rho - 1.0);
-    f[1] = feat_bf16((u - r.u) / r.a);
-    f[2] = feat_bf16((v - r.v) / r.a);
-    f[3] = feat_bf16(p / r.p - 1.0);
+    d[0] = rho / r.rho - 1.0;
+    d[1] = (u - r.u) / r.a;
+    d[2] = (v - r.v) / r.a;
+    d[3] = p / r.p - 1.0;
+    return rho > 0.0 && p > 0.0 && d[0] == d[0] && d[1] == d[1] && d[2] == d[2] && d[3] == d[3];
 }
+
+// scale slot of member 0
+__device__ inline double scale_feature(double s) { return s > 0.0 ? fmax(-2.0, (double)ilogb(s) / 32.0) : -2.5; }
+
+// pattern slot of members 1..11
+__device__ inline double pattern_feature(double d, double s) { return s > 0.0 ? d / s : 0.0; }
 
 __device__ inline v4s pack4(const uint16_t* f) {
     v4s x;

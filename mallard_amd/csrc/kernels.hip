@@ -345,25 +345,43 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         const D4 Uo = ld4(U, vm ? cm : 0);
         const ind::CellRef r = ind::cell_ref(t.gamma, Uo.v);
         const size_t crow = (size_t)lb * 4 * Mp;  // stencil 0 rows of this block
+        // lane (column cell, group gq) holds members 4 kc + gq, kc = 0..2
+        double d[3][4], smax[4] = {0.0, 0.0, 0.0, 0.0};
+        bool ok = r.ok;
+#pragma unroll
+        for (int kc = 0; kc < 3; ++kc) {
+            const int m = 4 * kc + gq;
+            if (m == 0) continue;
+            const uint32_t pos = loc[(crow + (m - 1)) * 64 + (cm & 63u)];
+            const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
+            const double Um[4] = {u01.x, u01.y, u23.x, u23.y};
+            ok = ind::member_diff(t.gamma, r, Um, d[kc]) && ok;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) smax[v] = fmax(smax[v], fabs(d[kc][v]));
+        }
+        // stencil-wide maxima and validity: the 4 lanes of a column
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            smax[v] = fmax(smax[v], __shfl_xor(smax[v], 16));
+            smax[v] = fmax(smax[v], __shfl_xor(smax[v], 32));
+        }
+        int okc = ok ? 1 : 0;
+        okc &= __shfl_xor(okc, 16);
+        okc &= __shfl_xor(okc, 32);
         ind::v4s xb[3];
 #pragma unroll
         for (int kc = 0; kc < 3; ++kc) {
             const int m = 4 * kc + gq;
             uint16_t f[4];
-            if (m == 0) {
-                ind::member_features(t.gamma, r, Uo.v, true, f);
-            } else {
-                const uint32_t pos = loc[(crow + (m - 1)) * 64 + (cm & 63u)];
-                const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
-                const double Um[4] = {u01.x, u01.y, u23.x, u23.y};
-                ind::member_features(t.gamma, r, Um, false, f);
-            }
-            if (!r.ok) f[0] = f[1] = f[2] = f[3] = 0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                f[v] = !okc ? 0 : ind::feat_bf16(m == 0 ? ind::scale_feature(smax[v])
+                                                         : ind::pattern_feature(d[kc][v], smax[v]));
             xb[kc] = ind::pack4(f);
         }
         const float z = ind::mlp_wave(t.net, xb, lane);
         const float prob = 1.0f / (1.0f + expf(-z));
-        const int sm = (vm && r.ok && prob >= t.net.threshold) ? 1 : 0;
+        const int sm = (vm && okc && prob >= t.net.threshold) ? 1 : 0;
         const uint64_t bal = __ballot(sm && gq == 0);
         if (lane == 0 && bal) atomicAdd(t.gated, (unsigned long long)__popcll(bal));
         smooth = __shfl(sm, lane >> 2) != 0;

@@ -54,15 +54,20 @@ def build_dataset(n_cases=512, nx=64, snapshots=(0, 8, 20), seed=1234, jitter=0.
             if k > done:
                 dev.run_steps(k - done)
                 done = k
+            # a case whose expansion reaches vacuum stops contributing rows
+            U, W = dev.state()
+            if not (np.isfinite(U).all() and (U[:, 0] > 0).all() and (W[:, 2] > 0).all()):
+                break
             dev.collect_features(ds)
     _, y = ds.get(0, len(ds), features=False)
     return ds, y
 
 
 def make_pool(y, pos_per_neg=2.0, seed=0):
-    """All non-smooth rows plus a seeded sample of smooth ones."""
+    """All non-smooth rows plus a seeded sample of smooth ones (rows without
+    features, label -1, are left out)."""
     rng = np.random.default_rng(seed)
-    neg = np.flatnonzero(y < 0.5)
+    neg = np.flatnonzero((y >= 0.0) & (y < 0.5))
     pos = np.flatnonzero(y >= 0.5)
     k = min(len(pos), int(pos_per_neg * max(len(neg), 1)))
     pos = rng.choice(pos, k, replace=False) if k < len(pos) else pos

@@ -40,33 +40,51 @@ def bf16(x):
     return bf16_value(bf16_bits(x))
 
 
-def _feat(x):
-    return bf16_bits(np.clip(x, -8.0, 8.0).astype(np.float32))
+def _bits(x):
+    return bf16_bits(np.asarray(x, dtype=np.float64).astype(np.float32))
+
+
+def _ilogb(x):
+    """Unbiased binary exponent of positive finite doubles (C ilogb)."""
+    return np.frexp(x)[1] - 1
 
 
 def features(U, central, gamma):
     """bf16 features [N][48] of conserved states U[N][4] (FP64) with central
-    stencils central[N][12] (target first), reference numbering."""
+    stencils central[N][12] (target first), reference numbering: per member
+    1..11 the differences (rho_m/rho_c - 1, (u_m - u_c)/a_c, (v_m - v_c)/a_c,
+    p_m/p_c - 1) divided by their stencil-wide maximum magnitude s_v per
+    variable; member 0 carries max(-2, ilogb(s_v)/32) (-2.5 for s_v = 0).
+    Rows with a non-physical cell or member are zero (valid[i] False)."""
     U = np.asarray(U, dtype=np.float64)
     rho = U[:, 0]
-    u, v = U[:, 1] / rho, U[:, 2] / rho
-    p = (gamma - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] * U[:, 1] + U[:, 2] * U[:, 2]) / U[:, 0])
-    ok = (rho > 0.0) & (p > 0.0)
     with np.errstate(all="ignore"):
+        u, v = U[:, 1] / rho, U[:, 2] / rho
+        p = (gamma - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] * U[:, 1] + U[:, 2] * U[:, 2]) / U[:, 0])
+        ok = (rho > 0.0) & (p > 0.0)
         a = np.where(ok, np.sqrt(gamma * p / rho), 1.0)
-        X = np.zeros((len(U), N_FEAT), np.uint16)
-        X[:, 0] = _feat(np.zeros(len(U)))
-        X[:, 1] = _feat(u / a)
-        X[:, 2] = _feat(v / a)
-        X[:, 3] = _feat(np.zeros(len(U)))
+        D = np.zeros((len(U), N_MEMBERS, 4))
         for m in range(1, N_MEMBERS):
             j = central[:, m]
-            X[:, 4 * m + 0] = _feat(rho[j] / rho - 1.0)
-            X[:, 4 * m + 1] = _feat((u[j] - u) / a)
-            X[:, 4 * m + 2] = _feat((v[j] - v) / a)
-            X[:, 4 * m + 3] = _feat(p[j] / p - 1.0)
+            D[:, m, 0] = rho[j] / rho - 1.0
+            D[:, m, 1] = (u[j] - u) / a
+            D[:, m, 2] = (v[j] - v) / a
+            D[:, m, 3] = p[j] / p - 1.0
+            ok &= (rho[j] > 0.0) & (p[j] > 0.0) & np.isfinite(D[:, m]).all(axis=1)
+        s = np.abs(D[:, 1:]).max(axis=1)
+        pos = s > 0.0
+        f0 = np.where(pos, np.maximum(-2.0, _ilogb(np.where(pos, s, 1.0)) / 32.0), -2.5)
+        pat = np.where(pos[:, None, :], D / np.where(pos, s, 1.0)[:, None, :], 0.0)
+    X = np.zeros((len(U), N_FEAT), np.uint16)
+    X[:, 0:4] = _bits(f0)
+    X[:, 4:] = _bits(pat[:, 1:].reshape(len(U), -1))
     X[~ok] = 0
     return X
+
+
+def valid_rows(U, central, gamma):
+    """Rows features() fills (physical cell and members)."""
+    return ~np.all(features(U, central, gamma) == 0, axis=1)
 
 
 def unpack(theta, dtype=np.float32):

@@ -47,9 +47,10 @@ def test_features_and_labels_match_oracle(mlr, oracle):
     orc.set_state(U)
     orc.calc_rhs(U)
     d = orc.decisions()
-    yr = (d.sum(axis=1) == 4).astype(np.float32)
+    valid = ~np.all(Xr == 0, axis=1)
+    yr = np.where(valid, (d.sum(axis=1) == 4).astype(np.float32), -1.0).astype(np.float32)
     assert np.array_equal(y, yr)
-    assert 0 < y.sum() < len(y)  # both classes present
+    assert (y == 1).any() and (y == 0).any()  # both classes present
 
 
 def test_mlp_forward_matches_oracle(mlr):

@@ -40,22 +40,35 @@ def test_backward_matches_finite_differences():
 
 def test_features_closed_forms():
     g = 1.4
-    # cell 0 and its 11 members: members 1..5 equal, members 6..11 perturbed
+    # cell 0 and its 11 members: members 1..5 equal, members 6..11 denser
+    # (same velocity and pressure)
     U = np.zeros((12, 4))
-    rho, u, v, p = 1.2, 0.3, -0.4, 0.9
+    rho, u, v, p = 1.0, 0.5, -0.25, 1.0   # binary fractions: velocity ratios are exact
     U[:] = [rho, rho * u, rho * v, p / (g - 1) + 0.5 * rho * (u * u + v * v)]
-    U[6:, 0] *= 1.5  # density jump at fixed velocity and pressure: rho E changes with rho
+    U[6:, 0] *= 1.5
     U[6:, 1:3] *= 1.5
     U[6:, 3] = p / (g - 1) + 0.5 * 1.5 * rho * (u * u + v * v)
     central = np.tile(np.arange(12), (12, 1))
     X = ir.bf16_value(ir.features(U, central, g)).astype(np.float64)
-    a = np.sqrt(g * p / rho)
-    assert X[0, 0] == 0 and X[0, 3] == 0
-    assert X[0, 1] == pytest.approx(u / a, rel=2 ** -8) and X[0, 2] == pytest.approx(v / a, rel=2 ** -8)
-    assert np.all(X[0, 4:24] == 0)                      # identical members
-    assert np.allclose(X[0, 24::4], 0.5, rtol=2 ** -8)  # rho ratio - 1
-    assert np.allclose(X[0, 25::4], 0.0, atol=1e-12)    # same velocity
-    assert np.allclose(X[0, 27::4], 0.0, atol=1e-6)     # same pressure
-    # a non-physical cell gives all-zero features
-    U[3, 3] = -1.0
-    assert np.all(ir.features(U, central, g)[3] == 0)
+    # scale slots: rho differences peak at 0.5 = 2^-1 -> ilogb -1 -> -1/32;
+    # velocity differences vanish -> -2.5; pressure ones are rounding noise
+    assert X[0, 0] == -1 / 32 and X[0, 1] == -2.5 and X[0, 2] == -2.5
+    assert X[0, 3] <= -40 / 32
+    assert np.all(X[0, 4:24] == 0)                                # identical members
+    assert np.all(X[0, 24::4] == 1.0)                             # pattern = d / max|d|
+    assert np.all(X[0, 25::4] == 0) and np.all(X[0, 26::4] == 0)
+    # a uniform stencil has the zero pattern and the -2.5 scale everywhere
+    Uu = np.tile(U[0], (12, 1))
+    Xu = ir.bf16_value(ir.features(Uu, central, g)).astype(np.float64)
+    assert np.all(Xu[:, :4] == -2.5) and np.all(Xu[:, 4:] == 0)
+    # scale-free: the same pattern at amplitude 1e-6 differs only in the scale slots
+    Us = U.copy()
+    Us[6:, 0] = rho * (1 + 1e-6)
+    Us[6:, 1:3] = Us[6:, 0:1] * np.array([u, v])
+    Us[6:, 3] = p / (g - 1) + 0.5 * Us[6:, 0] * (u * u + v * v)
+    Xs = ir.bf16_value(ir.features(Us, central, g)).astype(np.float64)
+    assert np.array_equal(Xs[0, 24::4], X[0, 24::4]) and Xs[0, 0] == -20 / 32
+    # a non-physical member empties the row
+    U2 = U.copy()
+    U2[7, 3] = -1.0
+    assert np.all(ir.features(U2, central, g)[0] == 0)
