@@ -42,7 +42,9 @@ struct Net {
     const float* b2;
     const float* w3;
     const float* b3;
-    float threshold;  // probability at and above which a cell is judged smooth
+    // a cell is judged smooth when sigmoid(z) >= tau, that is z >= log(tau / (1 - tau))
+    // (+inf for tau = 1: never)
+    float logit_threshold;
 };
 }  // namespace ind
 
@@ -59,10 +61,12 @@ struct Teno {
     const double* fbasis;
     double OI[81];      // (K-1) x (K-1) block of the oscillation indicator, row-major
     // learned indicator gate: when gate != 0 a cell the network judges smooth
-    // takes the central stencil alone; gated counts smooth cells
+    // takes the central stencil alone; gated counts smooth cells, one
+    // running total per wave slot (block * 4 + wave), each written only by
+    // its own wave, so the count needs no atomics
     int32_t gate;
     ind::Net net;
-    unsigned long long* gated;
+    uint32_t* gated;
     double gamma;
 };
 
@@ -157,7 +161,7 @@ hipError_t teno_aplus(const PreInput& in, int K, int M, double* A, double* ta, i
 // Learned indicator (indicator.hip). The network lives in one device buffer of
 // net_bytes() that net_pack fills from the FP32 master parameters.
 size_t net_bytes();
-ind::Net net_view(void* buf, float threshold, const float** W2f);
+ind::Net net_view(void* buf, double threshold, const float** W2f);
 hipError_t net_pack(const float* theta, void* buf, hipStream_t stream);
 // features (bf16 [N][48]) and TENO labels of every cell at rows base..base+N-1
 hipError_t ind_features(const Geom& g, const Teno& t, const double* U, const uint8_t* dec, double gamma,

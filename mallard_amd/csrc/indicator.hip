@@ -4,6 +4,7 @@
 // fixed sample order, reduced in a fixed block order, Adam in FP32).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 
 #include "device.hpp"
@@ -41,9 +42,20 @@ __global__ __launch_bounds__(256) void features_kernel(Geom g, Teno t, const dou
         for (int v = 0; v < 4; ++v) smax[v] = fmax(smax[v], fabs(d[m][v]));
     }
     uint16_t* row = X + (base + c) * kFeat;
-    for (int v = 0; v < 4; ++v) row[v] = ok ? feat_bf16(scale_feature(smax[v])) : 0;
-    for (int m = 1; m < kMembers; ++m)
-        for (int v = 0; v < 4; ++v) row[4 * m + v] = ok ? feat_bf16(pattern_feature(d[m][v], smax[v])) : 0;
+    auto put = [&](int m, const double* x) {
+        const v4s p = pack_feats(x);
+        for (int v = 0; v < 4; ++v) row[4 * m + v] = (uint16_t)p[v];
+    };
+    double x[4], is[4];
+    for (int v = 0; v < 4; ++v) {
+        x[v] = ok ? scale_feature(smax[v]) : 0.0;
+        is[v] = inv_scale(smax[v]);
+    }
+    put(0, x);
+    for (int m = 1; m < kMembers; ++m) {
+        for (int v = 0; v < 4; ++v) x[v] = ok ? pattern_feature(d[m][v], is[v]) : 0.0;
+        put(m, x);
+    }
     y[base + c] = !ok ? -1.0f : (dec[c] == 0xFu ? 1.0f : 0.0f);
 }
 
@@ -236,7 +248,7 @@ size_t net_bytes() {
            (size_t)kHid * kHid * sizeof(float);
 }
 
-ind::Net net_view(void* buf, float threshold, const float** W2f) {
+ind::Net net_view(void* buf, double threshold, const float** W2f) {
     char* p = static_cast<char*>(buf);
     ind::Net n{};
     n.W1p = p;
@@ -248,13 +260,13 @@ ind::Net net_view(void* buf, float threshold, const float** W2f) {
     n.w3 = n.b2 + kHid;
     n.b3 = n.w3 + kHid;
     if (W2f) *W2f = n.b3 + 1;
-    n.threshold = threshold;
+    n.logit_threshold = threshold >= 1.0 ? INFINITY : (float)std::log(threshold / (1.0 - threshold));
     return n;
 }
 
 hipError_t net_pack(const float* theta, void* buf, hipStream_t stream) {
     const float* W2f = nullptr;
-    const ind::Net n = net_view(buf, 0.0f, &W2f);
+    const ind::Net n = net_view(buf, 0.5, &W2f);
     hipLaunchKernelGGL(pack_kernel, dim3((kHid * kHid + 255) / 256), dim3(256), 0, stream, theta,
                        static_cast<v4s*>(const_cast<void*>(n.W1p)), static_cast<v4s*>(const_cast<void*>(n.W2p)),
                        const_cast<float*>(n.b1),

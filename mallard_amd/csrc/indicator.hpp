@@ -4,13 +4,15 @@
 //
 // Features of cell c, from the 12 members of its central stencil (target
 // first). For members m = 1..11 the raw differences, in FP64,
-//   d = (rho_m / rho_c - 1, (u_m - u_c) / a_c, (v_m - v_c) / a_c, p_m / p_c - 1),
-// a_c = sqrt(gamma p_c / rho_c), are divided per variable by their largest
-// magnitude s_v over the stencil (0 when s_v = 0), so the pattern is scale
-// free; member 0's four slots carry the scale, ilogb(s_v) / 32 clipped below
-// at -2 (-2.5 when s_v = 0). Values are rounded to FP32 then BF16 (nearest
-// even). A cell with a non-physical member (or itself) has no features and
-// is never judged smooth.
+//   d = (rho_m ir_c - 1, (u_m - u_c) ia_c, (v_m - v_c) ia_c, p_m ip_c - 1),
+// with the cell's reciprocals ir_c = 1 / rho_c, ia_c = 1 / a_c,
+// ip_c = 1 / p_c, a_c = sqrt(gamma p_c ir_c), and the member's velocity and
+// pressure formed with its own 1 / rho_m, are multiplied per variable by the
+// reciprocal of their largest magnitude s_v over the stencil (0 when s_v = 0),
+// so the pattern is scale free; member 0's four slots carry the scale,
+// ilogb(s_v) / 32 clipped below at -2 (-2.5 when s_v = 0). Values are rounded
+// to FP32 then BF16 (nearest even). A cell with a non-physical member (or
+// itself) has no features and is never judged smooth.
 //
 // MFMA mapping (one wave = 16 cells, MFMA column n = lane % 16, row group
 // g = lane / 16): every layer is computed transposed, D = W^T X^T, so the
@@ -40,6 +42,9 @@ constexpr int kW1pVec = 4 * 3 * 64, kW2pVec = 4 * 4 * 64;
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
 
 __host__ __device__ inline uint16_t f32_to_bf16(float x) {
     uint32_t u = __builtin_bit_cast(uint32_t, x);
@@ -51,57 +56,83 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
     return __builtin_bit_cast(float, (uint32_t)h << 16);
 }
 
-// tanh form of GELU and its derivative, FP32
-__device__ inline float gelu(float x) {
-    const float k = 0.7978845608028654f;
-    const float t = tanhf(k * (x + 0.044715f * x * x * x));
-    return 0.5f * x * (1.0f + t);
+// Two FP32 values to a bf16 pair, round to nearest even, lo in the low half
+// (the hardware converter; the same rounding as f32_to_bf16 for every
+// non-NaN input).
+__device__ inline uint32_t bf16x2(float lo, float hi) {
+    const v2f f = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, v2bf));
 }
-__device__ inline float gelu_d(float x) {
-    const float k = 0.7978845608028654f;
-    const float t = tanhf(k * (x + 0.044715f * x * x * x));
-    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k * (1.0f + 3.0f * 0.044715f * x * x);
-}
+__device__ inline float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+__device__ inline float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xFFFF0000u); }
 
-struct CellRef {
-    double rho, u, v, p, a;
-    bool ok;
-};
-
-__device__ inline double pressure_of(double gamma, const double* U) {
-    return (gamma - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) / U[0]);
-}
-
-__device__ inline CellRef cell_ref(double gamma, const double* U) {
-    CellRef r;
-    r.rho = U[0];
-    r.u = U[1] / U[0];
-    r.v = U[2] / U[0];
-    r.p = pressure_of(gamma, U);
-    r.ok = r.rho > 0.0 && r.p > 0.0;
-    r.a = r.ok ? sqrt(gamma * r.p / r.rho) : 1.0;
+// FP64 -> FP32 as a step of its own: features are rounded to FP32, then to
+// bf16 (kept opaque so the two roundings are not merged into one).
+__device__ inline float to_f32(double x) {
+    float r;
+    asm("v_cvt_f32_f64 %0, %1" : "=v"(r) : "v"(x));
     return r;
 }
 
-__device__ inline uint16_t feat_bf16(double x) { return f32_to_bf16((float)x); }
+// four features (FP64) as the bf16 MFMA operand of one lane
+__device__ inline v4s pack_feats(const double* x) {
+    const v2u w = {bf16x2(to_f32(x[0]), to_f32(x[1])), bf16x2(to_f32(x[2]), to_f32(x[3]))};
+    return __builtin_bit_cast(v4s, w);
+}
+
+// tanh form of GELU, 0.5 x (1 + tanh(u)) = x sigmoid(2u) with
+// u = k (x + 0.044715 x^3), k = sqrt(2 / pi), in FP32 with the hardware exp2
+// and reciprocal (about 1 ulp each): sigmoid(2u) = 1 / (1 + 2^t),
+// t = -2u log2(e) = x (-2k log2(e) - 2k 0.044715 log2(e) x^2). Its derivative
+// is sigmoid(2u) + 2 x sigmoid(2u) (1 - sigmoid(2u)) u'.
+__device__ inline float gelu_sig(float x) {
+    const float t = x * __builtin_fmaf(-0.1029432395800235f, x * x, -2.302208198144325f);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
+}
+__device__ inline float gelu(float x) { return x * gelu_sig(x); }
+__device__ inline float gelu_d(float x) {
+    const float s = gelu_sig(x);
+    const float du = 0.7978845608028654f * __builtin_fmaf(0.134145f, x * x, 1.0f);
+    return s + 2.0f * x * s * (1.0f - s) * du;
+}
+
+struct CellRef {
+    double u, v, ir, ia, ip;
+    bool ok;
+};
+
+__device__ inline CellRef cell_ref(double gamma, const double* U) {
+    CellRef r;
+    r.ir = 1.0 / U[0];
+    r.u = U[1] * r.ir;
+    r.v = U[2] * r.ir;
+    const double p = (gamma - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) * r.ir);
+    r.ok = U[0] > 0.0 && p > 0.0;
+    r.ia = r.ok ? 1.0 / sqrt(gamma * p * r.ir) : 1.0;
+    r.ip = 1.0 / p;
+    return r;
+}
 
 // raw differences of member state Um against the cell; false when Um is not physical
 __device__ inline bool member_diff(double gamma, const CellRef& r, const double* Um, double* d) {
     const double rho = Um[0];
-    const double u = Um[1] / rho, v = Um[2] / rho;
-    const double p = pressure_of(gamma, Um);
-    d[0] = rho / r.rho - 1.0;
-    d[1] = (u - r.u) / r.a;
-    d[2] = (v - r.v) / r.a;
-    d[3] = p / r.p - 1.0;
-    return rho > 0.0 && p > 0.0 && d[0] == d[0] && d[1] == d[1] && d[2] == d[2] && d[3] == d[3];
+    const double irm = 1.0 / rho;
+    const double u = Um[1] * irm, v = Um[2] * irm;
+    const double p = (gamma - 1.0) * (Um[3] - 0.5 * (Um[1] * Um[1] + Um[2] * Um[2]) * irm);
+    d[0] = rho * r.ir - 1.0;
+    d[1] = (u - r.u) * r.ia;
+    d[2] = (v - r.v) * r.ia;
+    d[3] = p * r.ip - 1.0;
+    return rho > 0.0 && p > 0.0 && isfinite(d[0]) && isfinite(d[1]) && isfinite(d[2]) && isfinite(d[3]);
 }
 
 // scale slot of member 0
 __device__ inline double scale_feature(double s) { return s > 0.0 ? fmax(-2.0, (double)ilogb(s) / 32.0) : -2.5; }
 
-// pattern slot of members 1..11
-__device__ inline double pattern_feature(double d, double s) { return s > 0.0 ? d / s : 0.0; }
+// reciprocal of the stencil-wide maximum s (0 when s = 0) and the pattern
+// slot of members 1..11
+__device__ inline double inv_scale(double s) { return s > 0.0 ? 1.0 / s : 0.0; }
+__device__ inline double pattern_feature(double d, double is) { return d * is; }
 
 __device__ inline v4s pack4(const uint16_t* f) {
     v4s x;
@@ -114,45 +145,53 @@ __device__ inline v4s pack4(const uint16_t* f) {
 
 // Forward pass of one wave. xb[kc] holds, for cell lane % 16, the features
 // kc*16 + 4g .. kc*16 + 4g + 3 (that is member 4 kc + g). Every lane returns
-// the logit of cell lane % 16. When pre1 / h1 / pre2 are given the lane also
-// stores its 16 pre-activations and hidden values (hidden index
-// nt*16 + 4g + r at [nt*4 + r]) for the backward pass.
+// the logit of cell lane % 16. The biases enter as the accumulators' initial
+// values. When pre1 / h1 / pre2 are given the lane also stores its 16
+// pre-activations and hidden values (hidden index nt*16 + 4g + r at
+// [nt*4 + r]) for the backward pass.
 __device__ inline float mlp_wave(const Net& net, const v4s* xb, int lane, float* pre1 = nullptr,
                                  float* h1s = nullptr, float* pre2 = nullptr) {
     const int g = lane >> 4;
+    const float4* B1 = reinterpret_cast<const float4*>(net.b1);
+    const float4* B2 = reinterpret_cast<const float4*>(net.b2);
+    const float4* W3 = reinterpret_cast<const float4*>(net.w3);
     v4s h1[4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const float4 bb = B1[nt * 4 + g];
+        v4f acc = {bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
         for (int kc = 0; kc < 3; ++kc)
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(static_cast<const v4s*>(net.W1p)[(nt * 3 + kc) * 64 + lane],
                                                            xb[kc], acc, 0, 0, 0);
+        const v2u w = {bf16x2(gelu(acc[0]), gelu(acc[1])), bf16x2(gelu(acc[2]), gelu(acc[3]))};
+        h1[nt] = __builtin_bit_cast(v4s, w);
+        if (pre1) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const float pre = acc[r] + net.b1[nt * 16 + 4 * g + r];
-            const uint16_t hb = f32_to_bf16(gelu(pre));
-            h1[nt][r] = (short)hb;
-            if (pre1) {
-                pre1[nt * 4 + r] = pre;
-                h1s[nt * 4 + r] = bf16_to_f32(hb);
-            }
+            for (int r = 0; r < 4; ++r) pre1[nt * 4 + r] = acc[r];
+            h1s[nt * 4 + 0] = bf16_lo(w[0]);
+            h1s[nt * 4 + 1] = bf16_hi(w[0]);
+            h1s[nt * 4 + 2] = bf16_lo(w[1]);
+            h1s[nt * 4 + 3] = bf16_hi(w[1]);
         }
     }
     float z = 0.0f;
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const float4 bb = B2[nt * 4 + g];
+        v4f acc = {bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
         for (int kc = 0; kc < 4; ++kc)
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(static_cast<const v4s*>(net.W2p)[(nt * 4 + kc) * 64 + lane],
                                                            h1[kc], acc, 0, 0, 0);
+        const float4 ww = W3[nt * 4 + g];
+        if (pre2)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const float pre = acc[r] + net.b2[nt * 16 + 4 * g + r];
-            if (pre2) pre2[nt * 4 + r] = pre;
-            z += net.w3[nt * 16 + 4 * g + r] * gelu(pre);
-        }
+            for (int r = 0; r < 4; ++r) pre2[nt * 4 + r] = acc[r];
+        z = __builtin_fmaf(ww.x, gelu(acc[0]), z);
+        z = __builtin_fmaf(ww.y, gelu(acc[1]), z);
+        z = __builtin_fmaf(ww.z, gelu(acc[2]), z);
+        z = __builtin_fmaf(ww.w, gelu(acc[3]), z);
     }
     // the 4 lanes of a column hold the 4 row groups of the same cell
     z += __shfl_xor(z, 16);

@@ -271,8 +271,9 @@ __device__ inline double qperm(double x) {
     const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
-// quad_perm controls: lane j of a quad reads lane (j - r) & 3, r = 1, 2, 3
-constexpr int kRot1 = 0x93, kRot2 = 0x4E, kRot3 = 0x39;
+// quad_perm controls: lane j of a quad reads lane (j - r) & 3, r = 1, 2, 3;
+// kXor1: lane j reads lane j ^ 1 (kRot2 is also j ^ 2)
+constexpr int kRot1 = 0x93, kRot2 = 0x4E, kRot3 = 0x39, kXor1 = 0xB1;
 
 __device__ inline double sel4(const double* x, int i) {
     return i == 0 ? x[0] : (i == 1 ? x[1] : (i == 2 ? x[2] : x[3]));
@@ -368,87 +369,130 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         int okc = ok ? 1 : 0;
         okc &= __shfl_xor(okc, 16);
         okc &= __shfl_xor(okc, 32);
+        double is[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) is[v] = ind::inv_scale(smax[v]);
         ind::v4s xb[3];
 #pragma unroll
         for (int kc = 0; kc < 3; ++kc) {
             const int m = 4 * kc + gq;
-            uint16_t f[4];
+            double x[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                f[v] = !okc ? 0 : ind::feat_bf16(m == 0 ? ind::scale_feature(smax[v])
-                                                         : ind::pattern_feature(d[kc][v], smax[v]));
-            xb[kc] = ind::pack4(f);
+                x[v] = !okc ? 0.0 : (m == 0 ? ind::scale_feature(smax[v]) : ind::pattern_feature(d[kc][v], is[v]));
+            xb[kc] = ind::pack_feats(x);
         }
         const float z = ind::mlp_wave(t.net, xb, lane);
-        const float prob = 1.0f / (1.0f + expf(-z));
-        const int sm = (vm && okc && prob >= t.net.threshold) ? 1 : 0;
+        const int sm = (vm && okc && z >= t.net.logit_threshold) ? 1 : 0;
         const uint64_t bal = __ballot(sm && gq == 0);
-        if (lane == 0 && bal) atomicAdd(t.gated, (unsigned long long)__popcll(bal));
+        if (lane == 0 && bal) t.gated[lb * 4u + (threadIdx.x >> 6)] += (uint32_t)__popcll(bal);
         smooth = __shfl(sm, lane >> 2) != 0;
     }
 
     const uint32_t info0 = g.info[cc];
-    const bool active = valid && ((info0 >> (20 + q)) & 1u) && !(smooth && q != 0);
     const D4 uc = ld4(U, cc);
-
-    // ---- phase A: lane q = stencil
-    double a[KP][4];
-#pragma unroll
-    for (int k = 0; k < KP; ++k)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) a[k][v] = 0.0;
-    double w[4] = {0.0, 0.0, 0.0, 0.0};
-    if (active) {
-        // blocked tables: this block's rows start at (lb * 4 + q) * Mp
-        const size_t row0 = ((size_t)lb * 4 + q) * Mp;
-        const uint32_t cl = c & 63u;
-        auto member = [&](int i) {
-            const size_t o = (row0 + i) * 64 + cl;
-            const uint32_t pos = loc[o];
-            const double tt = t.ta[o];
-            const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
-            const D4 un = {{u01.x, u01.y, u23.x, u23.y}};
-            double b[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) b[v] = tt * (un.v[v] - uc.v[v]);
-            const double* Ai = t.A + (row0 + i) * (KP * 64) + cl;
-#pragma unroll
-            for (int k = 0; k < KP; ++k) {
-                const double Ak = Ai[k * 64];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) a[k][v] = a[k][v] + Ak * b[v];
-            }
-        };
-        if constexpr (MP > 0) {
-#pragma unroll
-            for (int i = 0; i < MP; ++i) member(i);
-        } else {
-            for (int i = 0; i < Mp; ++i) member(i);
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            double si = 0.0;
-#pragma unroll
-            for (int k = 0; k < KP; ++k) {
-                double oia = 0.0;
-#pragma unroll
-                for (int j = 0; j < KP; ++j) oia = oia + t.OI[k * KP + j] * a[j][v];
-                si = si + a[k][v] * oia;
-            }
-            w[v] = 1.0 / pow6_cr(si + 1.0e-12);
-        }
-    }
-
-    // ---- exchange: lane q = variable
     double ws[4];
-    quad_transpose(w, ws, q);
     double at[4][KP];
+    const uint32_t cl = c & 63u;
+    // the member update a += A+_i (ta_i (u_i - u_c)) of stencil row0's member i
+    auto member = [&](size_t row0, int i, double (&a)[KP][4]) {
+        const size_t o = (row0 + i) * 64 + cl;
+        const uint32_t pos = loc[o];
+        const double tt = t.ta[o];
+        const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
+        const D4 un = {{u01.x, u01.y, u23.x, u23.y}};
+        double b[4];
 #pragma unroll
-    for (int k = 0; k < KP; ++k) {
-        double x[4] = {a[k][0], a[k][1], a[k][2], a[k][3]}, y[4];
-        quad_transpose(x, y, q);
+        for (int v = 0; v < 4; ++v) b[v] = tt * (un.v[v] - uc.v[v]);
+        const double* Ai = t.A + (row0 + i) * (KP * 64) + cl;
 #pragma unroll
-        for (int st = 0; st < 4; ++st) at[st][k] = y[st];
+        for (int k = 0; k < KP; ++k) {
+            const double Ak = Ai[k * 64];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[k][v] = a[k][v] + Ak * b[v];
+        }
+    };
+
+    if (t.gate && __all(smooth || !valid)) {
+        // ---- every cell of the wave takes its central stencil: the quad
+        // splits its members (lane q: members q, q+4, q+8, ...), then a
+        // reduce-scatter over the quad leaves lane v the coefficients of
+        // variable v, summed as (lanes 0 + 2) + (lanes 1 + 3).
+        smooth = true;  // lanes past the end too: they store nothing
+        double a[KP][4];
+#pragma unroll
+        for (int k = 0; k < KP; ++k)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[k][v] = 0.0;
+        if (valid && (info0 >> 20) & 1u) {
+            const size_t row0 = (size_t)lb * 4 * Mp;
+            if constexpr (MP > 0) {
+#pragma unroll
+                for (int j = 0; j < (MP + 3) / 4; ++j)
+                    if (q + 4 * j < MP) member(row0, q + 4 * j, a);
+            } else {
+                for (int i = q; i < Mp; i += 4) member(row0, i, a);
+            }
+        }
+        const bool hi = (q & 2) != 0, odd = (q & 1) != 0;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            // lanes q, q^2: keep variables 2 (q >> 1) + {0, 1}, send the other two
+            const double k0 = hi ? a[k][2] : a[k][0], k1 = hi ? a[k][3] : a[k][1];
+            const double s0 = hi ? a[k][0] : a[k][2], s1 = hi ? a[k][1] : a[k][3];
+            const double p0 = k0 + qperm<kRot2>(s0), p1 = k1 + qperm<kRot2>(s1);
+            // lanes q, q^1: keep variable q
+            const double kk = odd ? p1 : p0, ss = odd ? p0 : p1;
+            at[0][k] = kk + qperm<kXor1>(ss);
+#pragma unroll
+            for (int st = 1; st < 4; ++st) at[st][k] = 0.0;
+        }
+#pragma unroll
+        for (int st = 0; st < 4; ++st) ws[st] = 0.0;
+    } else {
+        const bool active = valid && ((info0 >> (20 + q)) & 1u) && !(smooth && q != 0);
+
+        // ---- phase A: lane q = stencil
+        double a[KP][4];
+#pragma unroll
+        for (int k = 0; k < KP; ++k)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[k][v] = 0.0;
+        double w[4] = {0.0, 0.0, 0.0, 0.0};
+        if (active) {
+            // blocked tables: this block's rows start at (lb * 4 + q) * Mp
+            const size_t row0 = ((size_t)lb * 4 + q) * Mp;
+            if constexpr (MP > 0) {
+#pragma unroll
+                for (int i = 0; i < MP; ++i) member(row0, i, a);
+            } else {
+                for (int i = 0; i < Mp; ++i) member(row0, i, a);
+            }
+            if (!smooth) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double si = 0.0;
+#pragma unroll
+                    for (int k = 0; k < KP; ++k) {
+                        double oia = 0.0;
+#pragma unroll
+                        for (int j = 0; j < KP; ++j) oia = oia + t.OI[k * KP + j] * a[j][v];
+                        si = si + a[k][v] * oia;
+                    }
+                    w[v] = 1.0 / pow6_cr(si + 1.0e-12);
+                }
+            }
+        }
+
+        // ---- exchange: lane q = variable
+        quad_transpose(w, ws, q);
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            double x[4] = {a[k][0], a[k][1], a[k][2], a[k][3]}, y[4];
+            quad_transpose(x, y, q);
+#pragma unroll
+            for (int st = 0; st < 4; ++st) at[st][k] = y[st];
+        }
     }
 
     // ---- phase B: cut-off (face_reconstruction.cpp:948-981)

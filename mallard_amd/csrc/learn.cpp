@@ -40,7 +40,7 @@ struct DeviceNet {
     const float* W2f = nullptr;
     void pack(const float* theta_dev, hipStream_t s) {
         hip_ok(dev::net_pack(theta_dev, buf.p, s), "net_pack");
-        net = dev::net_view(buf.p, 0.0f, &W2f);
+        net = dev::net_view(buf.p, 0.5, &W2f);
     }
 };
 

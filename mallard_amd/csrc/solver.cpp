@@ -418,7 +418,8 @@ void Solver::upload() {
     dt_hist_cap_ = kDtHistCap;
     lam_max_ = (unsigned long long*)dz(1);
     step_dev_ = (int64_t*)dz(1);
-    gated_ = (unsigned long long*)dz(1);
+    gated_slots_ = 4 * (size_t)((N + 63) / 64);
+    gated_ = (uint32_t*)dz((gated_slots_ + 1) / 2);
     dec_ = (uint8_t*)dalloc(N);
     HIP_OK(hipMemset(dec_, 0, N));
 }
@@ -488,18 +489,19 @@ void Solver::set_indicator(const float* params, double threshold) {
     HIP_OK(hipMemcpyAsync(theta_, params, kIndParams * sizeof(float), hipMemcpyHostToDevice, stream_));
     HIP_OK(dev::net_pack(theta_, net_buf_, stream_));
     synchronize();
-    gate_thr_ = (float)threshold;
+    gate_thr_ = threshold;
     gate_ = true;
     drop_graph();  // the captured step carries the gate in its kernel arguments
 }
 
 uint64_t Solver::gated_cells(bool reset) {
-    unsigned long long v = 0;
-    HIP_OK(hipMemcpyAsync(&v, gated_, sizeof(v), hipMemcpyDeviceToHost, stream_));
+    std::vector<uint32_t> slots(gated_slots_);
+    HIP_OK(hipMemcpyAsync(slots.data(), gated_, gated_slots_ * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
     synchronize();
+    uint64_t v = 0;
+    for (uint32_t s : slots) v += s;
     if (reset) {
-        const unsigned long long z = 0;
-        HIP_OK(hipMemcpyAsync(gated_, &z, sizeof(z), hipMemcpyHostToDevice, stream_));
+        HIP_OK(hipMemsetAsync(gated_, 0, gated_slots_ * sizeof(uint32_t), stream_));
         synchronize();
     }
     return v;

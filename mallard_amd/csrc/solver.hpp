@@ -181,11 +181,12 @@ private:
     double *fs_ = nullptr, *rhs_ = nullptr, *lam_cell_ = nullptr, *dt_dev_ = nullptr, *dt_hist_ = nullptr;
     unsigned long long* lam_max_ = nullptr;
     int64_t* step_dev_ = nullptr;
-    unsigned long long* gated_ = nullptr;
+    uint32_t* gated_ = nullptr;  // per-wave-slot smooth counts of the gated TENO kernel
+    size_t gated_slots_ = 0;
     void* net_buf_ = nullptr;
     float* theta_ = nullptr;
     bool gate_ = false;
-    float gate_thr_ = 0.9f;
+    double gate_thr_ = 0.9;
     uint8_t* dec_ = nullptr;
     int64_t dt_hist_cap_ = 0;
 

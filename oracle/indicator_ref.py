@@ -52,29 +52,34 @@ def _ilogb(x):
 def features(U, central, gamma):
     """bf16 features [N][48] of conserved states U[N][4] (FP64) with central
     stencils central[N][12] (target first), reference numbering: per member
-    1..11 the differences (rho_m/rho_c - 1, (u_m - u_c)/a_c, (v_m - v_c)/a_c,
-    p_m/p_c - 1) divided by their stencil-wide maximum magnitude s_v per
-    variable; member 0 carries max(-2, ilogb(s_v)/32) (-2.5 for s_v = 0).
-    Rows with a non-physical cell or member are zero (valid[i] False)."""
+    1..11 the differences (rho_m ir_c - 1, (u_m - u_c) ia_c, (v_m - v_c) ia_c,
+    p_m ip_c - 1), with ir = 1/rho, ia = 1/a, ip = 1/p and velocities and
+    pressures formed with 1/rho, multiplied by the reciprocal of their
+    stencil-wide maximum magnitude s_v per variable (0 for s_v = 0); member 0
+    carries max(-2, ilogb(s_v)/32) (-2.5 for s_v = 0). Rows with a
+    non-physical cell or member are zero (valid[i] False)."""
     U = np.asarray(U, dtype=np.float64)
     rho = U[:, 0]
     with np.errstate(all="ignore"):
-        u, v = U[:, 1] / rho, U[:, 2] / rho
-        p = (gamma - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] * U[:, 1] + U[:, 2] * U[:, 2]) / U[:, 0])
+        ir = 1.0 / rho
+        u, v = U[:, 1] * ir, U[:, 2] * ir
+        p = (gamma - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] * U[:, 1] + U[:, 2] * U[:, 2]) * ir)
         ok = (rho > 0.0) & (p > 0.0)
-        a = np.where(ok, np.sqrt(gamma * p / rho), 1.0)
+        ia = np.where(ok, 1.0 / np.sqrt(gamma * p * ir), 1.0)
+        ip = 1.0 / p
         D = np.zeros((len(U), N_MEMBERS, 4))
         for m in range(1, N_MEMBERS):
             j = central[:, m]
-            D[:, m, 0] = rho[j] / rho - 1.0
-            D[:, m, 1] = (u[j] - u) / a
-            D[:, m, 2] = (v[j] - v) / a
-            D[:, m, 3] = p[j] / p - 1.0
+            D[:, m, 0] = rho[j] * ir - 1.0
+            D[:, m, 1] = (u[j] - u) * ia
+            D[:, m, 2] = (v[j] - v) * ia
+            D[:, m, 3] = p[j] * ip - 1.0
             ok &= (rho[j] > 0.0) & (p[j] > 0.0) & np.isfinite(D[:, m]).all(axis=1)
         s = np.abs(D[:, 1:]).max(axis=1)
         pos = s > 0.0
         f0 = np.where(pos, np.maximum(-2.0, _ilogb(np.where(pos, s, 1.0)) / 32.0), -2.5)
-        pat = np.where(pos[:, None, :], D / np.where(pos, s, 1.0)[:, None, :], 0.0)
+        inv = np.where(pos, 1.0 / np.where(pos, s, 1.0), 0.0)
+        pat = D * inv[:, None, :]
     X = np.zeros((len(U), N_FEAT), np.uint16)
     X[:, 0:4] = _bits(f0)
     X[:, 4:] = _bits(pat[:, 1:].reshape(len(U), -1))
