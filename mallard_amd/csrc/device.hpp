@@ -106,7 +106,7 @@ struct CaseBufs {
     Geom g;
     Teno t;
     double *Un, *Ua, *Ub, *W, *prims, *h, *fs, *rhs, *lam_cell, *dt, *dt_hist;
-    unsigned long long* lam_max;
+    unsigned long long* lam_blk;  // per-block spectral maxima, (g.N + 255) / 256 of them
     int64_t* step;
     uint8_t* dec;
     uint32_t blk_teno0, blk_flux0;
@@ -121,11 +121,13 @@ struct BatchStage {
 hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs, uint8_t* decisions,
                       const Physics& ph, hipStream_t stream);
 hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, hipStream_t stream);
+// spectral radii: lam_cell per cell and, per 256-cell block, the bit pattern
+// of the block's largest radius in lam_blk (NaN radii are ignored)
 hipError_t spectral_radius(const Geom& g, const double* Un, const double* prims, const Physics& ph,
-                           double* lam_cell, unsigned long long* lam_max, hipStream_t stream);
-hipError_t finalize_dt(const unsigned long long* lam_max_in, double cfl, double fixed_dt, double* dt,
-                       double* dt_hist, int64_t hist_cap, int64_t* step, unsigned long long* lam_max_reset,
-                       hipStream_t stream);
+                           double* lam_cell, unsigned long long* lam_blk, hipStream_t stream);
+// dt = cfl / (max over the n_blk block maxima), or fixed_dt when > 0
+hipError_t finalize_dt(const unsigned long long* lam_blk, uint32_t n_blk, double cfl, double fixed_dt, double* dt,
+                       double* dt_hist, int64_t hist_cap, int64_t* step, hipStream_t stream);
 hipError_t copy_state(const double* src, double* dst, uint32_t N, hipStream_t stream);
 // Ensemble launchers: `cases` is a device array of n_cases descriptors whose
 // blk_* offsets are prefix sums; every case has the same K, M, face points,

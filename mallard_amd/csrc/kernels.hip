@@ -797,7 +797,7 @@ __global__ __launch_bounds__(256) void flux_batch_kernel(const CaseBufs* __restr
 // local order, max over the grid through a block max and one atomic.
 __device__ __forceinline__ void spectral_block(const Geom& g, const double* __restrict__ Un,
                                                const double* __restrict__ pr, const Physics& ph,
-                                               double* __restrict__ lam_cell, unsigned long long* lam_max,
+                                               double* __restrict__ lam_cell, unsigned long long* __restrict__ lam_blk,
                                                uint32_t lb) {
     const uint32_t N = g.N;
     const uint32_t c = lb * blockDim.x + threadIdx.x;
@@ -842,15 +842,33 @@ __device__ __forceinline__ void spectral_block(const Geom& g, const double* __re
     if (threadIdx.x == 0) {
         unsigned long long m = sh[0];
         for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = sh[i] > m ? sh[i] : m;
-        atomicMax(lam_max, m);
+        lam_blk[lb] = m;  // the block's maximum; the finalize kernel joins the blocks
     }
+}
+
+// max over n per-block maxima (bit patterns of non-negative doubles) by one
+// block; every thread returns the result
+__device__ inline unsigned long long join_blocks(const unsigned long long* __restrict__ lam_blk, uint32_t n) {
+    unsigned long long m = 0ull;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) m = lam_blk[i] > m ? lam_blk[i] : m;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o2 = __shfl_xor(m, off);
+        m = o2 > m ? o2 : m;
+    }
+    __shared__ unsigned long long sh[16];
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = sh[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = sh[i] > m ? sh[i] : m;
+    return m;
 }
 
 __global__ __launch_bounds__(256) void spectral_kernel(Geom g, const double* __restrict__ Un,
                                                        const double* __restrict__ pr, Physics ph,
                                                        double* __restrict__ lam_cell,
-                                                       unsigned long long* lam_max) {
-    spectral_block(g, Un, pr, ph, lam_cell, lam_max, blockIdx.x);
+                                                       unsigned long long* __restrict__ lam_blk) {
+    spectral_block(g, Un, pr, ph, lam_cell, lam_blk, blockIdx.x);
 }
 
 // spectral radii of every case in one launch (each block inside one case, so
@@ -860,42 +878,34 @@ __global__ __launch_bounds__(256) void spectral_batch_kernel(const CaseBufs* __r
     const uint32_t gb = blockIdx.x;
     if (gb >= total) return;
     const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, true)];
-    spectral_block(cs.g, cs.Un, cs.prims, ph, cs.lam_cell, cs.lam_max, gb - cs.blk_flux0);
+    spectral_block(cs.g, cs.Un, cs.prims, ph, cs.lam_cell, cs.lam_blk, gb - cs.blk_flux0);
 }
 
-__global__ void finalize_dt_kernel(const unsigned long long* lam_max_in, double cfl, double fixed_dt,
-                                   double* dt, double* dt_hist, int64_t hist_cap, int64_t* step,
-                                   unsigned long long* lam_max_reset) {
-    double d;
-    if (fixed_dt > 0.0) {
-        d = fixed_dt;
-    } else {
-        const double lm = __longlong_as_double((long long)*lam_max_in);
-        d = cfl / lm;
-    }
+// dt = cfl / max lambda over the per-block maxima (one block of 1024 threads)
+// or the fixed dt; its history entry and the step counter
+__global__ __launch_bounds__(1024) void finalize_dt_kernel(const unsigned long long* __restrict__ lam_blk,
+                                                           uint32_t n_blk, double cfl, double fixed_dt, double* dt,
+                                                           double* dt_hist, int64_t hist_cap, int64_t* step) {
+    const unsigned long long m = fixed_dt > 0.0 ? 0ull : join_blocks(lam_blk, n_blk);
+    if (threadIdx.x != 0) return;
+    const double d = fixed_dt > 0.0 ? fixed_dt : cfl / __longlong_as_double((long long)m);
     *dt = d;
     if (dt_hist && step) dt_hist[*step % hist_cap] = d;
     if (step) *step += 1;
-    if (lam_max_reset) *lam_max_reset = 0ull;
 }
 
-// per-case dt of an ensemble: thread k finalises case k
-__global__ void finalize_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, double cfl, double fixed_dt,
-                                      int64_t hist_cap) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// per-case dt of an ensemble: block k finalises case k
+__global__ __launch_bounds__(256) void finalize_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases,
+                                                             double cfl, double fixed_dt, int64_t hist_cap) {
+    const int k = blockIdx.x;
     if (k >= n_cases) return;
     const CaseBufs& cs = cases[k];
-    double d;
-    if (fixed_dt > 0.0) {
-        d = fixed_dt;
-    } else {
-        const double lm = __longlong_as_double((long long)*cs.lam_max);
-        d = cfl / lm;
-    }
+    const unsigned long long m = fixed_dt > 0.0 ? 0ull : join_blocks(cs.lam_blk, (cs.g.N + 255) / 256);
+    if (threadIdx.x != 0) return;
+    const double d = fixed_dt > 0.0 ? fixed_dt : cfl / __longlong_as_double((long long)m);
     *cs.dt = d;
     cs.dt_hist[*cs.step % hist_cap] = d;
     *cs.step += 1;
-    *cs.lam_max = 0ull;
 }
 
 __global__ void copy_kernel(const double* __restrict__ s, double* __restrict__ d, size_t n) {
@@ -1003,17 +1013,16 @@ hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, hip
 }
 
 hipError_t spectral_radius(const Geom& g, const double* Un, const double* prims, const Physics& ph,
-                           double* lam_cell, unsigned long long* lam_max, hipStream_t stream) {
+                           double* lam_cell, unsigned long long* lam_blk, hipStream_t stream) {
     const dim3 grid((g.N + 255) / 256), block(256);
-    hipLaunchKernelGGL(spectral_kernel, grid, block, 0, stream, g, Un, prims, ph, lam_cell, lam_max);
+    hipLaunchKernelGGL(spectral_kernel, grid, block, 0, stream, g, Un, prims, ph, lam_cell, lam_blk);
     return hipGetLastError();
 }
 
-hipError_t finalize_dt(const unsigned long long* lam_max_in, double cfl, double fixed_dt, double* dt,
-                       double* dt_hist, int64_t hist_cap, int64_t* step, unsigned long long* lam_max_reset,
-                       hipStream_t stream) {
-    hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1), 0, stream, lam_max_in, cfl, fixed_dt, dt, dt_hist,
-                       hist_cap, step, lam_max_reset);
+hipError_t finalize_dt(const unsigned long long* lam_blk, uint32_t n_blk, double cfl, double fixed_dt, double* dt,
+                       double* dt_hist, int64_t hist_cap, int64_t* step, hipStream_t stream) {
+    hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, stream, lam_blk, n_blk, cfl, fixed_dt, dt, dt_hist,
+                       hist_cap, step);
     return hipGetLastError();
 }
 
@@ -1064,8 +1073,8 @@ hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blo
 
 hipError_t finalize_batch(const CaseBufs* cases, int n_cases, double cfl, double fixed_dt, int64_t hist_cap,
                           hipStream_t stream) {
-    hipLaunchKernelGGL(finalize_batch_kernel, dim3((n_cases + 63) / 64), dim3(64), 0, stream, cases, n_cases, cfl,
-                       fixed_dt, hist_cap);
+    hipLaunchKernelGGL(finalize_batch_kernel, dim3(n_cases), dim3(256), 0, stream, cases, n_cases, cfl, fixed_dt,
+                       hist_cap);
     return hipGetLastError();
 }
 

@@ -416,7 +416,8 @@ void Solver::upload() {
     dt_dev_ = dz(1);
     dt_hist_ = dz(kDtHistCap);
     dt_hist_cap_ = kDtHistCap;
-    lam_max_ = (unsigned long long*)dz(1);
+    lam_blk_n_ = (N + 255) / 256;
+    lam_blk_ = (unsigned long long*)dz(lam_blk_n_);
     step_dev_ = (int64_t*)dz(1);
     gated_slots_ = 4 * (size_t)((N + 63) / 64);
     gated_ = (uint32_t*)dz((gated_slots_ + 1) / 2);
@@ -558,7 +559,7 @@ dev::CaseBufs Solver::case_bufs() const {
     c.lam_cell = lam_cell_;
     c.dt = dt_dev_;
     c.dt_hist = dt_hist_;
-    c.lam_max = lam_max_;
+    c.lam_blk = lam_blk_;
     c.step = step_dev_;
     c.dec = dec_;
     return c;
@@ -617,8 +618,8 @@ void Solver::stage_pass(int op, const double* Uin, double* Uout) {
 
 void Solver::dt_pass(double cfl, double fixed_dt) {
     if (fixed_dt <= 0.0)
-        HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_max_, stream_));
-    HIP_OK(dev::finalize_dt(lam_max_, cfl, fixed_dt, dt_dev_, dt_hist_, dt_hist_cap_, step_dev_, lam_max_, stream_));
+        HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_blk_, stream_));
+    HIP_OK(dev::finalize_dt(lam_blk_, lam_blk_n_, cfl, fixed_dt, dt_dev_, dt_hist_, dt_hist_cap_, step_dev_, stream_));
     mark(kStatDtMs);
 }
 
@@ -780,10 +781,10 @@ void Solver::finish_steps(int64_t s0, int64_t n, double* dts_out) {
 
 double Solver::calc_dt() {
     if (cfg_.use_cfl) {
-        HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_max_, stream_));
-        HIP_OK(dev::finalize_dt(lam_max_, cfg_.cfl, 0.0, dt_dev_, nullptr, 1, nullptr, lam_max_, stream_));
+        HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_blk_, stream_));
+        HIP_OK(dev::finalize_dt(lam_blk_, lam_blk_n_, cfg_.cfl, 0.0, dt_dev_, nullptr, 1, nullptr, stream_));
     } else {
-        HIP_OK(dev::finalize_dt(lam_max_, 0.0, cfg_.dt, dt_dev_, nullptr, 1, nullptr, nullptr, stream_));
+        HIP_OK(dev::finalize_dt(lam_blk_, lam_blk_n_, 0.0, cfg_.dt, dt_dev_, nullptr, 1, nullptr, stream_));
     }
     HIP_OK(hipMemcpyAsync(&dt_, dt_dev_, sizeof(double), hipMemcpyDeviceToHost, stream_));
     synchronize();

@@ -179,7 +179,8 @@ private:
     double *ta_ = nullptr, *A_ = nullptr, *fbasis_ = nullptr;
     double *Un_ = nullptr, *Ua_ = nullptr, *Ub_ = nullptr, *W_ = nullptr, *prims_ = nullptr, *h_ = nullptr;
     double *fs_ = nullptr, *rhs_ = nullptr, *lam_cell_ = nullptr, *dt_dev_ = nullptr, *dt_hist_ = nullptr;
-    unsigned long long* lam_max_ = nullptr;
+    unsigned long long* lam_blk_ = nullptr;  // per-block spectral maxima
+    uint32_t lam_blk_n_ = 0;
     int64_t* step_dev_ = nullptr;
     uint32_t* gated_ = nullptr;  // per-wave-slot smooth counts of the gated TENO kernel
     size_t gated_slots_ = 0;
