@@ -132,8 +132,9 @@ hipError_t copy_state(const double* src, double* dst, uint32_t N, hipStream_t st
 // Ensemble launchers: `cases` is a device array of n_cases descriptors whose
 // blk_* offsets are prefix sums; every case has the same K, M, face points,
 // tile-position width (small = 8-bit) and Riemann solver.
+// gated: some case runs the learned gate (the block stages the network in LDS)
 hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
-                      bool small, const Physics& ph, hipStream_t stream);
+                      bool small, bool gated, const Physics& ph, hipStream_t stream);
 hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const BatchStage& st, int riemann,
                       const Physics& ph, hipStream_t stream);
 hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const Physics& ph,

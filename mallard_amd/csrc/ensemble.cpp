@@ -67,8 +67,10 @@ void Ensemble::upload_table() {
     std::vector<dev::CaseBufs> tab;
     tab.reserve(cases_.size());
     uint32_t bt = 0, bf = 0;
+    gated_ = false;
     for (const auto& c : cases_) {
         dev::CaseBufs b = c->case_bufs();
+        gated_ = gated_ || b.t.gate != 0;  // the TENO launch then reserves LDS for the network
         b.blk_teno0 = bt;
         b.blk_flux0 = bf;
         bt += (c->L_.N + 63) / 64;
@@ -113,7 +115,8 @@ void Ensemble::launch_step(bool prof) {
     const int vac = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
     auto stage = [&](int op, int in, int out) {
         if (teno_) {
-            hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, ph_, stream_), "teno_batch");
+            hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, gated_, ph_, stream_),
+                   "teno_batch");
             mark(kStatTenoMs);
         }
         const dev::BatchStage st{op, teno_ ? 0 : 1, vac, in, out};

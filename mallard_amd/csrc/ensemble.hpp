@@ -52,7 +52,7 @@ private:
     dev::CaseBufs* d_cases_ = nullptr;
     uint32_t teno_blocks_ = 0, flux_blocks_ = 0;
     int K_ = 0, M_ = 0, FQ_ = 0;
-    bool small_ = true, teno_ = false;
+    bool small_ = true, teno_ = false, gated_ = false;
     dev::Physics ph_{};
     hipStream_t stream_ = nullptr;
     hipGraph_t graph_ = nullptr;

@@ -39,6 +39,9 @@ constexpr int kOffB3 = kOffW3 + kHid;
 static_assert(kOffB3 + 1 == kNParams, "parameter layout");
 // packed MFMA A operands: W1p[nt 4][kc 3][lane 64], W2p[nt 4][kc 4][lane 64], 4 bf16 each
 constexpr int kW1pVec = 4 * 3 * 64, kW2pVec = 4 * 4 * 64;
+// the inference part of the packed network (W1p, W2p, b1, b2, w3, b3 from
+// the buffer's start), padded to 16 bytes: what a block stages in LDS
+constexpr int kNetInferBytes = ((kW1pVec + kW2pVec) * 8 + (3 * kHid + 1) * 4 + 15) / 16 * 16;
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -141,6 +144,24 @@ __device__ inline v4s pack4(const uint16_t* f) {
     x[2] = (short)f[2];
     x[3] = (short)f[3];
     return x;
+}
+
+// Block-wide copy of the network's inference part into LDS (the caller
+// synchronises before use) and the view of that copy.
+__device__ inline void stage_net(const Net& net, unsigned char* lds) {
+    const uint4* src = static_cast<const uint4*>(net.W1p);
+    uint4* dst = reinterpret_cast<uint4*>(lds);
+    for (int i = threadIdx.x; i < kNetInferBytes / 16; i += blockDim.x) dst[i] = src[i];
+}
+__device__ inline Net lds_net(const Net& net, const unsigned char* lds) {
+    Net n = net;
+    n.W1p = lds;
+    n.W2p = lds + kW1pVec * 8;
+    n.b1 = reinterpret_cast<const float*>(lds + (kW1pVec + kW2pVec) * 8);
+    n.b2 = n.b1 + kHid;
+    n.w3 = n.b2 + kHid;
+    n.b3 = n.w3 + kHid;
+    return n;
 }
 
 // Forward pass of one wave. xb[kc] holds, for cell lane % 16, the features

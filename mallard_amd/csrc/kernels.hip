@@ -317,7 +317,7 @@ __device__ inline void quad_gather(double f, double* out, int q) {
 template <int KP, int MP, int FQ, typename IdxT, int TILE>
 __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const double* __restrict__ U,
                                            double* __restrict__ fs, uint8_t* __restrict__ dec, const Physics& ph,
-                                           uint32_t lb, double2* sh_u) {
+                                           uint32_t lb, double2* sh_u, unsigned char* sh_net) {
     const uint32_t N = g.N;
     if (lb * 64u >= N) return;  // whole block past the end (uniform)
     {
@@ -325,6 +325,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         const double2* U2 = reinterpret_cast<const double2*>(U);
         for (uint32_t j = threadIdx.x; j < 2 * nloc; j += blockDim.x)
             sh_u[j] = U2[2 * (size_t)t.blk_ids[t0 + (j >> 1)] + (j & 1u)];
+        if (t.gate) ind::stage_net(t.net, sh_net);  // the network, once per block
     }
     __syncthreads();
     const IdxT* __restrict__ loc = sizeof(IdxT) == 1 ? reinterpret_cast<const IdxT*>(t.loc8)
@@ -382,7 +383,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
                 x[v] = !okc ? 0.0 : (m == 0 ? ind::scale_feature(smax[v]) : ind::pattern_feature(d[kc][v], is[v]));
             xb[kc] = ind::pack_feats(x);
         }
-        const float z = ind::mlp_wave(t.net, xb, lane);
+        const float z = ind::mlp_wave(ind::lds_net(t.net, sh_net), xb, lane);
         const int sm = (vm && okc && z >= t.net.logit_threshold) ? 1 : 0;
         const uint64_t bal = __ballot(sm && gq == 0);
         if (lane == 0 && bal) t.gated[lb * 4u + (threadIdx.x >> 6)] += (uint32_t)__popcll(bal);
@@ -533,7 +534,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         for (int k = 0; k < KP; ++k) pw[st][k] = ws[st] * at[st][k];
 
     // face points (face_reconstruction.cpp:985-1031)
-    const double ucv = uc.v[v];
+    const double ucv = sel4(uc.v, v);  // lane-dependent pick by selects, not a stack array
     double f[FQ];
 #pragma unroll
     for (int fq = 0; fq < FQ; ++fq) {
@@ -585,7 +586,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         }
         if (theta < 1.0)
 #pragma unroll
-            for (int fq = 0; fq < FQ; ++fq) f[fq] = Uc[v] + theta * (f[fq] - Uc[v]);
+            for (int fq = 0; fq < FQ; ++fq) f[fq] = ucv + theta * (f[fq] - ucv);
     }
     const uint64_t cmask = __ballot(central);
     if (valid) {
@@ -599,9 +600,11 @@ template <int KP, int MP, int FQ, typename IdxT, int TILE>
 __global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
                                                    double* __restrict__ fs, uint8_t* __restrict__ dec,
                                                    Physics ph, uint32_t blocks_per_xcd) {
-    __shared__ double2 sh_u[2 * TILE];
+    // dynamic LDS: the tile's states (TILE cells), then the network when gated
+    extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
-    teno_block<KP, MP, FQ, IdxT, TILE>(g, t, U, fs, dec, ph, lb, sh_u);
+    teno_block<KP, MP, FQ, IdxT, TILE>(g, t, U, fs, dec, ph, lb, reinterpret_cast<double2*>(teno_smem),
+                                       teno_smem + TILE * 32);
 }
 
 // case of global block gb: the last case whose first block is <= gb
@@ -628,12 +631,12 @@ __device__ inline double* case_buf_w(const CaseBufs& c, int role) {
 template <int KP, int MP, int FQ, typename IdxT, int TILE>
 __global__ __launch_bounds__(256) void teno_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, int in_role,
                                                          uint32_t total, Physics ph, uint32_t blocks_per_xcd) {
-    __shared__ double2 sh_u[2 * TILE];
+    extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t gb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
     if (gb >= total) return;
     const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, false)];
     teno_block<KP, MP, FQ, IdxT, TILE>(cs.g, cs.t, case_buf(cs, in_role), cs.fs, cs.dec, ph, gb - cs.blk_teno0,
-                                       sh_u);
+                                       reinterpret_cast<double2*>(teno_smem), teno_smem + TILE * 32);
 }
 
 // ------------------------------------------------------------ flux + update
@@ -969,14 +972,15 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
     const dim3 grid(8 * per_xcd), block(256);
     const bool small = t.loc8 != nullptr;
     if (!small && t.loc16 == nullptr) return hipErrorInvalidValue;
+    const size_t net_lds = t.gate ? ind::kNetInferBytes : 0;
 #define MLR_TENO(KP, MP, FQV)                                                                              \
     do {                                                                                                   \
         if (small)                                                                                         \
-            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint8_t, 256>), grid, block, 0, stream, g, t, U, fs, \
-                               dec, ph, per_xcd);                                                          \
+            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint8_t, 256>), grid, block, 256 * 32 + net_lds, stream, \
+                               g, t, U, fs, dec, ph, per_xcd);                                             \
         else                                                                                               \
-            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint16_t, 1024>), grid, block, 0, stream, g, t, U,   \
-                               fs, dec, ph, per_xcd);                                                      \
+            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint16_t, 1024>), grid, block, 1024 * 32 + net_lds,  \
+                               stream, g, t, U, fs, dec, ph, per_xcd);                                     \
     } while (0)
     // the default stencil size (factor 2: M = 2K) is compiled with the member
     // loop unrolled; other sizes take the runtime loop
@@ -1027,19 +1031,20 @@ hipError_t finalize_dt(const unsigned long long* lam_blk, uint32_t n_blk, double
 }
 
 hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
-                      bool small, const Physics& ph, hipStream_t stream) {
+                      bool small, bool gated, const Physics& ph, hipStream_t stream) {
     const int Kp = K - 1, Mp = M - 1;
+    const size_t net_lds = gated ? ind::kNetInferBytes : 0;
     if (n_cases < 1 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
     const uint32_t per_xcd = (total_blocks + 7) / 8;
     const dim3 grid(8 * per_xcd), block(256);
 #define MLR_TB(KP, MP, FQV)                                                                                    \
     do {                                                                                                       \
         if (small)                                                                                             \
-            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint8_t, 256>), grid, block, 0, stream, cases,      \
-                               n_cases, in_role, total_blocks, ph, per_xcd);                                    \
+            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint8_t, 256>), grid, block, 256 * 32 + net_lds,    \
+                               stream, cases, n_cases, in_role, total_blocks, ph, per_xcd);                     \
         else                                                                                                   \
-            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint16_t, 1024>), grid, block, 0, stream, cases,    \
-                               n_cases, in_role, total_blocks, ph, per_xcd);                                    \
+            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint16_t, 1024>), grid, block, 1024 * 32 + net_lds, \
+                               stream, cases, n_cases, in_role, total_blocks, ph, per_xcd);                     \
     } while (0)
     if (FQ == 3 && Kp == 5 && Mp == 11) MLR_TB(5, 11, 3);
     else if (FQ == 3 && Kp == 5) MLR_TB(5, 0, 3);
