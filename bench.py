@@ -18,7 +18,8 @@ reported as ms_per_step_graph.
 
 cpu_baseline: the CPU oracle (the restatement of the reference's OpenMP path:
 AoS state, per-face atomics) timed on this host on a bounded sample: the same
-numerics on a 256x256 (131,072-triangle) mesh for about --cpu-seconds.
+numerics on a 512x512 (524,288-triangle) mesh for about --cpu-seconds
+(at most the workload's 100 steps).
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 3]
 """
@@ -157,7 +158,7 @@ def main():
     ap.add_argument("--integrator", default="SSPRK3", choices=["SSPRK3", "FE"],
                     help="FE adds one plain state copy per step (a known-size kernel for "
                          "calibrating HBM byte counters); the metric is defined for SSPRK3")
-    ap.add_argument("--cpu-nx", type=int, default=256)
+    ap.add_argument("--cpu-nx", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-indicator run")

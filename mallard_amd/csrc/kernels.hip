@@ -314,7 +314,7 @@ __device__ inline void quad_gather(double f, double* out, int q) {
 // fetched once per block instead of once per stencil member. Blocks are
 // mapped so that each of the 8 XCDs walks one contiguous range of the
 // space-filling-curve order.
-template <int KP, int MP, int FQ, typename IdxT, int TILE>
+template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE>
 __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const double* __restrict__ U,
                                            double* __restrict__ fs, uint8_t* __restrict__ dec, const Physics& ph,
                                            uint32_t lb, double2* sh_u, unsigned char* sh_net) {
@@ -325,7 +325,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         const double2* U2 = reinterpret_cast<const double2*>(U);
         for (uint32_t j = threadIdx.x; j < 2 * nloc; j += blockDim.x)
             sh_u[j] = U2[2 * (size_t)t.blk_ids[t0 + (j >> 1)] + (j & 1u)];
-        if (t.gate) ind::stage_net(t.net, sh_net);  // the network, once per block
+        if (GATE && t.gate) ind::stage_net(t.net, sh_net);  // the network, once per block
     }
     __syncthreads();
     const IdxT* __restrict__ loc = sizeof(IdxT) == 1 ? reinterpret_cast<const IdxT*>(t.loc8)
@@ -340,7 +340,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
     // (features from the LDS tile, one MFMA column per cell); a smooth cell
     // skips its directional stencils and takes the central one.
     bool smooth = false;
-    if (t.gate) {
+    if (GATE && t.gate) {
         const int lane = threadIdx.x & 63, gq = lane >> 4;
         const uint32_t cm = lb * 64u + (threadIdx.x >> 6) * 16u + (lane & 15);
         const bool vm = cm < N;
@@ -414,7 +414,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         }
     };
 
-    if (t.gate && __all(smooth || !valid)) {
+    if (GATE && t.gate && __all(smooth || !valid)) {
         // ---- every cell of the wave takes its central stencil: the quad
         // splits its members (lane q: members q, q+4, q+8, ...), then a
         // reduce-scatter over the quad leaves lane v the coefficients of
@@ -596,14 +596,14 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
     }
 }
 
-template <int KP, int MP, int FQ, typename IdxT, int TILE>
+template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE>
 __global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
                                                    double* __restrict__ fs, uint8_t* __restrict__ dec,
                                                    Physics ph, uint32_t blocks_per_xcd) {
     // dynamic LDS: the tile's states (TILE cells), then the network when gated
     extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
-    teno_block<KP, MP, FQ, IdxT, TILE>(g, t, U, fs, dec, ph, lb, reinterpret_cast<double2*>(teno_smem),
+    teno_block<KP, MP, FQ, IdxT, TILE, GATE>(g, t, U, fs, dec, ph, lb, reinterpret_cast<double2*>(teno_smem),
                                        teno_smem + TILE * 32);
 }
 
@@ -628,14 +628,15 @@ __device__ inline double* case_buf_w(const CaseBufs& c, int role) {
 // The TENO pass of every case of an ensemble in one launch: the grid covers
 // all cases' 64-cell blocks (XCD-aware order over the whole grid), each
 // block finds its case in the offset table and runs that case's block.
-template <int KP, int MP, int FQ, typename IdxT, int TILE>
+template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE>
 __global__ __launch_bounds__(256) void teno_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, int in_role,
                                                          uint32_t total, Physics ph, uint32_t blocks_per_xcd) {
     extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t gb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
     if (gb >= total) return;
     const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, false)];
-    teno_block<KP, MP, FQ, IdxT, TILE>(cs.g, cs.t, case_buf(cs, in_role), cs.fs, cs.dec, ph, gb - cs.blk_teno0,
+    teno_block<KP, MP, FQ, IdxT, TILE, GATE>(cs.g, cs.t, case_buf(cs, in_role), cs.fs, cs.dec, ph,
+                                             gb - cs.blk_teno0,
                                        reinterpret_cast<double2*>(teno_smem), teno_smem + TILE * 32);
 }
 
@@ -870,15 +871,19 @@ __device__ inline unsigned long long join_blocks(const unsigned long long* __res
 __global__ __launch_bounds__(256) void spectral_kernel(Geom g, const double* __restrict__ Un,
                                                        const double* __restrict__ pr, Physics ph,
                                                        double* __restrict__ lam_cell,
-                                                       unsigned long long* __restrict__ lam_blk) {
-    spectral_block(g, Un, pr, ph, lam_cell, lam_blk, blockIdx.x);
+                                                       unsigned long long* __restrict__ lam_blk,
+                                                       uint32_t blocks_per_xcd) {
+    // each XCD walks one contiguous range of blocks (its neighbours' states stay in its L2)
+    const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
+    if (lb * 256u >= g.N) return;  // past the last block (uniform)
+    spectral_block(g, Un, pr, ph, lam_cell, lam_blk, lb);
 }
 
 // spectral radii of every case in one launch (each block inside one case, so
 // the block max goes to that case's maximum)
 __global__ __launch_bounds__(256) void spectral_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases,
-                                                             uint32_t total, Physics ph) {
-    const uint32_t gb = blockIdx.x;
+                                                             uint32_t total, Physics ph, uint32_t blocks_per_xcd) {
+    const uint32_t gb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
     if (gb >= total) return;
     const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, true)];
     spectral_block(cs.g, cs.Un, cs.prims, ph, cs.lam_cell, cs.lam_blk, gb - cs.blk_flux0);
@@ -973,32 +978,40 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
     const bool small = t.loc8 != nullptr;
     if (!small && t.loc16 == nullptr) return hipErrorInvalidValue;
     const size_t net_lds = t.gate ? ind::kNetInferBytes : 0;
-#define MLR_TENO(KP, MP, FQV)                                                                              \
+#define MLR_TENO_G(KP, MP, FQV, G)                                                                        \
     do {                                                                                                   \
         if (small)                                                                                         \
-            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint8_t, 256>), grid, block, 256 * 32 + net_lds, stream, \
-                               g, t, U, fs, dec, ph, per_xcd);                                             \
+            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint8_t, 256, G>), grid, block, 256 * 32 + net_lds,  \
+                               stream, g, t, U, fs, dec, ph, per_xcd);                                     \
         else                                                                                               \
-            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint16_t, 1024>), grid, block, 1024 * 32 + net_lds,  \
+            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint16_t, 1024, G>), grid, block, 1024 * 32 + net_lds, \
                                stream, g, t, U, fs, dec, ph, per_xcd);                                     \
     } while (0)
+    // the gated kernels exist where the central stencil has the network's 12 members or more
+#define MLR_TENO(KP, MP, FQV)                         \
+    do {                                              \
+        if (t.gate) MLR_TENO_G(KP, MP, FQV, true);    \
+        else MLR_TENO_G(KP, MP, FQV, false);          \
+    } while (0)
+    if (t.gate && Mp < ind::kMembers - 1) return hipErrorInvalidValue;
     // the default stencil size (factor 2: M = 2K) is compiled with the member
     // loop unrolled; other sizes take the runtime loop
     if (FQ == 3) {
         switch (Kp) {
-        case 2: if (Mp == 5) MLR_TENO(2, 5, 3); else MLR_TENO(2, 0, 3); break;
+        case 2: if (Mp == 5) MLR_TENO_G(2, 5, 3, false); else MLR_TENO_G(2, 0, 3, false); break;
         case 5: if (Mp == 11) MLR_TENO(5, 11, 3); else MLR_TENO(5, 0, 3); break;
         case 9: MLR_TENO(9, 0, 3); break;
         default: return hipErrorInvalidValue;
         }
     } else {
         switch (Kp) {
-        case 2: MLR_TENO(2, 0, 6); break;
+        case 2: MLR_TENO_G(2, 0, 6, false); break;
         case 5: MLR_TENO(5, 0, 6); break;
         case 9: if (Mp == 19) MLR_TENO(9, 19, 6); else MLR_TENO(9, 0, 6); break;
         default: return hipErrorInvalidValue;
         }
     }
+#undef MLR_TENO_G
 #undef MLR_TENO
     return hipGetLastError();
 }
@@ -1018,8 +1031,9 @@ hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, hip
 
 hipError_t spectral_radius(const Geom& g, const double* Un, const double* prims, const Physics& ph,
                            double* lam_cell, unsigned long long* lam_blk, hipStream_t stream) {
-    const dim3 grid((g.N + 255) / 256), block(256);
-    hipLaunchKernelGGL(spectral_kernel, grid, block, 0, stream, g, Un, prims, ph, lam_cell, lam_blk);
+    const uint32_t nblk = (g.N + 255) / 256, per_xcd = (nblk + 7) / 8;
+    hipLaunchKernelGGL(spectral_kernel, dim3(8 * per_xcd), dim3(256), 0, stream, g, Un, prims, ph, lam_cell, lam_blk,
+                       per_xcd);
     return hipGetLastError();
 }
 
@@ -1037,20 +1051,27 @@ hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks,
     if (n_cases < 1 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
     const uint32_t per_xcd = (total_blocks + 7) / 8;
     const dim3 grid(8 * per_xcd), block(256);
-#define MLR_TB(KP, MP, FQV)                                                                                    \
+#define MLR_TB_G(KP, MP, FQV, G)                                                                               \
     do {                                                                                                       \
         if (small)                                                                                             \
-            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint8_t, 256>), grid, block, 256 * 32 + net_lds,    \
+            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint8_t, 256, G>), grid, block, 256 * 32 + net_lds, \
                                stream, cases, n_cases, in_role, total_blocks, ph, per_xcd);                     \
         else                                                                                                   \
-            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint16_t, 1024>), grid, block, 1024 * 32 + net_lds, \
-                               stream, cases, n_cases, in_role, total_blocks, ph, per_xcd);                     \
+            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint16_t, 1024, G>), grid, block,                 \
+                               1024 * 32 + net_lds, stream, cases, n_cases, in_role, total_blocks, ph, per_xcd); \
     } while (0)
+#define MLR_TB(KP, MP, FQV)                        \
+    do {                                           \
+        if (gated) MLR_TB_G(KP, MP, FQV, true);    \
+        else MLR_TB_G(KP, MP, FQV, false);         \
+    } while (0)
+    if (gated && Mp < ind::kMembers - 1) return hipErrorInvalidValue;
     if (FQ == 3 && Kp == 5 && Mp == 11) MLR_TB(5, 11, 3);
     else if (FQ == 3 && Kp == 5) MLR_TB(5, 0, 3);
-    else if (FQ == 3 && Kp == 2) MLR_TB(2, 0, 3);
+    else if (FQ == 3 && Kp == 2) MLR_TB_G(2, 0, 3, false);
     else if (FQ == 6 && Kp == 9) MLR_TB(9, 0, 6);
     else return hipErrorInvalidValue;
+#undef MLR_TB_G
 #undef MLR_TB
     return hipGetLastError();
 }
@@ -1071,8 +1092,9 @@ hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks,
 
 hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const Physics& ph,
                           hipStream_t stream) {
-    hipLaunchKernelGGL(spectral_batch_kernel, dim3(total_blocks), dim3(256), 0, stream, cases, n_cases, total_blocks,
-                       ph);
+    const uint32_t per_xcd = (total_blocks + 7) / 8;
+    hipLaunchKernelGGL(spectral_batch_kernel, dim3(8 * per_xcd), dim3(256), 0, stream, cases, n_cases, total_blocks,
+                       ph, per_xcd);
     return hipGetLastError();
 }
 
