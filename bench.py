@@ -124,6 +124,7 @@ def learned_run(args, mlr, dev, U_plain, value_plain, wall_plain):
     N = dev.n_cells
     dev.set_indicator(params, args.threshold)
     dev.init_quadrant(0.5, 0.5, QUADRANT)
+    tune = dev.autotune()  # the gated kernel's shapes, before the timed region
     dev.gated(reset=True)
     t = time.perf_counter()
     st = dev.run_steps_profiled(args.steps)
@@ -139,8 +140,117 @@ def learned_run(args, mlr, dev, U_plain, value_plain, wall_plain):
     return {"value": value, "ms_per_step": wall / args.steps * 1e3, "speedup": value / value_plain,
             "l1_rel": [float(x) for x in l1], "matched_l1": bool(np.all(l1 <= 1e-3)),
             "gated_fraction": gated / (N * 3 * args.steps),
-            "teno_ms": st["teno_ms"] / max(st["teno_n"], 1), "train_s": train_s,
+            "teno_ms": st["teno_ms"] / max(st["teno_n"], 1), "train_s": train_s, "tune": list(tune),
             "train_rows": int(len(y)), "train_loss": float(hist[-50:].mean()), "threshold": args.threshold}
+
+
+def ensemble_sizes(n_cases, seed):
+    """Quads per side of each case: log-uniform in [64, 1024]."""
+    rng = np.random.default_rng(seed)
+    return np.round(np.exp(rng.uniform(np.log(64), np.log(1024), n_cases))).astype(int)
+
+
+def ensemble_run(args, mlr):
+    """Batched ensemble: independent four-quadrant problems with seeded random
+    states on jittered triangle meshes of uneven sizes, advanced together (one
+    launch per kernel phase over all cases, one hipGraph per step), plain
+    and with the learned gate. Returns the JSON record."""
+    from mallard_amd import indicator as ind
+    sizes = ensemble_sizes(args.ens_cases, args.ens_size_seed)
+    rng = np.random.default_rng(args.seed)
+    states = [ind.random_quadrant_states(rng) for _ in sizes]
+    gens = [mlr.mesh_gen("cartesian_tri", int(n), int(n), 1.0, 1.0, args.jitter, args.seed + k)
+            for k, n in enumerate(sizes)]
+    t0 = time.perf_counter()
+    log(f"[bench] building {len(sizes)} cases ({sizes.min()}^2 .. {sizes.max()}^2 quads) ...")
+    ens = mlr.Ensemble(gens, mlr.numerics(**numerics_kw(args)))
+    N = int(ens.n_cells.sum())
+    log(f"[bench] ensemble setup {time.perf_counter() - t0:.1f} s: {N} cells in {len(sizes)} cases")
+
+    def init():
+        for k in range(len(sizes)):
+            ens.init_quadrant(k, 0.5, 0.5, states[k])
+
+    def physical(k):
+        U, W = ens.state(k)
+        return bool(np.isfinite(U).all() and (U[:, 0] > 0).all() and (W[:, 2] > 0).all())
+
+    init()
+    tune = ens.autotune()
+    log(f"[bench] autotuned (TENO occupancy, block, block): {tune}")
+    if args.warmup > 0:
+        ens.run_steps(args.warmup)
+    init()
+    t = time.perf_counter()
+    ens.run_steps(args.ens_steps)
+    wall = time.perf_counter() - t
+    value_plain = N * 3 * args.ens_steps / wall
+    ok = [physical(k) for k in range(len(sizes))]
+    log(f"[bench] plain: {wall:.3f} s for {args.ens_steps} steps, {value_plain / 1e9:.3f} G cell-updates/s, "
+        f"{sum(ok)}/{len(ok)} cases physical")
+    U_plain = [ens.state(k)[0] for k in range(len(sizes))]
+    plain = {"value": value_plain, "ms_per_step": wall / args.ens_steps * 1e3, "cases_physical": int(sum(ok)),
+             "tune": list(tune)}
+
+    learned = None
+    if not args.no_learned:
+        t = time.perf_counter()
+        ds, y = ind.build_dataset(n_cases=args.train_cases, nx=64, snapshots=(0, 8, 20), seed=args.seed)
+        params, hist = ind.train_indicator(ds, ind.make_pool(y, seed=args.seed), steps=3000, batch=4096, lr=2e-3,
+                                           seed=args.seed)
+        del ds
+        train_s = time.perf_counter() - t
+        ens.set_indicator(params, args.threshold)
+        init()
+        tune_g = ens.autotune()
+        ens.gated(reset=True)
+        t = time.perf_counter()
+        ens.run_steps(args.ens_steps)
+        wall_g = time.perf_counter() - t
+        gated = ens.gated(reset=True)
+        num = np.zeros(4)
+        den = np.zeros(4)
+        for k in range(len(sizes)):
+            if not (ok[k] and physical(k)):
+                continue
+            v = ens.volumes(k)[:, None]
+            num += (np.abs(ens.state(k)[0] - U_plain[k]) * v).sum(0)
+            den += (np.abs(U_plain[k]) * v).sum(0)
+        l1 = num / np.maximum(den, 1e-300)
+        value_g = N * 3 * args.ens_steps / wall_g
+        log(f"[bench] learned: {wall_g:.3f} s, x{value_g / value_plain:.2f}, gated "
+            f"{gated / (N * 3 * args.ens_steps):.3f}, rel L1 {l1}")
+        learned = {"value": value_g, "ms_per_step": wall_g / args.ens_steps * 1e3,
+                   "speedup": value_g / value_plain, "l1_rel": [float(x) for x in l1],
+                   "matched_l1": bool(np.all(l1 <= 1e-3)), "gated_fraction": gated / (N * 3 * args.ens_steps),
+                   "train_s": train_s, "tune": list(tune_g), "threshold": args.threshold}
+    matched = learned is not None and learned["matched_l1"]
+    return {
+        "metric": METRIC,
+        "value": learned["value"] if matched else value_plain,
+        "unit": "cell-updates/s per RK stage",
+        "n_gpus": 1,
+        "steps": args.ens_steps,
+        "warmup": args.warmup,
+        "ms_per_step": learned["ms_per_step"] if matched else plain["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched ensemble: {len(sizes)} four-quadrant Riemann problems, random states "
+                        f"(rho, p ~ U[0.1, 2], u, v ~ U[-1.5, 1.5], seed {args.seed}) on jittered triangle "
+                        f"meshes of {sizes.min()}^2 .. {sizes.max()}^2 quads (log-uniform draw, seed "
+                        f"{args.ens_size_seed}), TENO p={args.order} + HLLC + SSPRK3, CFL 0.4, one launch "
+                        f"per kernel phase, hipGraph per step",
+            "n_cells": N,
+            "n_cases": len(sizes),
+            "indicator": "learned MLP gate, threshold %g" % args.threshold if matched else "none",
+        },
+        "plain": plain,
+        "learned": learned,
+    }
 
 
 def main():
@@ -164,6 +274,12 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-indicator run")
     ap.add_argument("--train-cases", type=int, default=512)
     ap.add_argument("--threshold", type=float, default=0.9)
+    ap.add_argument("--workload", default="riemann", choices=["riemann", "ensemble"],
+                    help="riemann: the 2048^2 problem (default); ensemble: the batched ensemble")
+    ap.add_argument("--ens-cases", type=int, default=256)
+    ap.add_argument("--ens-steps", type=int, default=50)
+    ap.add_argument("--ens-size-seed", type=int, default=2112,
+                    help="seed of the case-size draw (2112: 70.3M cells in 256 cases)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -174,6 +290,10 @@ def main():
 
     import mallard_amd as mlr
 
+    if args.workload == "ensemble":
+        print(json.dumps(ensemble_run(args, mlr)), flush=True)
+        return
+
     t0 = time.perf_counter()
     log(f"[bench] building {args.nx}x{args.nx} jittered triangle mesh + TENO p={args.order} layout ...")
     dev = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", args.nx, args.nx, 1.0, 1.0, args.jitter, args.seed),
@@ -183,6 +303,9 @@ def main():
     log(f"[bench] setup {time.perf_counter() - t0:.1f} s: {N} cells, {dev.n_stencils} stencils, "
         f"device layout {dev.device_bytes / 1e9:.2f} GB")
 
+    # launch shapes picked on this mesh before anything is timed
+    tune = dev.autotune()
+    log(f"[bench] autotuned launch shapes (TENO occupancy, flux block, spectral block): {tune}")
     if args.warmup > 0:
         dev.run_steps(args.warmup)
     log(f"[bench] warmup {args.warmup} steps done; timing {args.steps} steps")
@@ -273,6 +396,7 @@ def main():
         "kernels_ms": {"teno": teno_ms, "flux_update": flux_ms,
                        "spectral_dt": st["dt_ms"] / max(st["dt_n"], 1),
                        "copy": st["copy_ms"] / max(st["copy_n"], 1) if st["copy_n"] else None},
+        "tune": {"teno_occupancy": tune[0], "flux_block": tune[1], "spectral_block": tune[2]},
     }
     if not args.no_cpu:
         log("[bench] CPU baseline ...")

@@ -145,6 +145,17 @@ int mlr_run_steps(mlr_ctx* ctx, int64_t n, double* dts);
  *              TENO face-state ms, launches,  flux+stage-update ms, launches,
  *              spectral-radius+dt ms, launches, copy ms, launches }. */
 int mlr_run_steps_profiled(mlr_ctx* ctx, int64_t n, double* stats);
+/* Launch shapes; no reference counterpart (the reference leaves them to
+ * Kokkos' defaults). tune[3] = { TENO occupancy target in waves per SIMD
+ * (0: the compiler's choice, 5, 6; used at p = 2 with the default stencil
+ * size), cells per flux/stage-update block (64, 128, 256), cells per
+ * spectral-radius block (64, 128, 256) }. Every choice gives identical
+ * results. mlr_autotune times the candidates on the current state and
+ * returns the fastest (also run by itself before the first steps, cached per
+ * mesh size and numerics); mlr_set_tune pins a choice. */
+int mlr_autotune(mlr_ctx* ctx, int32_t* tune);
+int mlr_set_tune(mlr_ctx* ctx, const int32_t* tune);
+int mlr_get_tune(mlr_ctx* ctx, int32_t* tune);
 /* Solver::run (solver.cpp:352-373) with the configured stop conditions. */
 int mlr_run(mlr_ctx* ctx);
 int mlr_set_cfl(mlr_ctx* ctx, double cfl);
@@ -222,9 +233,15 @@ int mlr_ensemble_sizes(const mlr_ensemble* ens, int64_t* n_cells);  /* [n_cases]
 int mlr_ensemble_init_quadrant(mlr_ensemble* ens, int k, double xs, double ys, const double* states);
 int mlr_ensemble_get_state(mlr_ensemble* ens, int k, double* U, double* prim);
 int mlr_ensemble_get_time(mlr_ensemble* ens, int k, double* t, int64_t* step);
+/* cell volumes of case k, reference numbering */
+int mlr_ensemble_get_volumes(mlr_ensemble* ens, int k, double* vol);
 int mlr_ensemble_run_steps(mlr_ensemble* ens, int64_t n);
 /* kernel-by-kernel with events (the stats layout of mlr_run_steps_profiled) */
 int mlr_ensemble_run_steps_profiled(mlr_ensemble* ens, int64_t n, double* stats);
+/* launch shapes of the batched kernels, as mlr_autotune / mlr_set_tune (the
+ * flux and spectral block sizes are one value here: tune[2] = tune[1]) */
+int mlr_ensemble_autotune(mlr_ensemble* ens, int32_t* tune);
+int mlr_ensemble_set_tune(mlr_ensemble* ens, const int32_t* tune);
 int mlr_ensemble_set_indicator(mlr_ensemble* ens, const float* params, double threshold);
 int mlr_ensemble_gated(mlr_ensemble* ens, int reset, int64_t* gated);
 

@@ -63,6 +63,9 @@ _SIGS = {
     "mlr_step": (C.c_int, [_P]),
     "mlr_run_steps": (C.c_int, [_P, C.c_int64, _P]),
     "mlr_run_steps_profiled": (C.c_int, [_P, C.c_int64, _P]),
+    "mlr_autotune": (C.c_int, [_P, _P]),
+    "mlr_set_tune": (C.c_int, [_P, _P]),
+    "mlr_get_tune": (C.c_int, [_P, _P]),
     "mlr_run": (C.c_int, [_P]),
     "mlr_set_cfl": (C.c_int, [_P, C.c_double]),
     "mlr_get_time": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
@@ -94,6 +97,9 @@ _SIGS = {
     "mlr_ensemble_get_time": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mlr_ensemble_run_steps": (C.c_int, [_P, C.c_int64]),
     "mlr_ensemble_run_steps_profiled": (C.c_int, [_P, C.c_int64, _P]),
+    "mlr_ensemble_autotune": (C.c_int, [_P, _P]),
+    "mlr_ensemble_get_volumes": (C.c_int, [_P, C.c_int, _P]),
+    "mlr_ensemble_set_tune": (C.c_int, [_P, _P]),
     "mlr_ensemble_set_indicator": (C.c_int, [_P, _P, C.c_double]),
     "mlr_ensemble_gated": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
     "mlr_write_vtu": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_char_p), C.c_int]),
@@ -330,6 +336,22 @@ class Solver:
         return dict(total_ms=st[0], teno_ms=st[1], teno_n=int(st[2]), flux_ms=st[3], flux_n=int(st[4]),
                     dt_ms=st[5], dt_n=int(st[6]), copy_ms=st[7], copy_n=int(st[8]))
 
+    def autotune(self):
+        """Time the launch-shape candidates on the current state and keep the
+        fastest: (TENO occupancy target, flux block, spectral block)."""
+        t = np.zeros(3, np.int32)
+        self._check(lib().mlr_autotune(self.h, _p(t)))
+        return tuple(int(x) for x in t)
+
+    def set_tune(self, teno_occ=0, flux_block=256, spec_block=256):
+        t = np.array([teno_occ, flux_block, spec_block], np.int32)
+        self._check(lib().mlr_set_tune(self.h, _p(t)))
+
+    def tune(self):
+        t = np.zeros(3, np.int32)
+        self._check(lib().mlr_get_tune(self.h, _p(t)))
+        return tuple(int(x) for x in t)
+
     def run(self):
         self._check(lib().mlr_run(self.h))
 
@@ -463,6 +485,11 @@ class Ensemble:
         self._check(lib().mlr_ensemble_get_state(self.h, k, _p(U), _p(W)))
         return U, W
 
+    def volumes(self, k):
+        v = np.zeros(int(self.n_cells[k]))
+        self._check(lib().mlr_ensemble_get_volumes(self.h, k, _p(v)))
+        return v
+
     def time(self, k):
         t, s = C.c_double(), C.c_int64()
         lib().mlr_ensemble_get_time(self.h, k, C.byref(t), C.byref(s))
@@ -480,6 +507,15 @@ class Ensemble:
     def set_indicator(self, params, threshold=0.9):
         p = None if params is None else _params(params)
         self._check(lib().mlr_ensemble_set_indicator(self.h, _p(p), threshold))
+
+    def autotune(self):
+        t = np.zeros(3, np.int32)
+        self._check(lib().mlr_ensemble_autotune(self.h, _p(t)))
+        return tuple(int(x) for x in t)
+
+    def set_tune(self, teno_occ=0, block=256):
+        t = np.array([teno_occ, block, block], np.int32)
+        self._check(lib().mlr_ensemble_set_tune(self.h, _p(t)))
 
     def gated(self, reset=False):
         n = C.c_int64()

@@ -312,9 +312,39 @@ int mlr_run_steps(mlr_ctx* ctx, int64_t n, double* dts) {
     return guarded(ctx, [&] { ctx->solver.run_steps(n, dts); });
 }
 
+namespace {
+void tune_out(const mlr::dev::Tune& t, int32_t* out) {
+    out[0] = t.teno_occ;
+    out[1] = (int32_t)t.flux_block;
+    out[2] = (int32_t)t.spec_block;
+}
+mlr::dev::Tune tune_in(const int32_t* in) {
+    mlr::dev::Tune t;
+    t.teno_occ = in[0];
+    t.flux_block = (uint32_t)in[1];
+    t.spec_block = (uint32_t)in[2];
+    return t;
+}
+}  // namespace
+
 int mlr_run_steps_profiled(mlr_ctx* ctx, int64_t n, double* stats) {
     if (!ctx || !stats) return 1;
     return guarded(ctx, [&] { ctx->solver.run_steps_profiled(n, stats); });
+}
+int mlr_autotune(mlr_ctx* ctx, int32_t* tune) {
+    if (!ctx || !tune) return 1;
+    return guarded(ctx, [&] {
+        ctx->solver.autotune();
+        tune_out(ctx->solver.tune(), tune);
+    });
+}
+int mlr_set_tune(mlr_ctx* ctx, const int32_t* tune) {
+    if (!ctx || !tune) return 1;
+    return guarded(ctx, [&] { ctx->solver.set_tune(tune_in(tune)); });
+}
+int mlr_get_tune(mlr_ctx* ctx, int32_t* tune) {
+    if (!ctx || !tune) return 1;
+    return guarded(ctx, [&] { tune_out(ctx->solver.tune(), tune); });
 }
 
 int mlr_run(mlr_ctx* ctx) {
@@ -677,6 +707,11 @@ int mlr_ensemble_get_state(mlr_ensemble* ens, int k, double* U, double* prim) {
     return guarded(nullptr, [&] { ens->e->solver(k).get_state(U, prim); });
 }
 
+int mlr_ensemble_get_volumes(mlr_ensemble* ens, int k, double* vol) {
+    if (!ens || !vol || k < 0 || k >= ens->e->size()) return 1;
+    return guarded(nullptr, [&] { ens->e->solver(k).get_volumes(vol); });
+}
+
 int mlr_ensemble_get_time(mlr_ensemble* ens, int k, double* t, int64_t* step) {
     if (!ens || k < 0 || k >= ens->e->size()) return 1;
     if (t) *t = ens->e->solver(k).time();
@@ -692,6 +727,17 @@ int mlr_ensemble_run_steps(mlr_ensemble* ens, int64_t n) {
 int mlr_ensemble_run_steps_profiled(mlr_ensemble* ens, int64_t n, double* stats) {
     if (!ens || !stats) return 1;
     return guarded(nullptr, [&] { ens->e->run_steps_profiled(n, stats); });
+}
+int mlr_ensemble_autotune(mlr_ensemble* ens, int32_t* tune) {
+    if (!ens || !tune) return 1;
+    return guarded(nullptr, [&] {
+        ens->e->autotune();
+        tune_out(ens->e->tune(), tune);
+    });
+}
+int mlr_ensemble_set_tune(mlr_ensemble* ens, const int32_t* tune) {
+    if (!ens || !tune) return 1;
+    return guarded(nullptr, [&] { ens->e->set_tune(tune_in(tune)); });
 }
 
 int mlr_ensemble_set_indicator(mlr_ensemble* ens, const float* params, double threshold) {

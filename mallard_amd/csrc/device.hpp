@@ -106,7 +106,7 @@ struct CaseBufs {
     Geom g;
     Teno t;
     double *Un, *Ua, *Ub, *W, *prims, *h, *fs, *rhs, *lam_cell, *dt, *dt_hist;
-    unsigned long long* lam_blk;  // per-block spectral maxima, (g.N + 255) / 256 of them
+    unsigned long long* lam_blk;  // per-block spectral maxima, room for (g.N + 63) / 64 of them
     int64_t* step;
     uint8_t* dec;
     uint32_t blk_teno0, blk_flux0;
@@ -117,30 +117,41 @@ struct BatchStage {
     int32_t in, out;  // BufRole of the stage input and output
 };
 
+// Launch shapes an autotuner picks per mesh. Every choice gives identical
+// results; only the speed differs.
+struct Tune {
+    int32_t teno_occ = 0;       // TENO waves per SIMD targeted at p = 2, stencil factor 2: 0 (the
+                                // compiler's choice), 5 or 6
+    uint32_t flux_block = 256;  // cells per flux / stage-update block: 64, 128 or 256
+    uint32_t spec_block = 256;  // cells per spectral-radius block: 64, 128 or 256
+};
+
 // Launchers (asynchronous on `stream`). Each returns a hipError_t.
 hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs, uint8_t* decisions,
-                      const Physics& ph, hipStream_t stream);
-hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, hipStream_t stream);
-// spectral radii: lam_cell per cell and, per 256-cell block, the bit pattern
+                      const Physics& ph, int occ, hipStream_t stream);
+// bs: cells (threads) per block
+hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, uint32_t bs, hipStream_t stream);
+// spectral radii: lam_cell per cell and, per bs-cell block, the bit pattern
 // of the block's largest radius in lam_blk (NaN radii are ignored)
 hipError_t spectral_radius(const Geom& g, const double* Un, const double* prims, const Physics& ph,
-                           double* lam_cell, unsigned long long* lam_blk, hipStream_t stream);
+                           double* lam_cell, unsigned long long* lam_blk, uint32_t bs, hipStream_t stream);
 // dt = cfl / (max over the n_blk block maxima), or fixed_dt when > 0
 hipError_t finalize_dt(const unsigned long long* lam_blk, uint32_t n_blk, double cfl, double fixed_dt, double* dt,
                        double* dt_hist, int64_t hist_cap, int64_t* step, hipStream_t stream);
 hipError_t copy_state(const double* src, double* dst, uint32_t N, hipStream_t stream);
 // Ensemble launchers: `cases` is a device array of n_cases descriptors whose
-// blk_* offsets are prefix sums; every case has the same K, M, face points,
-// tile-position width (small = 8-bit) and Riemann solver.
+// blk_* offsets are prefix sums (flux and spectral blocks of bs cells, TENO
+// blocks of 64); every case has the same K, M, face points, tile-position
+// width (small = 8-bit) and Riemann solver.
 // gated: some case runs the learned gate (the block stages the network in LDS)
 hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
-                      bool small, bool gated, const Physics& ph, hipStream_t stream);
+                      bool small, bool gated, int occ, const Physics& ph, hipStream_t stream);
 hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const BatchStage& st, int riemann,
-                      const Physics& ph, hipStream_t stream);
-hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const Physics& ph,
+                      const Physics& ph, uint32_t bs, hipStream_t stream);
+hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const Physics& ph, uint32_t bs,
                           hipStream_t stream);
 hipError_t finalize_batch(const CaseBufs* cases, int n_cases, double cfl, double fixed_dt, int64_t hist_cap,
-                          hipStream_t stream);
+                          uint32_t bs, hipStream_t stream);
 hipError_t probe_math(const double* x, const double* y, double* out, int n, int which, hipStream_t stream);
 
 // TENO pseudo-inverses on the GPU (precompute.hip), bit-identical to the host

@@ -2,6 +2,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 
 namespace mlr {
@@ -66,6 +67,7 @@ uint64_t Ensemble::n_cells() const {
 void Ensemble::upload_table() {
     std::vector<dev::CaseBufs> tab;
     tab.reserve(cases_.size());
+    const uint32_t bs = tune_.flux_block;
     uint32_t bt = 0, bf = 0;
     gated_ = false;
     for (const auto& c : cases_) {
@@ -74,7 +76,7 @@ void Ensemble::upload_table() {
         b.blk_teno0 = bt;
         b.blk_flux0 = bf;
         bt += (c->L_.N + 63) / 64;
-        bf += (c->L_.N + 255) / 256;
+        bf += (c->L_.N + bs - 1) / bs;
         tab.push_back(b);
     }
     teno_blocks_ = bt;
@@ -91,6 +93,89 @@ void Ensemble::set_indicator(const float* params, double threshold) {
     for (auto& c : cases_) c->set_indicator(params, threshold);
     upload_table();
     drop_graph();
+    tuned_ = false;  // the gated kernel has its own best shape
+}
+
+void Ensemble::set_tune(const dev::Tune& t) {
+    auto bs_ok = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
+    if (!bs_ok(t.flux_block)) fail("block sizes must be 64, 128 or 256 cells");
+    if (t.teno_occ != 0 && t.teno_occ != 5 && t.teno_occ != 6) fail("TENO occupancy target must be 0, 5 or 6");
+    tune_ = t;
+    tune_.spec_block = t.flux_block;  // flux and spectral blocks share the case table's offsets
+    tuned_ = true;
+    upload_table();
+    drop_graph();
+}
+
+// Times the candidate shapes on the cases' current states with launches that
+// write only scratch outputs (face states, the first stage's U and
+// accumulator, the radii); gated-cell counters are restored afterwards.
+void Ensemble::autotune() {
+    drop_graph();
+    const int n = size();
+    hipEvent_t e0, e1;
+    hip_ok(hipEventCreate(&e0), "hipEventCreate");
+    hip_ok(hipEventCreate(&e1), "hipEventCreate");
+    auto timed = [&](const std::function<void()>& f) {
+        f();
+        hip_ok(hipEventRecord(e0, stream_), "hipEventRecord");
+        for (int r = 0; r < 3; ++r) f();
+        hip_ok(hipEventRecord(e1, stream_), "hipEventRecord");
+        hip_ok(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0f;
+        hip_ok(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        return ms;
+    };
+    dev::Tune best = tune_;
+    const int vac = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
+    const int op = cfg_.integrator == Integrator::RK4 ? dev::kRK4_1 : dev::kSSP1;
+    const dev::BatchStage st{op, teno_ ? 0 : 1, vac, dev::kBufUn, dev::kBufUa};
+    float bt = 1e30f;
+    for (uint32_t bs : {64u, 128u, 256u}) {
+        tune_.flux_block = tune_.spec_block = bs;
+        upload_table();
+        const float t = timed([&] {
+            hip_ok(dev::spectral_batch(d_cases_, n, flux_blocks_, ph_, bs, stream_), "spectral_batch");
+            hip_ok(dev::flux_batch(d_cases_, n, flux_blocks_, st, (int)cfg_.riemann, ph_, bs, stream_), "flux_batch");
+        });
+        if (t < bt) { bt = t; best.flux_block = bs; }
+    }
+    tune_.flux_block = tune_.spec_block = best.flux_block;
+    upload_table();
+    if (teno_) {
+        std::vector<uint32_t*> saved(cases_.size(), nullptr);
+        if (gated_) {
+            for (size_t k = 0; k < cases_.size(); ++k) {
+                const Solver& c = *cases_[k];
+                hip_ok(hipMalloc((void**)&saved[k], c.gated_slots_ * sizeof(uint32_t)), "hipMalloc");
+                hip_ok(hipMemcpyAsync(saved[k], c.gated_, c.gated_slots_ * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                      stream_), "hipMemcpyAsync");
+            }
+        }
+        const bool occ_tunable = K_ == 6 && M_ == 12 && FQ_ == 3 && small_;
+        bt = 1e30f;
+        for (int occ : {0, 5, 6}) {
+            if (occ != 0 && !occ_tunable) continue;
+            const float t = timed([&] {
+                hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, dev::kBufUn, K_, M_, FQ_, small_, gated_, occ, ph_,
+                                       stream_), "teno_batch");
+            });
+            if (t < bt) { bt = t; best.teno_occ = occ; }
+        }
+        if (gated_) {
+            for (size_t k = 0; k < cases_.size(); ++k) {
+                const Solver& c = *cases_[k];
+                hip_ok(hipMemcpyAsync(c.gated_, saved[k], c.gated_slots_ * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                      stream_), "hipMemcpyAsync");
+            }
+            hip_ok(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+            for (uint32_t* p : saved) hip_ok(hipFree(p), "hipFree");
+        }
+    }
+    hip_ok(hipEventDestroy(e0), "hipEventDestroy");
+    hip_ok(hipEventDestroy(e1), "hipEventDestroy");
+    hip_ok(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    set_tune(best);
 }
 
 void Ensemble::mark(int tag) {
@@ -109,18 +194,20 @@ void Ensemble::launch_step(bool prof) {
     (void)prof;
     const int n = size();
     const int64_t cap = cases_[0]->dt_hist_cap_;
-    if (cfg_.use_cfl) hip_ok(dev::spectral_batch(d_cases_, n, flux_blocks_, ph_, stream_), "spectral_batch");
-    hip_ok(dev::finalize_batch(d_cases_, n, cfg_.cfl, cfg_.use_cfl ? 0.0 : cfg_.dt, cap, stream_), "finalize_batch");
+    const uint32_t bs = tune_.flux_block;
+    if (cfg_.use_cfl) hip_ok(dev::spectral_batch(d_cases_, n, flux_blocks_, ph_, bs, stream_), "spectral_batch");
+    hip_ok(dev::finalize_batch(d_cases_, n, cfg_.cfl, cfg_.use_cfl ? 0.0 : cfg_.dt, cap, bs, stream_),
+           "finalize_batch");
     mark(kStatDtMs);
     const int vac = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
     auto stage = [&](int op, int in, int out) {
         if (teno_) {
-            hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, gated_, ph_, stream_),
-                   "teno_batch");
+            hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, gated_, tune_.teno_occ, ph_,
+                                   stream_), "teno_batch");
             mark(kStatTenoMs);
         }
         const dev::BatchStage st{op, teno_ ? 0 : 1, vac, in, out};
-        hip_ok(dev::flux_batch(d_cases_, n, flux_blocks_, st, (int)cfg_.riemann, ph_, stream_), "flux_batch");
+        hip_ok(dev::flux_batch(d_cases_, n, flux_blocks_, st, (int)cfg_.riemann, ph_, bs, stream_), "flux_batch");
         mark(kStatFluxMs);
     };
     if (cfg_.integrator == Integrator::SSPRK3) {
@@ -141,6 +228,7 @@ void Ensemble::finish(int64_t s0, int64_t n) {
 
 void Ensemble::run_steps(int64_t n) {
     if (n <= 0) return;
+    if (!tuned_) autotune();
     const int64_t s0 = cases_[0]->device_step();
     if (!exec_) {
         hip_ok(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -163,6 +251,7 @@ void Ensemble::run_steps(int64_t n) {
 void Ensemble::run_steps_profiled(int64_t n, double* stats) {
     for (int k = 0; k < kStatCount; ++k) stats[k] = 0.0;
     if (n <= 0) return;
+    if (!tuned_) autotune();
     const int64_t s0 = cases_[0]->device_step();
     prof_on_ = true;
     prof_used_ = 0;

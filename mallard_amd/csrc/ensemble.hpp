@@ -35,6 +35,11 @@ public:
 
     // the learned indicator on every case (params NULL: off)
     void set_indicator(const float* params, double threshold);
+    // launch shapes (flux and spectral blocks share one size here), timed on
+    // the cases' current states before the first steps unless pinned
+    void autotune();
+    void set_tune(const dev::Tune& t);
+    const dev::Tune& tune() const { return tune_; }
     // n steps of every case, graph-replayed
     void run_steps(int64_t n);
     // the same launches one at a time with an event after each (Solver's StepStats layout)
@@ -54,6 +59,8 @@ private:
     int K_ = 0, M_ = 0, FQ_ = 0;
     bool small_ = true, teno_ = false, gated_ = false;
     dev::Physics ph_{};
+    dev::Tune tune_{};
+    bool tuned_ = false;
     hipStream_t stream_ = nullptr;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

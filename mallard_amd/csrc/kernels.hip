@@ -596,8 +596,11 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
     }
 }
 
-template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE>
-__global__ __launch_bounds__(256) void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
+// OCC: waves per SIMD the register allocation targets (0: the compiler's
+// own choice); an autotuner picks it per mesh, results are identical
+template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC == 0 ? 1 : OCC, OCC == 0 ? 8 : OCC)))
+void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
                                                    double* __restrict__ fs, uint8_t* __restrict__ dec,
                                                    Physics ph, uint32_t blocks_per_xcd) {
     // dynamic LDS: the tile's states (TILE cells), then the network when gated
@@ -628,8 +631,9 @@ __device__ inline double* case_buf_w(const CaseBufs& c, int role) {
 // The TENO pass of every case of an ensemble in one launch: the grid covers
 // all cases' 64-cell blocks (XCD-aware order over the whole grid), each
 // block finds its case in the offset table and runs that case's block.
-template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE>
-__global__ __launch_bounds__(256) void teno_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, int in_role,
+template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC == 0 ? 1 : OCC, OCC == 0 ? 8 : OCC)))
+void teno_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, int in_role,
                                                          uint32_t total, Physics ph, uint32_t blocks_per_xcd) {
     extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t gb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
@@ -771,7 +775,8 @@ __global__ __launch_bounds__(256) void flux_update_kernel(Geom g, StageArgs a, P
     flux_cell<RS>(g, a, ph, lb * blockDim.x + threadIdx.x);
 }
 
-// The flux + stage update of every case in one launch (256-cell blocks).
+// The flux + stage update of every case in one launch (blocks of blockDim.x
+// cells; the case table's block offsets count blocks of that size).
 template <int RS>
 __global__ __launch_bounds__(256) void flux_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases,
                                                          BatchStage st, uint32_t total, Physics ph,
@@ -875,7 +880,7 @@ __global__ __launch_bounds__(256) void spectral_kernel(Geom g, const double* __r
                                                        uint32_t blocks_per_xcd) {
     // each XCD walks one contiguous range of blocks (its neighbours' states stay in its L2)
     const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
-    if (lb * 256u >= g.N) return;  // past the last block (uniform)
+    if (lb * blockDim.x >= g.N) return;  // past the last block (uniform)
     spectral_block(g, Un, pr, ph, lam_cell, lam_blk, lb);
 }
 
@@ -904,11 +909,12 @@ __global__ __launch_bounds__(1024) void finalize_dt_kernel(const unsigned long l
 
 // per-case dt of an ensemble: block k finalises case k
 __global__ __launch_bounds__(256) void finalize_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases,
-                                                             double cfl, double fixed_dt, int64_t hist_cap) {
+                                                             double cfl, double fixed_dt, int64_t hist_cap,
+                                                             uint32_t bs) {
     const int k = blockIdx.x;
     if (k >= n_cases) return;
     const CaseBufs& cs = cases[k];
-    const unsigned long long m = fixed_dt > 0.0 ? 0ull : join_blocks(cs.lam_blk, (cs.g.N + 255) / 256);
+    const unsigned long long m = fixed_dt > 0.0 ? 0ull : join_blocks(cs.lam_blk, (cs.g.N + bs - 1) / bs);
     if (threadIdx.x != 0) return;
     const double d = fixed_dt > 0.0 ? fixed_dt : cfl / __longlong_as_double((long long)m);
     *cs.dt = d;
@@ -969,23 +975,27 @@ __global__ void probe_kernel(const double* x, const double* y, double* out, int 
 }  // namespace
 
 hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs, uint8_t* dec,
-                      const Physics& ph, hipStream_t stream) {
+                      const Physics& ph, int occ, hipStream_t stream) {
     const int Kp = t.K - 1, Mp = t.M - 1;
     const uint32_t FQ = g.S * g.Q;
     if (g.S != 3 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
+    if (occ != 0 && occ != 5 && occ != 6) return hipErrorInvalidValue;
     const uint32_t nblk = (g.N + 63) / 64, per_xcd = (nblk + 7) / 8;
     const dim3 grid(8 * per_xcd), block(256);
     const bool small = t.loc8 != nullptr;
     if (!small && t.loc16 == nullptr) return hipErrorInvalidValue;
     const size_t net_lds = t.gate ? ind::kNetInferBytes : 0;
-#define MLR_TENO_G(KP, MP, FQV, G)                                                                        \
-    do {                                                                                                   \
-        if (small)                                                                                         \
-            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint8_t, 256, G>), grid, block, 256 * 32 + net_lds,  \
-                               stream, g, t, U, fs, dec, ph, per_xcd);                                     \
-        else                                                                                               \
-            hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint16_t, 1024, G>), grid, block, 1024 * 32 + net_lds, \
-                               stream, g, t, U, fs, dec, ph, per_xcd);                                     \
+    // small: 8-bit tile positions (tiles of <= 256 cells); O: occupancy target
+#define MLR_TENO_S(KP, MP, FQV, G, O)                                                                       \
+    hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint8_t, 256, G, O>), grid, block, 256 * 32 + net_lds, stream, \
+                       g, t, U, fs, dec, ph, per_xcd)
+#define MLR_TENO_L(KP, MP, FQV, G)                                                                         \
+    hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint16_t, 1024, G, 0>), grid, block, 1024 * 32 + net_lds,   \
+                       stream, g, t, U, fs, dec, ph, per_xcd)
+#define MLR_TENO_G(KP, MP, FQV, G)                  \
+    do {                                            \
+        if (small) MLR_TENO_S(KP, MP, FQV, G, 0);   \
+        else MLR_TENO_L(KP, MP, FQV, G);            \
     } while (0)
     // the gated kernels exist where the central stencil has the network's 12 members or more
 #define MLR_TENO(KP, MP, FQV)                         \
@@ -995,11 +1005,28 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
     } while (0)
     if (t.gate && Mp < ind::kMembers - 1) return hipErrorInvalidValue;
     // the default stencil size (factor 2: M = 2K) is compiled with the member
-    // loop unrolled; other sizes take the runtime loop
+    // loop unrolled, and at p = 2 in three occupancy targets; other sizes
+    // take the runtime loop
     if (FQ == 3) {
         switch (Kp) {
         case 2: if (Mp == 5) MLR_TENO_G(2, 5, 3, false); else MLR_TENO_G(2, 0, 3, false); break;
-        case 5: if (Mp == 11) MLR_TENO(5, 11, 3); else MLR_TENO(5, 0, 3); break;
+        case 5:
+            if (Mp == 11 && small) {
+                if (t.gate) {
+                    if (occ == 5) MLR_TENO_S(5, 11, 3, true, 5);
+                    else if (occ == 6) MLR_TENO_S(5, 11, 3, true, 6);
+                    else MLR_TENO_S(5, 11, 3, true, 0);
+                } else {
+                    if (occ == 5) MLR_TENO_S(5, 11, 3, false, 5);
+                    else if (occ == 6) MLR_TENO_S(5, 11, 3, false, 6);
+                    else MLR_TENO_S(5, 11, 3, false, 0);
+                }
+            } else if (Mp == 11) {
+                MLR_TENO(5, 11, 3);
+            } else {
+                MLR_TENO(5, 0, 3);
+            }
+            break;
         case 9: MLR_TENO(9, 0, 3); break;
         default: return hipErrorInvalidValue;
         }
@@ -1011,14 +1038,17 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
         default: return hipErrorInvalidValue;
         }
     }
+#undef MLR_TENO_S
+#undef MLR_TENO_L
 #undef MLR_TENO_G
 #undef MLR_TENO
     return hipGetLastError();
 }
 
-hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, hipStream_t stream) {
-    const uint32_t nblk = (g.N + 255) / 256, per_xcd = (nblk + 7) / 8;
-    const dim3 grid(8 * per_xcd), block(256);
+hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, uint32_t bs, hipStream_t stream) {
+    if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
+    const uint32_t nblk = (g.N + bs - 1) / bs, per_xcd = (nblk + 7) / 8;
+    const dim3 grid(8 * per_xcd), block(bs);
     switch (a.riemann) {
     case 0: hipLaunchKernelGGL((flux_update_kernel<0>), grid, block, 0, stream, g, a, ph, per_xcd); break;
     case 1: hipLaunchKernelGGL((flux_update_kernel<1>), grid, block, 0, stream, g, a, ph, per_xcd); break;
@@ -1030,9 +1060,10 @@ hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, hip
 }
 
 hipError_t spectral_radius(const Geom& g, const double* Un, const double* prims, const Physics& ph,
-                           double* lam_cell, unsigned long long* lam_blk, hipStream_t stream) {
-    const uint32_t nblk = (g.N + 255) / 256, per_xcd = (nblk + 7) / 8;
-    hipLaunchKernelGGL(spectral_kernel, dim3(8 * per_xcd), dim3(256), 0, stream, g, Un, prims, ph, lam_cell, lam_blk,
+                           double* lam_cell, unsigned long long* lam_blk, uint32_t bs, hipStream_t stream) {
+    if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
+    const uint32_t nblk = (g.N + bs - 1) / bs, per_xcd = (nblk + 7) / 8;
+    hipLaunchKernelGGL(spectral_kernel, dim3(8 * per_xcd), dim3(bs), 0, stream, g, Un, prims, ph, lam_cell, lam_blk,
                        per_xcd);
     return hipGetLastError();
 }
@@ -1045,19 +1076,22 @@ hipError_t finalize_dt(const unsigned long long* lam_blk, uint32_t n_blk, double
 }
 
 hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
-                      bool small, bool gated, const Physics& ph, hipStream_t stream) {
+                      bool small, bool gated, int occ, const Physics& ph, hipStream_t stream) {
     const int Kp = K - 1, Mp = M - 1;
     const size_t net_lds = gated ? ind::kNetInferBytes : 0;
     if (n_cases < 1 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
+    if (occ != 0 && occ != 5 && occ != 6) return hipErrorInvalidValue;
     const uint32_t per_xcd = (total_blocks + 7) / 8;
     const dim3 grid(8 * per_xcd), block(256);
+#define MLR_TB_S(KP, MP, FQV, G, O)                                                                             \
+    hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint8_t, 256, G, O>), grid, block, 256 * 32 + net_lds,   \
+                       stream, cases, n_cases, in_role, total_blocks, ph, per_xcd)
 #define MLR_TB_G(KP, MP, FQV, G)                                                                               \
     do {                                                                                                       \
         if (small)                                                                                             \
-            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint8_t, 256, G>), grid, block, 256 * 32 + net_lds, \
-                               stream, cases, n_cases, in_role, total_blocks, ph, per_xcd);                     \
+            MLR_TB_S(KP, MP, FQV, G, 0);                                                                       \
         else                                                                                                   \
-            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint16_t, 1024, G>), grid, block,                 \
+            hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint16_t, 1024, G, 0>), grid, block,              \
                                1024 * 32 + net_lds, stream, cases, n_cases, in_role, total_blocks, ph, per_xcd); \
     } while (0)
 #define MLR_TB(KP, MP, FQV)                        \
@@ -1066,20 +1100,32 @@ hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks,
         else MLR_TB_G(KP, MP, FQV, false);         \
     } while (0)
     if (gated && Mp < ind::kMembers - 1) return hipErrorInvalidValue;
-    if (FQ == 3 && Kp == 5 && Mp == 11) MLR_TB(5, 11, 3);
+    if (FQ == 3 && Kp == 5 && Mp == 11 && small) {
+        if (gated) {
+            if (occ == 5) MLR_TB_S(5, 11, 3, true, 5);
+            else if (occ == 6) MLR_TB_S(5, 11, 3, true, 6);
+            else MLR_TB_S(5, 11, 3, true, 0);
+        } else {
+            if (occ == 5) MLR_TB_S(5, 11, 3, false, 5);
+            else if (occ == 6) MLR_TB_S(5, 11, 3, false, 6);
+            else MLR_TB_S(5, 11, 3, false, 0);
+        }
+    } else if (FQ == 3 && Kp == 5 && Mp == 11) MLR_TB(5, 11, 3);
     else if (FQ == 3 && Kp == 5) MLR_TB(5, 0, 3);
     else if (FQ == 3 && Kp == 2) MLR_TB_G(2, 0, 3, false);
     else if (FQ == 6 && Kp == 9) MLR_TB(9, 0, 6);
     else return hipErrorInvalidValue;
+#undef MLR_TB_S
 #undef MLR_TB_G
 #undef MLR_TB
     return hipGetLastError();
 }
 
 hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const BatchStage& st, int riemann,
-                      const Physics& ph, hipStream_t stream) {
+                      const Physics& ph, uint32_t bs, hipStream_t stream) {
+    if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
     const uint32_t per_xcd = (total_blocks + 7) / 8;
-    const dim3 grid(8 * per_xcd), block(256);
+    const dim3 grid(8 * per_xcd), block(bs);
     switch (riemann) {
     case 0: hipLaunchKernelGGL((flux_batch_kernel<0>), grid, block, 0, stream, cases, n_cases, st, total_blocks, ph, per_xcd); break;
     case 1: hipLaunchKernelGGL((flux_batch_kernel<1>), grid, block, 0, stream, cases, n_cases, st, total_blocks, ph, per_xcd); break;
@@ -1090,18 +1136,20 @@ hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks,
     return hipGetLastError();
 }
 
-hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const Physics& ph,
+hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const Physics& ph, uint32_t bs,
                           hipStream_t stream) {
+    if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
     const uint32_t per_xcd = (total_blocks + 7) / 8;
-    hipLaunchKernelGGL(spectral_batch_kernel, dim3(8 * per_xcd), dim3(256), 0, stream, cases, n_cases, total_blocks,
+    hipLaunchKernelGGL(spectral_batch_kernel, dim3(8 * per_xcd), dim3(bs), 0, stream, cases, n_cases, total_blocks,
                        ph, per_xcd);
     return hipGetLastError();
 }
 
 hipError_t finalize_batch(const CaseBufs* cases, int n_cases, double cfl, double fixed_dt, int64_t hist_cap,
-                          hipStream_t stream) {
+                          uint32_t bs, hipStream_t stream) {
+    if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
     hipLaunchKernelGGL(finalize_batch_kernel, dim3(n_cases), dim3(256), 0, stream, cases, n_cases, cfl, fixed_dt,
-                       hist_cap);
+                       hist_cap, bs);
     return hipGetLastError();
 }
 

@@ -6,7 +6,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <iostream>
+#include <map>
+#include <mutex>
 #include <sstream>
 
 #include "expr.hpp"
@@ -416,8 +419,7 @@ void Solver::upload() {
     dt_dev_ = dz(1);
     dt_hist_ = dz(kDtHistCap);
     dt_hist_cap_ = kDtHistCap;
-    lam_blk_n_ = (N + 255) / 256;
-    lam_blk_ = (unsigned long long*)dz(lam_blk_n_);
+    lam_blk_ = (unsigned long long*)dz((N + 63) / 64);
     step_dev_ = (int64_t*)dz(1);
     gated_slots_ = 4 * (size_t)((N + 63) / 64);
     gated_ = (uint32_t*)dz((gated_slots_ + 1) / 2);
@@ -474,6 +476,7 @@ void Solver::set_indicator(const float* params, double threshold) {
     if (!params) {
         gate_ = false;
         drop_graph();
+        tuned_ = false;
         return;
     }
     if (!L_.teno) fail("the learned indicator gates TENO reconstruction only");
@@ -493,6 +496,7 @@ void Solver::set_indicator(const float* params, double threshold) {
     gate_thr_ = threshold;
     gate_ = true;
     drop_graph();  // the captured step carries the gate in its kernel arguments
+    tuned_ = false;  // the gated kernel has its own best shape
 }
 
 uint64_t Solver::gated_cells(bool reset) {
@@ -602,12 +606,12 @@ void Solver::mark(int tag) {
 }
 
 void Solver::launch_teno(const double* U) {
-    HIP_OK(dev::teno_faces(geom(), teno(), U, fs_, dec_, ph_, stream_));
+    HIP_OK(dev::teno_faces(geom(), teno(), U, fs_, dec_, ph_, tune_.teno_occ, stream_));
     mark(kStatTenoMs);
 }
 
 void Solver::launch_flux(int op, const double* Uin, double* Uout) {
-    HIP_OK(dev::flux_update(geom(), stage(op, Uin, Uout), ph_, stream_));
+    HIP_OK(dev::flux_update(geom(), stage(op, Uin, Uout), ph_, tune_.flux_block, stream_));
     mark(kStatFluxMs);
 }
 
@@ -618,8 +622,9 @@ void Solver::stage_pass(int op, const double* Uin, double* Uout) {
 
 void Solver::dt_pass(double cfl, double fixed_dt) {
     if (fixed_dt <= 0.0)
-        HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_blk_, stream_));
-    HIP_OK(dev::finalize_dt(lam_blk_, lam_blk_n_, cfl, fixed_dt, dt_dev_, dt_hist_, dt_hist_cap_, step_dev_, stream_));
+        HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_blk_, tune_.spec_block, stream_));
+    HIP_OK(dev::finalize_dt(lam_blk_, spec_blocks(), cfl, fixed_dt, dt_dev_, dt_hist_, dt_hist_cap_, step_dev_,
+                            stream_));
     mark(kStatDtMs);
 }
 
@@ -713,9 +718,110 @@ void Solver::device_matrices(uint32_t cell, double* aplus, double* ta) const {
     }
 }
 
+void Solver::set_tune(const dev::Tune& t) {
+    auto bs_ok = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
+    if (!bs_ok(t.flux_block) || !bs_ok(t.spec_block)) fail("block sizes must be 64, 128 or 256 cells");
+    if (t.teno_occ != 0 && t.teno_occ != 5 && t.teno_occ != 6) fail("TENO occupancy target must be 0, 5 or 6");
+    tune_ = t;
+    tuned_ = true;
+    drop_graph();
+}
+
+// Times each candidate shape on the current state: launches that only write
+// scratch outputs (face states, the first stage's U and accumulator, the
+// radii) a few times each, keeping the fastest. The gated-cell counters are
+// restored afterwards.
+void Solver::autotune() {
+    static std::mutex mu;
+    static std::map<std::vector<int64_t>, dev::Tune> cache;
+    const int fq = (int)(L_.S * L_.Q);
+    const std::vector<int64_t> key = {(int64_t)L_.N, L_.K, L_.M, fq, loc8_ ? 1 : 0, gate_ ? 1 : 0,
+                                      (int64_t)cfg_.riemann, L_.teno ? 1 : 0};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            set_tune(it->second);
+            return;
+        }
+    }
+    drop_graph();
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    auto timed = [&](const std::function<void()>& f) {
+        f();  // warm
+        HIP_OK(hipEventRecord(e0, stream_));
+        for (int r = 0; r < 3; ++r) f();
+        HIP_OK(hipEventRecord(e1, stream_));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        return ms;
+    };
+    dev::Tune best = tune_;
+    const bool prof = prof_on_;
+    prof_on_ = false;
+    // spectral radius block
+    float bt = 1e30f;
+    for (uint32_t bs : {64u, 128u, 256u}) {
+        const float t = timed([&] {
+            HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_blk_, bs, stream_));
+        });
+        if (t < bt) { bt = t; best.spec_block = bs; }
+    }
+    tune_.spec_block = best.spec_block;
+    // a valid dt for the stage kernels (not a step: no history, no counter)
+    HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_blk_, tune_.spec_block, stream_));
+    HIP_OK(dev::finalize_dt(lam_blk_, spec_blocks(), cfg_.cfl, cfg_.use_cfl ? 0.0 : cfg_.dt, dt_dev_, nullptr, 1,
+                            nullptr, stream_));
+    if (L_.teno) {
+        uint32_t* saved = nullptr;
+        if (gate_) {
+            HIP_OK(hipMalloc((void**)&saved, gated_slots_ * sizeof(uint32_t)));
+            HIP_OK(hipMemcpyAsync(saved, gated_, gated_slots_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
+        }
+        const bool occ_tunable = L_.K == 6 && L_.M == 12 && fq == 3 && loc8_ != nullptr;
+        bt = 1e30f;
+        for (int occ : {0, 5, 6}) {
+            if (occ != 0 && !occ_tunable) continue;
+            const float t = timed([&] {
+                HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, occ, stream_));
+            });
+            if (t < bt) { bt = t; best.teno_occ = occ; }
+        }
+        tune_.teno_occ = best.teno_occ;
+        HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, tune_.teno_occ, stream_));
+        if (saved) {
+            HIP_OK(hipMemcpyAsync(gated_, saved, gated_slots_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
+            synchronize();
+            HIP_OK(hipFree(saved));
+        }
+    }
+    // a first stage (never the final one, which would rewrite the primitives
+    // the next dt is computed from); FE's only stage is timed as the residual
+    const int op = cfg_.integrator == Integrator::FE ? dev::kRhsOnly
+                   : cfg_.integrator == Integrator::RK4 ? dev::kRK4_1 : dev::kSSP1;
+    bt = 1e30f;
+    for (uint32_t bs : {64u, 128u, 256u}) {
+        const float t = timed([&] {
+            HIP_OK(dev::flux_update(geom(), stage(op, Un_, Ua_), ph_, bs, stream_));
+        });
+        if (t < bt) { bt = t; best.flux_block = bs; }
+    }
+    prof_on_ = prof;
+    HIP_OK(hipEventDestroy(e0));
+    HIP_OK(hipEventDestroy(e1));
+    synchronize();
+    set_tune(best);
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = best;
+}
+
 void Solver::run_steps(int64_t n, double* dts_out) {
     if (n <= 0) return;
     if (cfg_.use_cfl && cfg_.cfl < 0.0) fail("dt negative: " + std::to_string(cfg_.cfl) + ".");
+    ensure_tuned();
     int64_t s0 = 0;
     HIP_OK(hipMemcpyAsync(&s0, step_dev_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
     synchronize();
@@ -732,6 +838,7 @@ void Solver::run_steps_profiled(int64_t n, double* stats) {
     for (int k = 0; k < kStatCount; ++k) stats[k] = 0.0;
     if (n <= 0) return;
     if (cfg_.use_cfl && cfg_.cfl < 0.0) fail("dt negative: " + std::to_string(cfg_.cfl) + ".");
+    ensure_tuned();
     int64_t s0 = 0;
     HIP_OK(hipMemcpyAsync(&s0, step_dev_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
     synchronize();
@@ -781,10 +888,10 @@ void Solver::finish_steps(int64_t s0, int64_t n, double* dts_out) {
 
 double Solver::calc_dt() {
     if (cfg_.use_cfl) {
-        HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_blk_, stream_));
-        HIP_OK(dev::finalize_dt(lam_blk_, lam_blk_n_, cfg_.cfl, 0.0, dt_dev_, nullptr, 1, nullptr, stream_));
+        HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_blk_, tune_.spec_block, stream_));
+        HIP_OK(dev::finalize_dt(lam_blk_, spec_blocks(), cfg_.cfl, 0.0, dt_dev_, nullptr, 1, nullptr, stream_));
     } else {
-        HIP_OK(dev::finalize_dt(lam_blk_, lam_blk_n_, 0.0, cfg_.dt, dt_dev_, nullptr, 1, nullptr, stream_));
+        HIP_OK(dev::finalize_dt(lam_blk_, spec_blocks(), 0.0, cfg_.dt, dt_dev_, nullptr, 1, nullptr, stream_));
     }
     HIP_OK(hipMemcpyAsync(&dt_, dt_dev_, sizeof(double), hipMemcpyDeviceToHost, stream_));
     synchronize();

@@ -141,6 +141,14 @@ public:
     int tile_pos_bytes() const { return loc8_ ? 1 : 2; }
     double* d_fs() { return fs_; }
 
+    // Launch shapes (results are identical for every choice). autotune()
+    // times the candidates on the current state and keeps the fastest,
+    // cached per mesh size and numerics for the process; it runs by itself
+    // before the first steps unless set_tune() pinned a choice.
+    void autotune();
+    void set_tune(const dev::Tune& t);
+    const dev::Tune& tune() const { return tune_; }
+
 private:
     void setup(Mesh&& mesh);
     void upload();
@@ -179,8 +187,13 @@ private:
     double *ta_ = nullptr, *A_ = nullptr, *fbasis_ = nullptr;
     double *Un_ = nullptr, *Ua_ = nullptr, *Ub_ = nullptr, *W_ = nullptr, *prims_ = nullptr, *h_ = nullptr;
     double *fs_ = nullptr, *rhs_ = nullptr, *lam_cell_ = nullptr, *dt_dev_ = nullptr, *dt_hist_ = nullptr;
-    unsigned long long* lam_blk_ = nullptr;  // per-block spectral maxima
-    uint32_t lam_blk_n_ = 0;
+    unsigned long long* lam_blk_ = nullptr;  // per-block spectral maxima (room for 64-cell blocks)
+    uint32_t spec_blocks() const { return (L_.N + tune_.spec_block - 1) / tune_.spec_block; }
+    dev::Tune tune_{};
+    bool tuned_ = false;
+    void ensure_tuned() {
+        if (!tuned_) autotune();
+    }
     int64_t* step_dev_ = nullptr;
     uint32_t* gated_ = nullptr;  // per-wave-slot smooth counts of the gated TENO kernel
     size_t gated_slots_ = 0;
