@@ -42,7 +42,7 @@ enum { MLR_RS_RUSANOV = 0, MLR_RS_HLL = 1, MLR_RS_HLLC = 2, MLR_RS_ROE = 3 };
 enum { MLR_TI_FE = 0, MLR_TI_RK4 = 1, MLR_TI_SSPRK3 = 2 };
 /* Boundary conditions (boundary.h:22-37). */
 enum { MLR_BC_SYMMETRY = 0, MLR_BC_EXTRAPOLATION = 1, MLR_BC_WALL_ADIABATIC = 2, MLR_BC_UPT = 3,
-       MLR_BC_P_OUT = 4 };
+       MLR_BC_P_OUT = 4, MLR_BC_WALL_NOSLIP = 5 /* adiabatic no-slip wall; no reference counterpart */ };
 
 /* One [[boundaries]] entry (solver.cpp:188-237): zone name, type and the
  * parameters UPT (u, p, T) and P_OUT (p) read. */
@@ -67,6 +67,12 @@ typedef struct {
     int32_t n_bc;
     mlr_bc_desc bc[14];
     int32_t n_threads; /* host precompute threads, 0 = default */
+    /* Navier-Stokes (no reference counterpart; the reference solves Euler):
+     * viscous != 0 adds the viscous fluxes with Sutherland viscosity
+     * mu = mu_ref (T/T_mu_ref)^1.5 (T_mu_ref + S)/(T + S) (constant mu_ref when
+     * sutherland_S < 0) and heat conductivity mu cp / prandtl. */
+    int32_t viscous;
+    double mu_ref, T_mu_ref, sutherland_S, prandtl;
 } mlr_numerics_desc;
 
 /* Structured generators of the reference (mesh.cpp:305-848). jitter moves

@@ -21,7 +21,7 @@ BASIS = {"monomial": 0, "legendre": 1}
 TENO_VARIANT = {"reference": 0, "corrected": 1}
 RIEMANN = {"Rusanov": 0, "HLL": 1, "HLLC": 2, "Roe": 3}
 INTEGRATOR = {"FE": 0, "RK4": 1, "SSPRK3": 2}
-BC = {"symmetry": 0, "extrapolation": 1, "wall_adiabatic": 2, "upt": 3, "p_out": 4}
+BC = {"symmetry": 0, "extrapolation": 1, "wall_adiabatic": 2, "upt": 3, "p_out": 4, "wall_noslip": 5}
 
 
 class MlrBC(C.Structure):
@@ -37,7 +37,9 @@ class MlrNumerics(C.Structure):
                 ("gamma", C.c_double), ("p_ref", C.c_double), ("T_ref", C.c_double),
                 ("rho_ref", C.c_double), ("p_min", C.c_double), ("p_max", C.c_double),
                 ("cfl", C.c_double), ("dt", C.c_double), ("n_bc", C.c_int32),
-                ("bc", MlrBC * 14), ("n_threads", C.c_int32)]
+                ("bc", MlrBC * 14), ("n_threads", C.c_int32),
+                ("viscous", C.c_int32), ("mu_ref", C.c_double), ("T_mu_ref", C.c_double),
+                ("sutherland_S", C.c_double), ("prandtl", C.c_double)]
 
 
 class MlrMeshGen(C.Structure):
@@ -156,10 +158,14 @@ class MallardError(RuntimeError):
 def numerics(recon="TENO", basis="legendre", order=2, stencil_factor=2.0, quad_order_cell=0,
              quad_order_face=0, teno_variant="reference", riemann="HLLC", integrator="SSPRK3",
              gamma=1.4, p_ref=101325.0, T_ref=298.15, rho_ref=1.225, p_min=-1e20, p_max=1e20,
-             cfl=0.4, dt=0.0, bcs=None, n_threads=0):
+             cfl=0.4, dt=0.0, bcs=None, n_threads=0, physics="euler", mu_ref=0.0, T_mu_ref=0.0,
+             sutherland_S=-1.0, prandtl=0.72):
     """Numerics descriptor with the reference's defaults and key strings.
 
     bcs: list of (zone, type, params) with the [[boundaries]] keys.
+    physics: "euler" (the reference's) or "navier-stokes" (Sutherland
+    viscosity mu_ref (T/T_mu_ref)^1.5 (T_mu_ref+S)/(T+S), constant for S < 0;
+    T_mu_ref <= 0 takes T_ref).
     """
     d = MlrNumerics()
     d.recon, d.basis, d.order = RECON[recon], BASIS[basis], order
@@ -180,6 +186,10 @@ def numerics(recon="TENO", basis="legendre", order=2, stencil_factor=2.0, quad_o
         b.u[0], b.u[1] = u
         b.p, b.T = par.get("p", 0.0), par.get("T", 0.0)
     d.n_threads = n_threads
+    if physics not in ("euler", "navier-stokes"):
+        raise ValueError("Unknown physics type: " + physics)
+    d.viscous = 1 if physics == "navier-stokes" else 0
+    d.mu_ref, d.T_mu_ref, d.sutherland_S, d.prandtl = mu_ref, T_mu_ref, sutherland_S, prandtl
     return d
 
 

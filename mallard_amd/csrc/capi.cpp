@@ -84,7 +84,7 @@ mlr::Config config_from_desc(const mlr_numerics_desc* d) {
     for (int b = 0; b < d->n_bc; ++b) {
         BoundarySpec s;
         s.zone = std::string(d->bc[b].zone, strnlen(d->bc[b].zone, sizeof(d->bc[b].zone)));
-        if (d->bc[b].type < 0 || d->bc[b].type > 4) fail("Unknown boundary type.");
+        if (d->bc[b].type < 0 || d->bc[b].type > 5) fail("Unknown boundary type.");
         s.type = (BCType)d->bc[b].type;
         s.u[0] = d->bc[b].u[0];
         s.u[1] = d->bc[b].u[1];
@@ -93,6 +93,14 @@ mlr::Config config_from_desc(const mlr_numerics_desc* d) {
         c.bcs.push_back(s);
     }
     c.n_threads = d->n_threads;
+    c.viscous = d->viscous != 0;
+    if (c.viscous) {
+        c.mu_ref = d->mu_ref;
+        c.T_mu = d->T_mu_ref > 0.0 ? d->T_mu_ref : c.T_ref;
+        c.S_mu = d->sutherland_S;
+        c.Pr = d->prandtl > 0.0 ? d->prandtl : 0.72;
+        if (!(c.mu_ref >= 0.0)) fail("Invalid viscosity parameters.");
+    }
     c.n_steps = 0;
     return c;
 }
@@ -140,6 +148,11 @@ void desc_from_config(const mlr::Config& c, mlr_mesh_gen* mesh, mlr_numerics_des
             d->bc[b].T = c.bcs[b].T;
         }
         d->n_threads = c.n_threads;
+        d->viscous = c.viscous ? 1 : 0;
+        d->mu_ref = c.mu_ref;
+        d->T_mu_ref = c.T_mu;
+        d->sutherland_S = c.S_mu;
+        d->prandtl = c.Pr;
     }
 }
 

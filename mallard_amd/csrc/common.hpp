@@ -18,7 +18,8 @@ enum class Recon { FO, TENO };                                            // fac
 enum class BasisType { Monomial, Legendre };                              // basis.h:24-32
 enum class Riemann { Rusanov = 0, HLL = 1, HLLC = 2, Roe = 3 };           // riemann_solver.h:21-31 (+Roe)
 enum class Integrator { FE, RK4, SSPRK3 };                                // time_integrator.h:29-33
-enum class BCType { Symmetry = 0, Extrapolation = 1, WallAdiabatic = 2, UPT = 3, POut = 4 };  // boundary.h:22-37
+// boundary.h:22-37; WallNoSlip (adiabatic no-slip wall, viscous runs) has no reference counterpart
+enum class BCType { Symmetry = 0, Extrapolation = 1, WallAdiabatic = 2, UPT = 3, POut = 4, WallNoSlip = 5 };
 enum class TenoVariant { Reference = 0, Corrected = 1 };
 
 struct Error : std::runtime_error {

@@ -10,6 +10,10 @@ namespace mlr::dev {
 
 struct Physics {
     double gamma, R, cp, cv, p_min, p_max;
+    // Navier-Stokes: viscous != 0 adds the viscous fluxes; Sutherland law
+    // mu = mu_ref (T/T_mu)^1.5 (T_mu + S)/(T + S) (constant mu_ref when S < 0)
+    int32_t viscous;
+    double mu_ref, T_mu, S_mu, Pr;
 };
 
 struct BCParam {
@@ -29,6 +33,10 @@ struct Geom {
     const double* dxn;
     const double* qw;     // face quadrature weights (device copy, Q entries)
     const BCParam* bc;    // boundary conditions in input order (device copy)
+    const double* dcx;    // [S][N] centroid to neighbour centroid, or to the face
+    const double* dcy;    //        midpoint on boundaries (viscous runs only)
+    const double* mfx;    // [S][N] centroid to face midpoint (viscous runs only)
+    const double* mfy;
 };
 
 namespace ind {
@@ -97,6 +105,8 @@ struct StageArgs {
     double* rhs;          // kRhsOnly output
     const double* fs;     // face states [S*Q][N][4] (TENO)
     const double* dt;     // device scalar
+    const double* vp;     // viscous runs: (u, v, T, mu) [N][4] and the
+    const double* vg;     // cell gradients (u_x, u_y, v_x, v_y, T_x, T_y) [N][6] of Uin
 };
 
 // One case of a batched ensemble: its geometry, stencils and buffers, and the
@@ -139,6 +149,10 @@ hipError_t spectral_radius(const Geom& g, const double* Un, const double* prims,
 hipError_t finalize_dt(const unsigned long long* lam_blk, uint32_t n_blk, double cfl, double fixed_dt, double* dt,
                        double* dt_hist, int64_t hist_cap, int64_t* step, hipStream_t stream);
 hipError_t copy_state(const double* src, double* dst, uint32_t N, hipStream_t stream);
+// viscous runs: per cell (u, v, T, mu) and least-squares gradients of u, v, T
+// of the state U (bs cells per block)
+hipError_t viscous_grad(const Geom& g, const double* U, const Physics& ph, double* vp, double* vg, uint32_t bs,
+                        hipStream_t stream);
 // Ensemble launchers: `cases` is a device array of n_cases descriptors whose
 // blk_* offsets are prefix sums (flux and spectral blocks of bs cells, TENO
 // blocks of 64); every case has the same K, M, face points, tile-position

@@ -18,6 +18,7 @@ void hip_ok(hipError_t e, const char* what) {
 Ensemble::Ensemble(const Config& base, const std::vector<MeshSpec>& meshes) : cfg_(base) {
     if (meshes.empty()) fail("an ensemble needs at least one case");
     if (cfg_.integrator == Integrator::FE) fail("ensembles run the RK4 and SSPRK3 integrators");
+    if (cfg_.viscous) fail("ensembles run the Euler equations");
     cases_.reserve(meshes.size());
     for (const MeshSpec& ms : meshes) {
         Config c = cfg_;

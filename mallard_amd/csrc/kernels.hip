@@ -237,7 +237,7 @@ __device__ inline void point_flux(const Physics& ph, const BCParam* __restrict__
         } else if (b.type == 1) {  // extrapolation
             for (int i = 0; i < 5; i++) Wr[i] = Wl[i];
             rr = Cl[0];
-        } else if (b.type == 2) {  // adiabatic slip wall
+        } else if (b.type == 2 || b.type == 5) {  // adiabatic wall (slip; no-slip adds viscous terms)
             F[0] = 0.0; F[1] = Wl[2] * n0; F[2] = Wl[2] * n1; F[3] = 0.0;
             return;
         } else if (b.type == 3) {  // fixed u, p, T (boundary_upt.cpp:83-100)
@@ -644,6 +644,162 @@ void teno_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, int in_r
                                        reinterpret_cast<double2*>(teno_smem), teno_smem + TILE * 32);
 }
 
+// ------------------------------------------------------------ viscous terms
+// Navier-Stokes extension (no reference counterpart). Sutherland viscosity.
+__device__ inline double viscosity(const Physics& ph, double T) {
+    if (ph.S_mu < 0.0) return ph.mu_ref;
+    const double t = T / ph.T_mu;
+    return ph.mu_ref * (t * sqrt(t)) * ((ph.T_mu + ph.S_mu) / (T + ph.S_mu));
+}
+
+// Values of (u, v, T) a boundary face holds for the gradients: no-slip wall
+// (u, v) = 0; slip wall and symmetry the tangential velocity; fixed (u, p, T)
+// its values; extrapolation and p_out the cell's own (no contribution).
+__device__ inline void boundary_uvT(const BCParam& b, double n0, double n1, const double* c, double* f) {
+    if (b.type == 5) {
+        f[0] = 0.0; f[1] = 0.0; f[2] = c[2];
+    } else if (b.type == 0 || b.type == 2) {
+        const double un = c[0] * n0 + c[1] * n1;
+        f[0] = c[0] - un * n0; f[1] = c[1] - un * n1; f[2] = c[2];
+    } else if (b.type == 3) {
+        f[0] = b.u0; f[1] = b.u1; f[2] = b.T;
+    } else {
+        f[0] = c[0]; f[1] = c[1]; f[2] = c[2];
+    }
+}
+
+// One thread per cell: (u, v, T, mu) of the state and the weighted
+// least-squares gradients of u, v and T over the face neighbours (weights
+// 1/|d|^2, d from the centroid to the neighbour's centroid or to the boundary
+// face's midpoint), exact for linear fields.
+__global__ __launch_bounds__(256) void viscous_grad_kernel(Geom g, const double* __restrict__ U, Physics ph,
+                                                           double* __restrict__ vp, double* __restrict__ vg,
+                                                           uint32_t blocks_per_xcd) {
+    const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
+    const uint32_t c = lb * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    double Wc[5];
+    const D4 uc = ld4(U, c);
+    prims(ph, uc.v, Wc);
+    const double pc[3] = {Wc[0], Wc[1], Wc[3]};
+    double a11 = 0.0, a12 = 0.0, a22 = 0.0, b[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (uint32_t l = 0; l < g.S; ++l) {
+        const size_t o = (size_t)l * g.N + c;
+        const uint32_t inf = g.info[o];
+        const double dx = g.dcx[o], dy = g.dcy[o];
+        double pf[3];
+        if (!(inf & kBoundaryBit)) {
+            double Wn[5];
+            const D4 un = ld4(U, (uint32_t)g.nb[o]);
+            prims(ph, un.v, Wn);
+            pf[0] = Wn[0]; pf[1] = Wn[1]; pf[2] = Wn[3];
+        } else {
+            boundary_uvT(g.bc[(inf >> 4) & 0xFu], g.nx[o], g.ny[o], pc, pf);
+        }
+        const double w = 1.0 / (dx * dx + dy * dy);
+        a11 += w * dx * dx;
+        a12 += w * dx * dy;
+        a22 += w * dy * dy;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double dp = w * (pf[k] - pc[k]);
+            b[k][0] += dx * dp;
+            b[k][1] += dy * dp;
+        }
+    }
+    const double id = 1.0 / (a11 * a22 - a12 * a12);
+    double gr[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        gr[2 * k] = (a22 * b[k][0] - a12 * b[k][1]) * id;
+        gr[2 * k + 1] = (a11 * b[k][1] - a12 * b[k][0]) * id;
+    }
+    st4(vp, c, D4{{pc[0], pc[1], pc[2], viscosity(ph, pc[2])}});
+    double2* G2 = reinterpret_cast<double2*>(vg + 6 * (size_t)c);
+    G2[0] = make_double2(gr[0], gr[1]);
+    G2[1] = make_double2(gr[2], gr[3]);
+    G2[2] = make_double2(gr[4], gr[5]);
+}
+
+// Viscous flux through a face, in the face's orientation (unit normal n from
+// its side-0 cell L to R): G = (0, tau n, (u tau + k grad T) n). Face
+// gradients are the mean of the two cells' gradients with the directional
+// derivative along d = x_R - x_L replaced by (phi_R - phi_L) / |d|; the face
+// velocity is the mean of both cells' linear extrapolations to the face
+// midpoint. On a boundary face the cell's gradient is corrected toward the
+// boundary value along the centroid-to-midpoint vector. All of it is exact
+// for linear fields, and both cells of a face evaluate the same operations on
+// the same values, so they agree bit for bit.
+__device__ inline void viscous_face(const Physics& ph, const Geom& g, const StageArgs& a, size_t o, uint32_t c,
+                                    uint32_t inf, bool bnd, uint32_t side, double n0, double n1, double* G) {
+    G[0] = 0.0; G[1] = 0.0; G[2] = 0.0; G[3] = 0.0;
+    int bt = -1;
+    if (bnd) {
+        bt = g.bc[(inf >> 4) & 0xFu].type;
+        if (bt == 0 || bt == 2) return;  // symmetry, slip wall: no viscous stress or heat flux
+    }
+    const uint32_t L = (!bnd && side) ? (uint32_t)g.nb[o] : c;
+    const D4 pL = ld4(a.vp, L);
+    const double2* gL2 = reinterpret_cast<const double2*>(a.vg + 6 * (size_t)L);
+    const double2 gL0 = gL2[0], gL1 = gL2[1], gL3 = gL2[2];
+    double dx = g.dcx[o], dy = g.dcy[o];
+    if (side) { dx = -dx; dy = -dy; }
+    const double il = 1.0 / sqrt(dx * dx + dy * dy);
+    const double tx = dx * il, ty = dy * il;
+    double gx[3], gy[3], uf, vf, mu;
+    const double phL[3] = {pL.v[0], pL.v[1], pL.v[2]};
+    if (!bnd) {
+        const uint32_t R = side ? c : (uint32_t)g.nb[o];
+        const D4 pR = ld4(a.vp, R);
+        const double2* gR2 = reinterpret_cast<const double2*>(a.vg + 6 * (size_t)R);
+        const double2 gR0 = gR2[0], gR1 = gR2[1], gR3 = gR2[2];
+        gx[0] = 0.5 * (gL0.x + gR0.x); gy[0] = 0.5 * (gL0.y + gR0.y);
+        gx[1] = 0.5 * (gL1.x + gR1.x); gy[1] = 0.5 * (gL1.y + gR1.y);
+        gx[2] = 0.5 * (gL3.x + gR3.x); gy[2] = 0.5 * (gL3.y + gR3.y);
+        const double phR[3] = {pR.v[0], pR.v[1], pR.v[2]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double corr = (phR[k] - phL[k]) * il - (gx[k] * tx + gy[k] * ty);
+            gx[k] += corr * tx;
+            gy[k] += corr * ty;
+        }
+        // centroid-to-midpoint vectors of L and R (own slot and the neighbour's slot)
+        const size_t on = (size_t)(inf & 3u) * g.N + (uint32_t)g.nb[o];
+        const size_t oL = side ? on : o, oR = side ? o : on;
+        const double mLx = g.mfx[oL], mLy = g.mfy[oL], mRx = g.mfx[oR], mRy = g.mfy[oR];
+        uf = 0.5 * ((pL.v[0] + (gL0.x * mLx + gL0.y * mLy)) + (pR.v[0] + (gR0.x * mRx + gR0.y * mRy)));
+        vf = 0.5 * ((pL.v[1] + (gL1.x * mLx + gL1.y * mLy)) + (pR.v[1] + (gR1.x * mRx + gR1.y * mRy)));
+        mu = 0.5 * (pL.v[3] + pR.v[3]);
+    } else {
+        gx[0] = gL0.x; gy[0] = gL0.y;
+        gx[1] = gL1.x; gy[1] = gL1.y;
+        gx[2] = gL3.x; gy[2] = gL3.y;
+        double pf[3];
+        boundary_uvT(g.bc[(inf >> 4) & 0xFu], n0, n1, phL, pf);
+        if (bt != 1 && bt != 4) {  // a prescribed value: correct toward it
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double corr = (pf[k] - phL[k]) * il - (gx[k] * tx + gy[k] * ty);
+                gx[k] += corr * tx;
+                gy[k] += corr * ty;
+            }
+        }
+        uf = pf[0];
+        vf = pf[1];
+        mu = pL.v[3];
+    }
+    const double div = gx[0] + gy[1];
+    const double txx = mu * (2.0 * gx[0] - (2.0 / 3.0) * div);
+    const double tyy = mu * (2.0 * gy[1] - (2.0 / 3.0) * div);
+    const double txy = mu * (gy[0] + gx[1]);
+    G[1] = txx * n0 + txy * n1;
+    G[2] = txy * n0 + tyy * n1;
+    // adiabatic no-slip wall: u = 0 and no heat flux
+    if (bt == 5) return;
+    const double k = mu * ph.cp / ph.Pr;
+    G[3] = uf * G[1] + vf * G[2] + k * (gx[2] * n0 + gy[2] * n1);
+}
+
 // ------------------------------------------------------------ flux + update
 // One thread per cell: the residual of the cell in the canonical face order,
 // divided by the volume, then the stage's SSP-RK update (no atomics).
@@ -691,6 +847,12 @@ __device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, con
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) F[i] *= 0.5;
+        if (ph.viscous) {
+            double G[4];
+            viscous_face(ph, g, a, o, c, inf, bnd, side, n0, n1, G);
+#pragma unroll
+            for (int i = 1; i < 4; i++) F[i] -= G[i];
+        }
 #pragma unroll
         for (int i = 0; i < 4; i++) r[i] += side ? ar * F[i] : -ar * F[i];
     }
@@ -836,6 +998,16 @@ __device__ __forceinline__ void spectral_block(const Geom& g, const double* __re
         conv *= 1.37 * gf;
         acou = 1.37 * sqrt(gf * acou);
         lam = conv + acou;
+        if (ph.viscous) {
+            // viscous radius max(4/3, gamma/Pr) (mu / rho) sum_f (A_f / V)^2
+            const double V = g.volume[c];
+            double s2 = 0.0;
+            for (uint32_t l = 0; l < S; ++l) {
+                const double q = g.area[(size_t)l * N + c] / V;
+                s2 += q * q;
+            }
+            lam += fmax(4.0 / 3.0, ph.gamma / ph.Pr) * (viscosity(ph, wc.v[3]) / rc) * s2;
+        }
         lam_cell[c] = lam;
         ok = lam == lam;  // NaN radii are ignored by the reference's max join
     }
@@ -1150,6 +1322,14 @@ hipError_t finalize_batch(const CaseBufs* cases, int n_cases, double cfl, double
     if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
     hipLaunchKernelGGL(finalize_batch_kernel, dim3(n_cases), dim3(256), 0, stream, cases, n_cases, cfl, fixed_dt,
                        hist_cap, bs);
+    return hipGetLastError();
+}
+
+hipError_t viscous_grad(const Geom& g, const double* U, const Physics& ph, double* vp, double* vg, uint32_t bs,
+                        hipStream_t stream) {
+    if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
+    const uint32_t nblk = (g.N + bs - 1) / bs, per_xcd = (nblk + 7) / 8;
+    hipLaunchKernelGGL(viscous_grad_kernel, dim3(8 * per_xcd), dim3(bs), 0, stream, g, U, ph, vp, vg, per_xcd);
     return hipGetLastError();
 }
 

@@ -542,6 +542,10 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
     L.ny.assign((size_t)S * N, 0.0);
     L.area.assign((size_t)S * N, 0.0);
     L.dxn.assign((size_t)S * N, 0.0);
+    L.dcx.assign((size_t)S * N, 0.0);
+    L.dcy.assign((size_t)S * N, 0.0);
+    L.mfx.assign((size_t)S * N, 0.0);
+    L.mfy.assign((size_t)S * N, 0.0);
     L.face_slot.assign(m.n_faces, ~0u);
     bool bad = false;
 #pragma omp parallel for schedule(static)
@@ -588,6 +592,20 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
                 s1 = m.cc(c1, 1) - m.cc(c0, 1);
             }
             L.dxn[o] = std::fabs(s0 * u0 + s1 * u1);
+            // viscous face gradients: the vector the directional correction
+            // runs along, and the one to the face midpoint
+            {
+                const uint32_t a = m.node_of_face(f, 0), b = m.node_of_face(f, 1);
+                L.mfx[o] = 0.5 * (m.nc(a, 0) + m.nc(b, 0)) - m.cc(c, 0);
+                L.mfy[o] = 0.5 * (m.nc(a, 1) + m.nc(b, 1)) - m.cc(c, 1);
+            }
+            if (bnd) {
+                L.dcx[o] = L.mfx[o];
+                L.dcy[o] = L.mfy[o];
+            } else {
+                L.dcx[o] = m.cc(other, 0) - m.cc(c, 0);
+                L.dcy[o] = m.cc(other, 1) - m.cc(c, 1);
+            }
             keys[l] = key[f];
         }
         // accumulation order: slots by face key, contributing ones first

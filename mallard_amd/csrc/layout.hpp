@@ -56,6 +56,8 @@ struct Layout {
     std::vector<int32_t> nb;              // [S][N] neighbour (new id), self for boundary slots
     std::vector<uint32_t> info;           // [S][N]
     std::vector<double> nx, ny, area, dxn;  // [S][N]
+    std::vector<double> dcx, dcy;           // [S][N] centroid to neighbour centroid (face midpoint on boundaries)
+    std::vector<double> mfx, mfy;           // [S][N] centroid to face midpoint
 
     // TENO (empty for FO): G = S + 1 stencils per cell, K = n_dof, M = cells per stencil.
     bool teno = false;

@@ -73,16 +73,25 @@ Config config_from_toml(const toml::Value& in) {
 
     // [physics] (solver.cpp:88-108, physics.cpp:27-53)
     const std::string pt = find_or_str(in, {"physics", "type"}, "euler");
-    if (pt != "euler") fail("Unknown physics type: " + pt + ".");
+    if (pt != "euler" && pt != "navier-stokes") fail("Unknown physics type: " + pt + ".");
     const Value& ph = find(in, "physics");
     for (const char* k : {"gamma", "p_ref", "T_ref", "rho_ref"})
-        if (!ph.contains(k)) fail(std::string("Missing ") + k + " for physics: euler.");
+        if (!ph.contains(k)) fail(std::string("Missing ") + k + " for physics: " + pt + ".");
     c.gamma = find(ph, "gamma").as_float();
     c.p_ref = find(ph, "p_ref").as_float();
     c.T_ref = find(ph, "T_ref").as_float();
     c.rho_ref = find(ph, "rho_ref").as_float();
     c.p_min = find_or(in, {"physics", "p_min"}, -1e20);
     c.p_max = find_or(in, {"physics", "p_max"}, 1e20);
+    if (pt == "navier-stokes") {
+        if (!ph.contains("mu_ref")) fail("Missing mu_ref for physics: navier-stokes.");
+        c.viscous = true;
+        c.mu_ref = find(ph, "mu_ref").as_float();
+        c.T_mu = find_or(in, {"physics", "T_mu_ref"}, c.T_ref);
+        c.S_mu = find_or(in, {"physics", "sutherland_S"}, -1.0);
+        c.Pr = find_or(in, {"physics", "Pr"}, 0.72);
+        if (!(c.mu_ref >= 0.0) || !(c.T_mu > 0.0) || !(c.Pr > 0.0)) fail("Invalid viscosity parameters.");
+    }
 
     // [numerics] (solver.cpp:110-186, face_reconstruction.cpp:109-152)
     const Value& fr = find(in, "numerics", "face_reconstruction");
@@ -126,7 +135,7 @@ Config config_from_toml(const toml::Value& in) {
         const std::string ty = find(b, "type").as_string();
         s.type = lookup_enum<BCType>(ty, {{"symmetry", BCType::Symmetry}, {"extrapolation", BCType::Extrapolation},
                                           {"wall_adiabatic", BCType::WallAdiabatic}, {"upt", BCType::UPT},
-                                          {"p_out", BCType::POut}},
+                                          {"p_out", BCType::POut}, {"wall_noslip", BCType::WallNoSlip}},
                                      "boundary type");
         if (s.type == BCType::UPT) {
             for (const char* k : {"u", "p", "T"})
@@ -280,6 +289,11 @@ void Solver::setup(Mesh&& mesh) {
     ph_.cv = ph_.cp / cfg_.gamma;
     ph_.p_min = cfg_.p_min;
     ph_.p_max = cfg_.p_max;
+    ph_.viscous = cfg_.viscous ? 1 : 0;
+    ph_.mu_ref = cfg_.mu_ref;
+    ph_.T_mu = cfg_.T_mu;
+    ph_.S_mu = cfg_.S_mu;
+    ph_.Pr = cfg_.Pr;
     if (cfg_.integrator != Integrator::FE && cfg_.integrator != Integrator::RK4 &&
         cfg_.integrator != Integrator::SSPRK3)
         fail("Unknown time integrator type.");
@@ -360,6 +374,12 @@ void Solver::upload() {
     up(area_, L_.area);
     up(dxn_, L_.dxn);
     up(qw_, L_.qface_w);
+    if (cfg_.viscous) {
+        up(dcx_, L_.dcx);
+        up(dcy_, L_.dcy);
+        up(mfx_, L_.mfx);
+        up(mfy_, L_.mfy);
+    }
     std::vector<dev::BCParam> bct(std::max<size_t>(cfg_.bcs.size(), 1));
     for (size_t b = 0; b < cfg_.bcs.size(); ++b) {
         const BoundarySpec& s = cfg_.bcs[b];
@@ -409,6 +429,10 @@ void Solver::upload() {
     };
     Un_ = dz(4 * N);
     Ua_ = dz(4 * N);
+    if (cfg_.viscous) {
+        vp_ = dz(4 * N);
+        vg_ = dz(6 * N);
+    }
     Ub_ = dz(4 * N);
     W_ = dz(4 * N);
     prims_ = dz(4 * N);
@@ -441,6 +465,10 @@ dev::Geom Solver::geom() const {
     g.dxn = dxn_;
     g.qw = qw_;
     g.bc = bc_;
+    g.dcx = dcx_;
+    g.dcy = dcy_;
+    g.mfx = mfx_;
+    g.mfy = mfy_;
     return g;
 }
 
@@ -590,6 +618,8 @@ dev::StageArgs Solver::stage(int op, const double* Uin, double* Uout) const {
     a.rhs = rhs_;
     a.fs = fs_;
     a.dt = dt_dev_;
+    a.vp = vp_;
+    a.vg = vg_;
     return a;
 }
 
@@ -611,6 +641,7 @@ void Solver::launch_teno(const double* U) {
 }
 
 void Solver::launch_flux(int op, const double* Uin, double* Uout) {
+    if (cfg_.viscous) HIP_OK(dev::viscous_grad(geom(), Uin, ph_, vp_, vg_, tune_.flux_block, stream_));
     HIP_OK(dev::flux_update(geom(), stage(op, Uin, Uout), ph_, tune_.flux_block, stream_));
     mark(kStatFluxMs);
 }
@@ -805,6 +836,7 @@ void Solver::autotune() {
     bt = 1e30f;
     for (uint32_t bs : {64u, 128u, 256u}) {
         const float t = timed([&] {
+            if (cfg_.viscous) HIP_OK(dev::viscous_grad(geom(), Un_, ph_, vp_, vg_, bs, stream_));
             HIP_OK(dev::flux_update(geom(), stage(op, Un_, Ua_), ph_, bs, stream_));
         });
         if (t < bt) { bt = t; best.flux_block = bs; }

@@ -46,6 +46,11 @@ struct Config {
     uint64_t seed = 0;
     // [physics]
     double gamma = 1.4, p_ref = 101325.0, T_ref = 298.15, rho_ref = 1.225, p_min = -1e20, p_max = 1e20;
+    // Navier-Stokes (no reference counterpart): Sutherland viscosity
+    // mu = mu_ref (T / T_mu)^1.5 (T_mu + S) / (T + S) (constant mu_ref when S < 0),
+    // heat conductivity mu cp / Pr
+    bool viscous = false;
+    double mu_ref = 0.0, T_mu = 1.0, S_mu = -1.0, Pr = 0.72;
     // [numerics]
     Recon recon = Recon::FO;
     TenoSpec teno;
@@ -177,6 +182,8 @@ private:
     std::vector<void*> allocs_;
     uint64_t dev_bytes_ = 0;
     double *volume_ = nullptr, *nx_ = nullptr, *ny_ = nullptr, *area_ = nullptr, *dxn_ = nullptr, *qw_ = nullptr;
+    // viscous runs: slot vectors, (u, v, T, mu) and cell gradients of the stage input
+    double *dcx_ = nullptr, *dcy_ = nullptr, *mfx_ = nullptr, *mfy_ = nullptr, *vp_ = nullptr, *vg_ = nullptr;
     int32_t* nb_ = nullptr;
     uint32_t* info_ = nullptr;
     dev::BCParam* bc_ = nullptr;
