@@ -42,6 +42,14 @@ class MlrNumerics(C.Structure):
                 ("sutherland_S", C.c_double), ("prandtl", C.c_double)]
 
 
+class MlrMeshDesc(C.Structure):
+    _fields_ = [("n_nodes", C.c_uint32), ("node_coords", C.c_void_p), ("n_cells", C.c_uint32),
+                ("offsets_nodes_of_cell", C.c_void_p), ("nodes_of_cell", C.c_void_p),
+                ("faces_of_cell", C.c_void_p), ("n_faces", C.c_uint32), ("nodes_of_face", C.c_void_p),
+                ("cells_of_face", C.c_void_p), ("n_zones", C.c_uint32), ("zone_names", C.c_void_p),
+                ("zone_offsets", C.c_void_p), ("zone_faces", C.c_void_p)]
+
+
 class MlrMeshGen(C.Structure):
     _fields_ = [("type", C.c_int32), ("nx", C.c_uint32), ("ny", C.c_uint32), ("Lx", C.c_double),
                 ("Ly", C.c_double), ("jitter", C.c_double), ("seed", C.c_uint64)]
@@ -273,11 +281,35 @@ class Layout:
 class Solver:
     """GPU solver context (Solver, solver.h:30-279) behind the C ABI."""
 
-    def __init__(self, toml_path=None, gen=None, num=None):
+    def __init__(self, toml_path=None, gen=None, num=None, mesh=None):
+        """From a TOML input, a generated mesh (gen) or mesh arrays in the
+        reference layout (mesh: the dictionary meshgen.mesh_from_triangles
+        returns), with the numerics descriptor num."""
         L = lib()
         h = C.c_void_p()
         if toml_path is not None:
             rc = L.mlr_ctx_create_toml(os.fspath(toml_path).encode(), C.byref(h))
+        elif mesh is not None:
+            keep = {k: np.ascontiguousarray(mesh[k], dt) for k, dt in (
+                ("node_coords", np.float64), ("offsets_nodes_of_cell", np.uint32), ("nodes_of_cell", np.uint32),
+                ("faces_of_cell", np.uint32), ("nodes_of_face", np.uint32), ("cells_of_face", np.int32),
+                ("zone_offsets", np.uint32), ("zone_faces", np.uint32))}
+            names = (C.c_char_p * len(mesh["zone_names"]))(*[z.encode() for z in mesh["zone_names"]])
+            d = MlrMeshDesc()
+            d.n_nodes = len(keep["node_coords"]) // 2 if keep["node_coords"].ndim == 1 else len(keep["node_coords"])
+            d.node_coords = _p(keep["node_coords"])
+            d.n_cells = len(keep["offsets_nodes_of_cell"]) - 1
+            d.offsets_nodes_of_cell = _p(keep["offsets_nodes_of_cell"])
+            d.nodes_of_cell = _p(keep["nodes_of_cell"])
+            d.faces_of_cell = _p(keep["faces_of_cell"])
+            d.n_faces = len(keep["nodes_of_face"]) // 2
+            d.nodes_of_face = _p(keep["nodes_of_face"])
+            d.cells_of_face = _p(keep["cells_of_face"])
+            d.n_zones = len(mesh["zone_names"])
+            d.zone_names = C.cast(names, C.c_void_p)
+            d.zone_offsets = _p(keep["zone_offsets"])
+            d.zone_faces = _p(keep["zone_faces"])
+            rc = L.mlr_ctx_create(C.byref(d), C.byref(num), C.byref(h))
         else:
             rc = L.mlr_ctx_create_generated(C.byref(gen), C.byref(num), C.byref(h))
         if rc:

@@ -152,3 +152,31 @@ def test_viscous_runs_steps_with_teno(mlr):
     low = cc[:, 1] < 1.0 / 32
     mid = np.abs(cc[:, 1] - 0.5) < 0.05
     assert u[low].mean() < u[mid].mean()
+
+
+def test_delaunay_wedge_viscous_roe(mlr):
+    """The viscous wedge problem on a small graded Delaunay mesh: TENO p=2 +
+    Roe + viscous terms, supersonic inflow, no-slip wedge surface."""
+    from mallard_amd import meshgen
+    mesh = meshgen.wedge_mesh(n_cells=30000, seed=2)
+    Tinf = 1.0 / GAMMA
+    bcs = [("left", "upt", {"u": (2.5, 0.0), "p": 1.0, "T": Tinf}), ("right", "extrapolation", {}),
+           ("top", "extrapolation", {}), ("symmetry", "symmetry", {}), ("wall", "wall_noslip", {})]
+    num = mlr.numerics(recon="TENO", order=2, teno_variant="corrected", riemann="Roe", cfl=0.4, bcs=bcs,
+                       physics="navier-stokes", mu_ref=1.4 * 2.5 / 1e4, T_mu_ref=Tinf,
+                       sutherland_S=110.4 / 288.15 * Tinf, prandtl=0.72, **PHYS)
+    s = mlr.Solver(mesh=mesh, num=num)
+    s.init_constant([2.5, 0.0], 1.0, Tinf)
+    s.run_steps(40)
+    U, W = s.state()
+    assert np.isfinite(U).all() and (U[:, 0] > 0).all() and (W[:, 2] > 0).all()
+    t, steps = s.time()
+    assert steps == 40 and t > 0
+    # the wedge compresses the flow: pressure rises in the cells on the wedge
+    # surface, and the no-slip wall slows them down
+    names, off, zf = mesh["zone_names"], mesh["zone_offsets"], mesh["zone_faces"]
+    k = names.index("wall")
+    wall_cells = np.unique(mesh["cells_of_face"].reshape(-1, 2)[zf[off[k]:off[k + 1]], 0])
+    print("wall cells: p mean %.4f, u mean %.4f" % (W[wall_cells, 2].mean(), W[wall_cells, 0].mean()))
+    assert W[wall_cells, 2].mean() > 1.05
+    assert W[wall_cells, 0].mean() < 2.5
