@@ -114,34 +114,51 @@ def learned_run(args, mlr, dev, U_plain, value_plain, wall_plain):
     compare the final state with the plain run's."""
     from mallard_amd import indicator as ind
     t = time.perf_counter()
-    ds, y = ind.build_dataset(n_cases=args.train_cases, nx=64, snapshots=(0, 8, 20), seed=args.seed)
-    params, hist = ind.train_indicator(ds, ind.make_pool(y, seed=args.seed), steps=3000, batch=4096, lr=2e-3,
-                                       seed=args.seed)
+    if args.indicator_file:
+        # parameters saved by an earlier run (--save-indicator): no dataset, no training
+        params = np.load(args.indicator_file, allow_pickle=False).astype(np.float32)
+        y, hist = np.zeros(0), np.zeros(50)
+        log(f"[bench] indicator parameters loaded from {args.indicator_file}")
+    else:
+        ds, y = ind.build_dataset(n_cases=args.train_cases, nx=64, snapshots=(0, 8, 20), seed=args.seed)
+        params, hist = ind.train_indicator(ds, ind.make_pool(y, seed=args.seed), steps=3000, batch=4096, lr=2e-3,
+                                           seed=args.seed)
+        del ds
+        log(f"[bench] indicator: {len(y)} rows ({int((y < 0.5).sum())} non-smooth), trained in "
+            f"{time.perf_counter() - t:.1f} s, final loss {hist[-50:].mean():.4f}")
+        if args.save_indicator:
+            np.save(args.save_indicator, params)
     train_s = time.perf_counter() - t
-    log(f"[bench] indicator: {len(y)} rows ({int((y < 0.5).sum())} non-smooth), trained in {train_s:.1f} s, "
-        f"final loss {hist[-50:].mean():.4f}")
-    del ds
     N = dev.n_cells
-    dev.set_indicator(params, args.threshold)
-    dev.init_quadrant(0.5, 0.5, QUADRANT)
-    tune = dev.autotune()  # the gated kernel's shapes, before the timed region
-    dev.gated(reset=True)
-    t = time.perf_counter()
-    st = dev.run_steps_profiled(args.steps)
-    wall = time.perf_counter() - t
-    gated = dev.gated(reset=True)
-    check_state(dev, "learned-indicator steps")
-    U = dev.state()[0]
-    l1 = ind.rel_l1(U, U_plain, dev.volumes())
+
+    def timed(every_stage):
+        dev.set_indicator(params, args.threshold, every_stage=every_stage)
+        dev.init_quadrant(0.5, 0.5, QUADRANT)
+        tune = dev.autotune()  # the gated kernel's shapes, before the timed region
+        dev.gated(reset=True)
+        t = time.perf_counter()
+        st = dev.run_steps_profiled(args.steps)
+        wall = time.perf_counter() - t
+        gated = dev.gated(reset=True)
+        check_state(dev, "learned-indicator steps")
+        l1 = ind.rel_l1(dev.state()[0], U_plain, dev.volumes())
+        value = N * 3 * args.steps / wall
+        log(f"[bench] learned ({'every stage' if every_stage else 'once per step'}): {wall:.3f} s wall, "
+            f"x{value / value_plain:.2f}, gated {gated / (N * 3 * args.steps):.3f}, rel L1 {l1}")
+        return {"value": value, "ms_per_step": wall / args.steps * 1e3, "speedup": value / value_plain,
+                "l1_rel": [float(x) for x in l1], "matched_l1": bool(np.all(l1 <= 1e-3)),
+                "gated_fraction": gated / (N * 3 * args.steps),
+                "teno_ms": st["teno_ms"] / max(st["teno_n"], 1), "tune": list(tune)}
+
+    out = timed(False)
+    out["network_cadence"] = "once per step (first stage), decisions reused by stages 2 and 3"
+    every = timed(True)
     dev.set_indicator(None)
-    value = N * 3 * args.steps / wall
-    log(f"[bench] learned: {wall:.3f} s wall, x{value / value_plain:.2f}, gated {gated / (N * 3 * args.steps):.3f}, "
-        f"rel L1 {l1}")
-    return {"value": value, "ms_per_step": wall / args.steps * 1e3, "speedup": value / value_plain,
-            "l1_rel": [float(x) for x in l1], "matched_l1": bool(np.all(l1 <= 1e-3)),
-            "gated_fraction": gated / (N * 3 * args.steps),
-            "teno_ms": st["teno_ms"] / max(st["teno_n"], 1), "train_s": train_s, "tune": list(tune),
-            "train_rows": int(len(y)), "train_loss": float(hist[-50:].mean()), "threshold": args.threshold}
+    out["every_stage"] = {k: every[k] for k in ("value", "ms_per_step", "speedup", "l1_rel", "gated_fraction",
+                                                "teno_ms")}
+    out.update({"train_s": train_s, "train_rows": int(len(y)), "train_loss": float(hist[-50:].mean()),
+                "threshold": args.threshold})
+    return out
 
 
 def ensemble_sizes(n_cases, seed):
@@ -321,6 +338,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-indicator run")
     ap.add_argument("--train-cases", type=int, default=512)
     ap.add_argument("--threshold", type=float, default=0.9)
+    ap.add_argument("--save-indicator", default=None, help="save the trained indicator parameters (.npy)")
+    ap.add_argument("--indicator-file", default=None,
+                    help="use saved indicator parameters instead of training (profiling runs)")
     ap.add_argument("--workload", default="riemann", choices=["riemann", "ensemble", "wedge"],
                     help="riemann: the 2048^2 problem (default); ensemble: the batched ensemble; "
                          "wedge: the viscous wedge")

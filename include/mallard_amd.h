@@ -303,6 +303,11 @@ int mlr_mlp_train(float* params, const mlr_dataset* ds, const uint32_t* pool, in
 /* gate the context's TENO with the network (params NULL: off); cells with
  * sigmoid(z) >= threshold take the central stencil alone */
 int mlr_set_indicator(mlr_ctx* ctx, const float* params, double threshold);
+/* every_stage = 0 (default): the network judges the cells once per step, on
+ * the step's first stage, and the later stages reuse its decisions;
+ * 1: every stage evaluates it */
+int mlr_indicator_cadence(mlr_ctx* ctx, int every_stage);
+int mlr_ensemble_indicator_cadence(mlr_ensemble* ens, int every_stage);
 /* cells judged smooth (summed over every gated TENO pass) since the last reset */
 int mlr_indicator_gated(mlr_ctx* ctx, int reset, int64_t* gated);
 /* cell volumes in reference numbering */

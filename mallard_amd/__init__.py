@@ -108,6 +108,8 @@ _SIGS = {
     "mlr_ensemble_run_steps": (C.c_int, [_P, C.c_int64]),
     "mlr_ensemble_run_steps_profiled": (C.c_int, [_P, C.c_int64, _P]),
     "mlr_ensemble_autotune": (C.c_int, [_P, _P]),
+    "mlr_indicator_cadence": (C.c_int, [_P, C.c_int]),
+    "mlr_ensemble_indicator_cadence": (C.c_int, [_P, C.c_int]),
     "mlr_ensemble_get_volumes": (C.c_int, [_P, C.c_int, _P]),
     "mlr_ensemble_set_tune": (C.c_int, [_P, _P]),
     "mlr_ensemble_set_indicator": (C.c_int, [_P, _P, C.c_double]),
@@ -430,8 +432,10 @@ class Solver:
         self._check(lib().mlr_write_vtu(self.h, os.fspath(path).encode(), names, len(variables)))
 
     # ---- learned troubled-cell indicator
-    def set_indicator(self, params, threshold=0.9):
-        """Gate TENO with the network (params: 7361 FP32 values; None: off)."""
+    def set_indicator(self, params, threshold=0.9, every_stage=False):
+        """Gate TENO with the network (params: 7361 FP32 values; None: off).
+        every_stage False: the network judges the cells on each step's first
+        stage and the later stages reuse the decisions; True: every stage."""
         if params is None:
             self._check(lib().mlr_set_indicator(self.h, None, threshold))
         else:
@@ -439,6 +443,7 @@ class Solver:
             if p.size != N_PARAMS:
                 raise ValueError("indicator parameters must have %d values" % N_PARAMS)
             self._check(lib().mlr_set_indicator(self.h, _p(p), threshold))
+            self._check(lib().mlr_indicator_cadence(self.h, int(every_stage)))
 
     def gated(self, reset=False):
         n = C.c_int64()
@@ -546,9 +551,10 @@ class Ensemble:
         return dict(total_ms=st[0], teno_ms=st[1], teno_n=int(st[2]), flux_ms=st[3], flux_n=int(st[4]),
                     dt_ms=st[5], dt_n=int(st[6]))
 
-    def set_indicator(self, params, threshold=0.9):
+    def set_indicator(self, params, threshold=0.9, every_stage=False):
         p = None if params is None else _params(params)
         self._check(lib().mlr_ensemble_set_indicator(self.h, _p(p), threshold))
+        self._check(lib().mlr_ensemble_indicator_cadence(self.h, int(every_stage)))
 
     def autotune(self):
         t = np.zeros(3, np.int32)

@@ -848,6 +848,16 @@ int mlr_set_indicator(mlr_ctx* ctx, const float* params, double threshold) {
     return guarded(ctx, [&] { ctx->solver.set_indicator(params, threshold); });
 }
 
+int mlr_indicator_cadence(mlr_ctx* ctx, int every_stage) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.set_indicator_cadence(every_stage != 0); });
+}
+
+int mlr_ensemble_indicator_cadence(mlr_ensemble* ens, int every_stage) {
+    if (!ens) return 1;
+    return guarded(nullptr, [&] { ens->e->set_indicator_cadence(every_stage != 0); });
+}
+
 int mlr_indicator_gated(mlr_ctx* ctx, int reset, int64_t* gated) {
     if (!ctx || !gated) return 1;
     return guarded(ctx, [&] { *gated = (int64_t)ctx->solver.gated_cells(reset != 0); });

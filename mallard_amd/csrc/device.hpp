@@ -75,6 +75,7 @@ struct Teno {
     int32_t gate;
     ind::Net net;
     uint32_t* gated;
+    uint8_t* gate_dec;  // per cell: 1 when the last network evaluation judged it smooth
     double gamma;
 };
 
@@ -137,8 +138,10 @@ struct Tune {
 };
 
 // Launchers (asynchronous on `stream`). Each returns a hipError_t.
+// gate_reuse: with the gate on, take each cell's recorded decision instead of
+// evaluating the network again (later stages of a step)
 hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs, uint8_t* decisions,
-                      const Physics& ph, int occ, hipStream_t stream);
+                      const Physics& ph, int occ, int gate_reuse, hipStream_t stream);
 // bs: cells (threads) per block
 hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, uint32_t bs, hipStream_t stream);
 // spectral radii: lam_cell per cell and, per bs-cell block, the bit pattern
@@ -159,7 +162,7 @@ hipError_t viscous_grad(const Geom& g, const double* U, const Physics& ph, doubl
 // width (small = 8-bit) and Riemann solver.
 // gated: some case runs the learned gate (the block stages the network in LDS)
 hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
-                      bool small, bool gated, int occ, const Physics& ph, hipStream_t stream);
+                      bool small, bool gated, int occ, int gate_reuse, const Physics& ph, hipStream_t stream);
 hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const BatchStage& st, int riemann,
                       const Physics& ph, uint32_t bs, hipStream_t stream);
 hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const Physics& ph, uint32_t bs,

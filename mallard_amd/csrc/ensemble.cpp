@@ -97,6 +97,11 @@ void Ensemble::set_indicator(const float* params, double threshold) {
     tuned_ = false;  // the gated kernel has its own best shape
 }
 
+void Ensemble::set_indicator_cadence(bool every_stage) {
+    gate_per_stage_ = every_stage;
+    drop_graph();
+}
+
 void Ensemble::set_tune(const dev::Tune& t) {
     auto bs_ok = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
     if (!bs_ok(t.flux_block)) fail("block sizes must be 64, 128 or 256 cells");
@@ -158,7 +163,7 @@ void Ensemble::autotune() {
         for (int occ : {0, 5, 6}) {
             if (occ != 0 && !occ_tunable) continue;
             const float t = timed([&] {
-                hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, dev::kBufUn, K_, M_, FQ_, small_, gated_, occ, ph_,
+                hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, dev::kBufUn, K_, M_, FQ_, small_, gated_, occ, 0, ph_,
                                        stream_), "teno_batch");
             });
             if (t < bt) { bt = t; best.teno_occ = occ; }
@@ -203,8 +208,10 @@ void Ensemble::launch_step(bool prof) {
     const int vac = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
     auto stage = [&](int op, int in, int out) {
         if (teno_) {
-            hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, gated_, tune_.teno_occ, ph_,
-                                   stream_), "teno_batch");
+            const bool first = op == dev::kSSP1 || op == dev::kRK4_1;
+            const int reuse = (gated_ && !gate_per_stage_ && !first) ? 1 : 0;
+            hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, gated_, tune_.teno_occ, reuse,
+                                   ph_, stream_), "teno_batch");
             mark(kStatTenoMs);
         }
         const dev::BatchStage st{op, teno_ ? 0 : 1, vac, in, out};

@@ -35,6 +35,8 @@ public:
 
     // the learned indicator on every case (params NULL: off)
     void set_indicator(const float* params, double threshold);
+    // as Solver::set_indicator_cadence
+    void set_indicator_cadence(bool every_stage);
     // launch shapes (flux and spectral blocks share one size here), timed on
     // the cases' current states before the first steps unless pinned
     void autotune();
@@ -57,7 +59,7 @@ private:
     dev::CaseBufs* d_cases_ = nullptr;
     uint32_t teno_blocks_ = 0, flux_blocks_ = 0;
     int K_ = 0, M_ = 0, FQ_ = 0;
-    bool small_ = true, teno_ = false, gated_ = false;
+    bool small_ = true, teno_ = false, gated_ = false, gate_per_stage_ = false;
     dev::Physics ph_{};
     dev::Tune tune_{};
     bool tuned_ = false;

@@ -317,7 +317,7 @@ __device__ inline void quad_gather(double f, double* out, int q) {
 template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE>
 __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const double* __restrict__ U,
                                            double* __restrict__ fs, uint8_t* __restrict__ dec, const Physics& ph,
-                                           uint32_t lb, double2* sh_u, unsigned char* sh_net) {
+                                           uint32_t lb, double2* sh_u, unsigned char* sh_net, int gate_reuse) {
     const uint32_t N = g.N;
     if (lb * 64u >= N) return;  // whole block past the end (uniform)
     {
@@ -325,7 +325,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         const double2* U2 = reinterpret_cast<const double2*>(U);
         for (uint32_t j = threadIdx.x; j < 2 * nloc; j += blockDim.x)
             sh_u[j] = U2[2 * (size_t)t.blk_ids[t0 + (j >> 1)] + (j & 1u)];
-        if (GATE && t.gate) ind::stage_net(t.net, sh_net);  // the network, once per block
+        if (GATE && t.gate && !gate_reuse) ind::stage_net(t.net, sh_net);  // the network, once per block
     }
     __syncthreads();
     const IdxT* __restrict__ loc = sizeof(IdxT) == 1 ? reinterpret_cast<const IdxT*>(t.loc8)
@@ -340,7 +340,12 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
     // (features from the LDS tile, one MFMA column per cell); a smooth cell
     // skips its directional stencils and takes the central one.
     bool smooth = false;
-    if (GATE && t.gate) {
+    if (GATE && t.gate && gate_reuse) {
+        // the decisions the network recorded earlier in the step
+        smooth = valid && t.gate_dec[c] != 0;
+        const uint64_t bal = __ballot(smooth && q == 0);
+        if ((threadIdx.x & 63) == 0 && bal) t.gated[lb * 4u + (threadIdx.x >> 6)] += (uint32_t)__popcll(bal);
+    } else if (GATE && t.gate) {
         const int lane = threadIdx.x & 63, gq = lane >> 4;
         const uint32_t cm = lb * 64u + (threadIdx.x >> 6) * 16u + (lane & 15);
         const bool vm = cm < N;
@@ -385,6 +390,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         }
         const float z = ind::mlp_wave(ind::lds_net(t.net, sh_net), xb, lane);
         const int sm = (vm && okc && z >= t.net.logit_threshold) ? 1 : 0;
+        if (vm && gq == 0) t.gate_dec[cm] = (uint8_t)sm;
         const uint64_t bal = __ballot(sm && gq == 0);
         if (lane == 0 && bal) t.gated[lb * 4u + (threadIdx.x >> 6)] += (uint32_t)__popcll(bal);
         smooth = __shfl(sm, lane >> 2) != 0;
@@ -602,12 +608,12 @@ template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC == 0 ? 1 : OCC, OCC == 0 ? 8 : OCC)))
 void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
                                                    double* __restrict__ fs, uint8_t* __restrict__ dec,
-                                                   Physics ph, uint32_t blocks_per_xcd) {
+                                                   Physics ph, uint32_t blocks_per_xcd, int gate_reuse) {
     // dynamic LDS: the tile's states (TILE cells), then the network when gated
     extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
     teno_block<KP, MP, FQ, IdxT, TILE, GATE>(g, t, U, fs, dec, ph, lb, reinterpret_cast<double2*>(teno_smem),
-                                       teno_smem + TILE * 32);
+                                             teno_smem + TILE * 32, gate_reuse);
 }
 
 // case of global block gb: the last case whose first block is <= gb
@@ -634,14 +640,15 @@ __device__ inline double* case_buf_w(const CaseBufs& c, int role) {
 template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC == 0 ? 1 : OCC, OCC == 0 ? 8 : OCC)))
 void teno_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, int in_role,
-                                                         uint32_t total, Physics ph, uint32_t blocks_per_xcd) {
+                                                         uint32_t total, Physics ph, uint32_t blocks_per_xcd,
+                                                         int gate_reuse) {
     extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t gb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
     if (gb >= total) return;
     const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, false)];
     teno_block<KP, MP, FQ, IdxT, TILE, GATE>(cs.g, cs.t, case_buf(cs, in_role), cs.fs, cs.dec, ph,
-                                             gb - cs.blk_teno0,
-                                       reinterpret_cast<double2*>(teno_smem), teno_smem + TILE * 32);
+                                             gb - cs.blk_teno0, reinterpret_cast<double2*>(teno_smem),
+                                             teno_smem + TILE * 32, gate_reuse);
 }
 
 // ------------------------------------------------------------ viscous terms
@@ -1147,7 +1154,7 @@ __global__ void probe_kernel(const double* x, const double* y, double* out, int 
 }  // namespace
 
 hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs, uint8_t* dec,
-                      const Physics& ph, int occ, hipStream_t stream) {
+                      const Physics& ph, int occ, int gate_reuse, hipStream_t stream) {
     const int Kp = t.K - 1, Mp = t.M - 1;
     const uint32_t FQ = g.S * g.Q;
     if (g.S != 3 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
@@ -1156,14 +1163,14 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
     const dim3 grid(8 * per_xcd), block(256);
     const bool small = t.loc8 != nullptr;
     if (!small && t.loc16 == nullptr) return hipErrorInvalidValue;
-    const size_t net_lds = t.gate ? ind::kNetInferBytes : 0;
+    const size_t net_lds = (t.gate && !gate_reuse) ? ind::kNetInferBytes : 0;
     // small: 8-bit tile positions (tiles of <= 256 cells); O: occupancy target
 #define MLR_TENO_S(KP, MP, FQV, G, O)                                                                       \
     hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint8_t, 256, G, O>), grid, block, 256 * 32 + net_lds, stream, \
-                       g, t, U, fs, dec, ph, per_xcd)
+                       g, t, U, fs, dec, ph, per_xcd, gate_reuse)
 #define MLR_TENO_L(KP, MP, FQV, G)                                                                         \
     hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint16_t, 1024, G, 0>), grid, block, 1024 * 32 + net_lds,   \
-                       stream, g, t, U, fs, dec, ph, per_xcd)
+                       stream, g, t, U, fs, dec, ph, per_xcd, gate_reuse)
 #define MLR_TENO_G(KP, MP, FQV, G)                  \
     do {                                            \
         if (small) MLR_TENO_S(KP, MP, FQV, G, 0);   \
@@ -1248,23 +1255,24 @@ hipError_t finalize_dt(const unsigned long long* lam_blk, uint32_t n_blk, double
 }
 
 hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
-                      bool small, bool gated, int occ, const Physics& ph, hipStream_t stream) {
+                      bool small, bool gated, int occ, int gate_reuse, const Physics& ph, hipStream_t stream) {
     const int Kp = K - 1, Mp = M - 1;
-    const size_t net_lds = gated ? ind::kNetInferBytes : 0;
+    const size_t net_lds = (gated && !gate_reuse) ? ind::kNetInferBytes : 0;
     if (n_cases < 1 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
     if (occ != 0 && occ != 5 && occ != 6) return hipErrorInvalidValue;
     const uint32_t per_xcd = (total_blocks + 7) / 8;
     const dim3 grid(8 * per_xcd), block(256);
 #define MLR_TB_S(KP, MP, FQV, G, O)                                                                             \
     hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint8_t, 256, G, O>), grid, block, 256 * 32 + net_lds,   \
-                       stream, cases, n_cases, in_role, total_blocks, ph, per_xcd)
+                       stream, cases, n_cases, in_role, total_blocks, ph, per_xcd, gate_reuse)
 #define MLR_TB_G(KP, MP, FQV, G)                                                                               \
     do {                                                                                                       \
         if (small)                                                                                             \
             MLR_TB_S(KP, MP, FQV, G, 0);                                                                       \
         else                                                                                                   \
             hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint16_t, 1024, G, 0>), grid, block,              \
-                               1024 * 32 + net_lds, stream, cases, n_cases, in_role, total_blocks, ph, per_xcd); \
+                               1024 * 32 + net_lds, stream, cases, n_cases, in_role, total_blocks, ph, per_xcd, \
+                               gate_reuse);                                                             \
     } while (0)
 #define MLR_TB(KP, MP, FQV)                        \
     do {                                           \

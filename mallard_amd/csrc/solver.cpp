@@ -490,6 +490,7 @@ dev::Teno Solver::teno() const {
         for (int j = 0; j < Kp; ++j) t.OI[i * Kp + j] = L_.OI[(size_t)(i + 1) * L_.K + (j + 1)];
     t.gamma = ph_.gamma;
     t.gated = gated_;
+    t.gate_dec = gate_dec_;
     t.gate = gate_ ? 1 : 0;
     if (gate_) t.net = dev::net_view(net_buf_, gate_thr_, nullptr);
     return t;
@@ -517,6 +518,9 @@ void Solver::set_indicator(const float* params, double threshold) {
         HIP_OK(hipMalloc(&th, kIndParams * sizeof(float)));
         allocs_.push_back(th);
         theta_ = static_cast<float*>(th);
+        HIP_OK(hipMalloc((void**)&gate_dec_, L_.N));
+        allocs_.push_back(gate_dec_);
+        HIP_OK(hipMemset(gate_dec_, 0, L_.N));
     }
     HIP_OK(hipMemcpyAsync(theta_, params, kIndParams * sizeof(float), hipMemcpyHostToDevice, stream_));
     HIP_OK(dev::net_pack(theta_, net_buf_, stream_));
@@ -525,6 +529,11 @@ void Solver::set_indicator(const float* params, double threshold) {
     gate_ = true;
     drop_graph();  // the captured step carries the gate in its kernel arguments
     tuned_ = false;  // the gated kernel has its own best shape
+}
+
+void Solver::set_indicator_cadence(bool every_stage) {
+    gate_per_stage_ = every_stage;
+    drop_graph();  // the captured step carries the cadence in its kernel arguments
 }
 
 uint64_t Solver::gated_cells(bool reset) {
@@ -635,8 +644,11 @@ void Solver::mark(int tag) {
     ++prof_used_;
 }
 
-void Solver::launch_teno(const double* U) {
-    HIP_OK(dev::teno_faces(geom(), teno(), U, fs_, dec_, ph_, tune_.teno_occ, stream_));
+void Solver::launch_teno(const double* U, bool first_stage) {
+    // with the gate on, the network judges the cells on a step's first stage;
+    // later stages reuse its decisions unless every stage evaluates it
+    const int reuse = (gate_ && !gate_per_stage_ && !first_stage) ? 1 : 0;
+    HIP_OK(dev::teno_faces(geom(), teno(), U, fs_, dec_, ph_, tune_.teno_occ, reuse, stream_));
     mark(kStatTenoMs);
 }
 
@@ -647,7 +659,7 @@ void Solver::launch_flux(int op, const double* Uin, double* Uout) {
 }
 
 void Solver::stage_pass(int op, const double* Uin, double* Uout) {
-    if (L_.teno) launch_teno(Uin);
+    if (L_.teno) launch_teno(Uin, op == dev::kFE || op == dev::kSSP1 || op == dev::kRK4_1 || op == dev::kRhsOnly);
     launch_flux(op, Uin, Uout);
 }
 
@@ -817,12 +829,12 @@ void Solver::autotune() {
         for (int occ : {0, 5, 6}) {
             if (occ != 0 && !occ_tunable) continue;
             const float t = timed([&] {
-                HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, occ, stream_));
+                HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, occ, 0, stream_));
             });
             if (t < bt) { bt = t; best.teno_occ = occ; }
         }
         tune_.teno_occ = best.teno_occ;
-        HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, tune_.teno_occ, stream_));
+        HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, tune_.teno_occ, 0, stream_));
         if (saved) {
             HIP_OK(hipMemcpyAsync(gated_, saved, gated_slots_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
             synchronize();

@@ -130,7 +130,7 @@ public:
 
     // kernel-level entry points used by the benchmark's timing harness
     void launch_step();                 // one full step, asynchronously
-    void launch_teno(const double* U);  // one TENO face-state pass
+    void launch_teno(const double* U, bool first_stage = true);  // one TENO face-state pass
     void launch_flux(int op, const double* Uin, double* Uout);
     double* d_state() { return Un_; }
     uint64_t device_bytes() const { return dev_bytes_; }  // all device allocations
@@ -139,6 +139,9 @@ public:
     // smooth-probability threshold; NULL turns the gate off
     void set_indicator(const float* params, double threshold);
     bool indicator_on() const { return gate_; }
+    // false (default): the network judges the cells once per step, on its
+    // first stage, and the later stages reuse the decisions; true: every stage
+    void set_indicator_cadence(bool every_stage);
     uint64_t gated_cells(bool reset);      // cells judged smooth since the last reset
     void collect_features(Dataset& ds);    // features + TENO labels of the current state
     void indicator_features(uint16_t* X_ref, float* y_ref);  // same, reference numbering
@@ -203,6 +206,8 @@ private:
     }
     int64_t* step_dev_ = nullptr;
     uint32_t* gated_ = nullptr;  // per-wave-slot smooth counts of the gated TENO kernel
+    uint8_t* gate_dec_ = nullptr;  // per-cell decisions of the last network evaluation
+    bool gate_per_stage_ = false;
     size_t gated_slots_ = 0;
     void* net_buf_ = nullptr;
     float* theta_ = nullptr;

@@ -93,7 +93,8 @@ def test_training_is_deterministic_and_learns(mlr):
     assert h1[-50:].mean() < 0.6 * h1[:50].mean()
 
 
-def test_gate_that_never_fires_is_plain_teno(mlr):
+@pytest.mark.parametrize("every_stage", [False, True])
+def test_gate_that_never_fires_is_plain_teno(mlr, every_stage):
     num = mlr.numerics(recon="TENO", order=2, teno_variant="corrected", cfl=0.4)
     gen = mlr.mesh_gen("cartesian_tri", 40, 40, 1.0, 1.0, 0.2, 8)
     a, b = mlr.Solver(gen=gen, num=num), mlr.Solver(gen=gen, num=num)
@@ -101,7 +102,7 @@ def test_gate_that_never_fires_is_plain_teno(mlr):
         s.init_quadrant(0.5, 0.5, RIEMANN2D)
     th = np.zeros(mlr.N_PARAMS, np.float32)
     th[ir.OFF_B3] = -50.0  # p = sigmoid(-50) < 0.9 everywhere
-    b.set_indicator(th, 0.9)
+    b.set_indicator(th, 0.9, every_stage=every_stage)
     a.run_steps(20)
     b.run_steps(20)
     assert np.array_equal(a.state()[0], b.state()[0])
@@ -125,10 +126,12 @@ def test_gate_that_always_fires_on_smooth_field_is_plain_teno(mlr):
     assert b.gated() == a.n_cells * 3 * 5
 
 
-def test_trained_indicator_gains_at_matched_l1(mlr):
+@pytest.mark.parametrize("every_stage", [False, True])
+def test_trained_indicator_gains_at_matched_l1(mlr, every_stage):
     """The trained network skips the directional stencils in most cells of a
     Riemann problem it was not trained on, and the solution stays within
-    1e-3 relative L1 of plain TENO per variable."""
+    1e-3 relative L1 of plain TENO per variable, whether it judges the cells
+    once per step (the default) or on every stage."""
     from mallard_amd import indicator as ind
     ds, y = ind.build_dataset(n_cases=64, nx=48, snapshots=(0, 8, 20), seed=21)
     params, hist = ind.train_indicator(ds, ind.make_pool(y, seed=2), steps=1500, batch=2048, lr=2e-3, seed=3)
@@ -137,7 +140,7 @@ def test_trained_indicator_gains_at_matched_l1(mlr):
     a, b = mlr.Solver(gen=gen, num=num), mlr.Solver(gen=gen, num=num)
     for s in (a, b):
         s.init_quadrant(0.5, 0.5, RIEMANN2D)
-    b.set_indicator(params, 0.9)
+    b.set_indicator(params, 0.9, every_stage=every_stage)
     a.run_steps(100)
     b.run_steps(100)
     Ua, Ub = a.state()[0], b.state()[0]
