@@ -469,7 +469,8 @@ def main():
         "kernels_ms": {"teno": teno_ms, "flux_update": flux_ms,
                        "spectral_dt": st["dt_ms"] / max(st["dt_n"], 1),
                        "copy": st["copy_ms"] / max(st["copy_n"], 1) if st["copy_n"] else None},
-        "tune": {"teno_occupancy": tune[0], "flux_block": tune[1], "spectral_block": tune[2]},
+        "tune": {"teno_occupancy": tune[0], "flux_block": tune[1], "spectral_block": tune[2],
+                 "teno_occupancy_reuse": tune[3]},
     }
     if not args.no_cpu:
         log("[bench] CPU baseline ...")

@@ -152,10 +152,11 @@ int mlr_run_steps(mlr_ctx* ctx, int64_t n, double* dts);
  *              spectral-radius+dt ms, launches, copy ms, launches }. */
 int mlr_run_steps_profiled(mlr_ctx* ctx, int64_t n, double* stats);
 /* Launch shapes; no reference counterpart (the reference leaves them to
- * Kokkos' defaults). tune[3] = { TENO occupancy target in waves per SIMD
+ * Kokkos' defaults). tune[4] = { TENO occupancy target in waves per SIMD
  * (0: the compiler's choice, 5, 6; used at p = 2 with the default stencil
  * size), cells per flux/stage-update block (64, 128, 256), cells per
- * spectral-radius block (64, 128, 256) }. Every choice gives identical
+ * spectral-radius block (64, 128, 256), the TENO occupancy target of gated
+ * passes that reuse recorded decisions }. Every choice gives identical
  * results. mlr_autotune times the candidates on the current state and
  * returns the fastest (also run by itself before the first steps, cached per
  * mesh size and numerics); mlr_set_tune pins a choice. */
@@ -245,7 +246,7 @@ int mlr_ensemble_run_steps(mlr_ensemble* ens, int64_t n);
 /* kernel-by-kernel with events (the stats layout of mlr_run_steps_profiled) */
 int mlr_ensemble_run_steps_profiled(mlr_ensemble* ens, int64_t n, double* stats);
 /* launch shapes of the batched kernels, as mlr_autotune / mlr_set_tune (the
- * flux and spectral block sizes are one value here: tune[2] = tune[1]) */
+ * flux and spectral block sizes are one value here: tune[2] = tune[1]), tune[4] */
 int mlr_ensemble_autotune(mlr_ensemble* ens, int32_t* tune);
 int mlr_ensemble_set_tune(mlr_ensemble* ens, const int32_t* tune);
 int mlr_ensemble_set_indicator(mlr_ensemble* ens, const float* params, double threshold);

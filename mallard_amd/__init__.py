@@ -382,17 +382,18 @@ class Solver:
 
     def autotune(self):
         """Time the launch-shape candidates on the current state and keep the
-        fastest: (TENO occupancy target, flux block, spectral block)."""
-        t = np.zeros(3, np.int32)
+        fastest: (TENO occupancy target, flux block, spectral block, TENO
+        occupancy target of gated passes reusing recorded decisions)."""
+        t = np.zeros(4, np.int32)
         self._check(lib().mlr_autotune(self.h, _p(t)))
         return tuple(int(x) for x in t)
 
-    def set_tune(self, teno_occ=0, flux_block=256, spec_block=256):
-        t = np.array([teno_occ, flux_block, spec_block], np.int32)
+    def set_tune(self, teno_occ=0, flux_block=256, spec_block=256, teno_occ_reuse=0):
+        t = np.array([teno_occ, flux_block, spec_block, teno_occ_reuse], np.int32)
         self._check(lib().mlr_set_tune(self.h, _p(t)))
 
     def tune(self):
-        t = np.zeros(3, np.int32)
+        t = np.zeros(4, np.int32)
         self._check(lib().mlr_get_tune(self.h, _p(t)))
         return tuple(int(x) for x in t)
 
@@ -557,12 +558,12 @@ class Ensemble:
         self._check(lib().mlr_ensemble_indicator_cadence(self.h, int(every_stage)))
 
     def autotune(self):
-        t = np.zeros(3, np.int32)
+        t = np.zeros(4, np.int32)
         self._check(lib().mlr_ensemble_autotune(self.h, _p(t)))
         return tuple(int(x) for x in t)
 
-    def set_tune(self, teno_occ=0, block=256):
-        t = np.array([teno_occ, block, block], np.int32)
+    def set_tune(self, teno_occ=0, block=256, teno_occ_reuse=0):
+        t = np.array([teno_occ, block, block, teno_occ_reuse], np.int32)
         self._check(lib().mlr_ensemble_set_tune(self.h, _p(t)))
 
     def gated(self, reset=False):

@@ -330,12 +330,14 @@ void tune_out(const mlr::dev::Tune& t, int32_t* out) {
     out[0] = t.teno_occ;
     out[1] = (int32_t)t.flux_block;
     out[2] = (int32_t)t.spec_block;
+    out[3] = t.teno_occ_reuse;
 }
 mlr::dev::Tune tune_in(const int32_t* in) {
     mlr::dev::Tune t;
     t.teno_occ = in[0];
     t.flux_block = (uint32_t)in[1];
     t.spec_block = (uint32_t)in[2];
+    t.teno_occ_reuse = in[3];
     return t;
 }
 }  // namespace

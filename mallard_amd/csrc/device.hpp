@@ -135,6 +135,7 @@ struct Tune {
                                 // compiler's choice), 5 or 6
     uint32_t flux_block = 256;  // cells per flux / stage-update block: 64, 128 or 256
     uint32_t spec_block = 256;  // cells per spectral-radius block: 64, 128 or 256
+    int32_t teno_occ_reuse = 0; // the same target for gated passes that reuse recorded decisions
 };
 
 // Launchers (asynchronous on `stream`). Each returns a hipError_t.

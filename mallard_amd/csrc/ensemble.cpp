@@ -105,7 +105,8 @@ void Ensemble::set_indicator_cadence(bool every_stage) {
 void Ensemble::set_tune(const dev::Tune& t) {
     auto bs_ok = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
     if (!bs_ok(t.flux_block)) fail("block sizes must be 64, 128 or 256 cells");
-    if (t.teno_occ != 0 && t.teno_occ != 5 && t.teno_occ != 6) fail("TENO occupancy target must be 0, 5 or 6");
+    auto occ_ok = [](int o) { return o == 0 || o == 5 || o == 6; };
+    if (!occ_ok(t.teno_occ) || !occ_ok(t.teno_occ_reuse)) fail("TENO occupancy target must be 0, 5 or 6");
     tune_ = t;
     tune_.spec_block = t.flux_block;  // flux and spectral blocks share the case table's offsets
     tuned_ = true;
@@ -168,6 +169,17 @@ void Ensemble::autotune() {
             });
             if (t < bt) { bt = t; best.teno_occ = occ; }
         }
+        if (gated_ && !gate_per_stage_) {  // passes that reuse the decisions just recorded
+            bt = 1e30f;
+            for (int occ : {0, 5, 6}) {
+                if (occ != 0 && !occ_tunable) continue;
+                const float t = timed([&] {
+                    hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, dev::kBufUn, K_, M_, FQ_, small_, gated_, occ,
+                                           1, ph_, stream_), "teno_batch");
+                });
+                if (t < bt) { bt = t; best.teno_occ_reuse = occ; }
+            }
+        }
         if (gated_) {
             for (size_t k = 0; k < cases_.size(); ++k) {
                 const Solver& c = *cases_[k];
@@ -210,8 +222,9 @@ void Ensemble::launch_step(bool prof) {
         if (teno_) {
             const bool first = op == dev::kSSP1 || op == dev::kRK4_1;
             const int reuse = (gated_ && !gate_per_stage_ && !first) ? 1 : 0;
-            hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, gated_, tune_.teno_occ, reuse,
-                                   ph_, stream_), "teno_batch");
+            hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, gated_,
+                                   reuse ? tune_.teno_occ_reuse : tune_.teno_occ, reuse, ph_, stream_),
+                   "teno_batch");
             mark(kStatTenoMs);
         }
         const dev::BatchStage st{op, teno_ ? 0 : 1, vac, in, out};

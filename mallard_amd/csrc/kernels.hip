@@ -314,10 +314,12 @@ __device__ inline void quad_gather(double f, double* out, int q) {
 // fetched once per block instead of once per stencil member. Blocks are
 // mapped so that each of the 8 XCDs walks one contiguous range of the
 // space-filling-curve order.
-template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE>
+// GM: 0 plain TENO; 1 the learned gate evaluates the network and records
+// its decisions; 2 the gate takes the recorded decisions (no network code)
+template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM>
 __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const double* __restrict__ U,
                                            double* __restrict__ fs, uint8_t* __restrict__ dec, const Physics& ph,
-                                           uint32_t lb, double2* sh_u, unsigned char* sh_net, int gate_reuse) {
+                                           uint32_t lb, double2* sh_u, unsigned char* sh_net) {
     const uint32_t N = g.N;
     if (lb * 64u >= N) return;  // whole block past the end (uniform)
     {
@@ -325,7 +327,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         const double2* U2 = reinterpret_cast<const double2*>(U);
         for (uint32_t j = threadIdx.x; j < 2 * nloc; j += blockDim.x)
             sh_u[j] = U2[2 * (size_t)t.blk_ids[t0 + (j >> 1)] + (j & 1u)];
-        if (GATE && t.gate && !gate_reuse) ind::stage_net(t.net, sh_net);  // the network, once per block
+        if (GM == 1 && t.gate) ind::stage_net(t.net, sh_net);  // the network, once per block
     }
     __syncthreads();
     const IdxT* __restrict__ loc = sizeof(IdxT) == 1 ? reinterpret_cast<const IdxT*>(t.loc8)
@@ -340,12 +342,12 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
     // (features from the LDS tile, one MFMA column per cell); a smooth cell
     // skips its directional stencils and takes the central one.
     bool smooth = false;
-    if (GATE && t.gate && gate_reuse) {
+    if (GM == 2 && t.gate) {
         // the decisions the network recorded earlier in the step
         smooth = valid && t.gate_dec[c] != 0;
         const uint64_t bal = __ballot(smooth && q == 0);
         if ((threadIdx.x & 63) == 0 && bal) t.gated[lb * 4u + (threadIdx.x >> 6)] += (uint32_t)__popcll(bal);
-    } else if (GATE && t.gate) {
+    } else if (GM == 1 && t.gate) {
         const int lane = threadIdx.x & 63, gq = lane >> 4;
         const uint32_t cm = lb * 64u + (threadIdx.x >> 6) * 16u + (lane & 15);
         const bool vm = cm < N;
@@ -420,7 +422,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         }
     };
 
-    if (GATE && t.gate && __all(smooth || !valid)) {
+    if (GM != 0 && t.gate && __all(smooth || !valid)) {
         // ---- every cell of the wave takes its central stencil: the quad
         // splits its members (lane q: members q, q+4, q+8, ...), then a
         // reduce-scatter over the quad leaves lane v the coefficients of
@@ -604,16 +606,16 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
 
 // OCC: waves per SIMD the register allocation targets (0: the compiler's
 // own choice); an autotuner picks it per mesh, results are identical
-template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE, int OCC>
+template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC == 0 ? 1 : OCC, OCC == 0 ? 8 : OCC)))
 void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
                                                    double* __restrict__ fs, uint8_t* __restrict__ dec,
-                                                   Physics ph, uint32_t blocks_per_xcd, int gate_reuse) {
+                                                   Physics ph, uint32_t blocks_per_xcd) {
     // dynamic LDS: the tile's states (TILE cells), then the network when gated
     extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
-    teno_block<KP, MP, FQ, IdxT, TILE, GATE>(g, t, U, fs, dec, ph, lb, reinterpret_cast<double2*>(teno_smem),
-                                             teno_smem + TILE * 32, gate_reuse);
+    teno_block<KP, MP, FQ, IdxT, TILE, GM>(g, t, U, fs, dec, ph, lb, reinterpret_cast<double2*>(teno_smem),
+                                           teno_smem + TILE * 32);
 }
 
 // case of global block gb: the last case whose first block is <= gb
@@ -637,18 +639,17 @@ __device__ inline double* case_buf_w(const CaseBufs& c, int role) {
 // The TENO pass of every case of an ensemble in one launch: the grid covers
 // all cases' 64-cell blocks (XCD-aware order over the whole grid), each
 // block finds its case in the offset table and runs that case's block.
-template <int KP, int MP, int FQ, typename IdxT, int TILE, bool GATE, int OCC>
+template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC == 0 ? 1 : OCC, OCC == 0 ? 8 : OCC)))
 void teno_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, int in_role,
-                                                         uint32_t total, Physics ph, uint32_t blocks_per_xcd,
-                                                         int gate_reuse) {
+                                                         uint32_t total, Physics ph, uint32_t blocks_per_xcd) {
     extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t gb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
     if (gb >= total) return;
     const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, false)];
-    teno_block<KP, MP, FQ, IdxT, TILE, GATE>(cs.g, cs.t, case_buf(cs, in_role), cs.fs, cs.dec, ph,
-                                             gb - cs.blk_teno0, reinterpret_cast<double2*>(teno_smem),
-                                             teno_smem + TILE * 32, gate_reuse);
+    teno_block<KP, MP, FQ, IdxT, TILE, GM>(cs.g, cs.t, case_buf(cs, in_role), cs.fs, cs.dec, ph,
+                                           gb - cs.blk_teno0, reinterpret_cast<double2*>(teno_smem),
+                                           teno_smem + TILE * 32);
 }
 
 // ------------------------------------------------------------ viscous terms
@@ -1167,20 +1168,21 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
     // small: 8-bit tile positions (tiles of <= 256 cells); O: occupancy target
 #define MLR_TENO_S(KP, MP, FQV, G, O)                                                                       \
     hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint8_t, 256, G, O>), grid, block, 256 * 32 + net_lds, stream, \
-                       g, t, U, fs, dec, ph, per_xcd, gate_reuse)
+                       g, t, U, fs, dec, ph, per_xcd)
 #define MLR_TENO_L(KP, MP, FQV, G)                                                                         \
     hipLaunchKernelGGL((teno_kernel<KP, MP, FQV, uint16_t, 1024, G, 0>), grid, block, 1024 * 32 + net_lds,   \
-                       stream, g, t, U, fs, dec, ph, per_xcd, gate_reuse)
+                       stream, g, t, U, fs, dec, ph, per_xcd)
 #define MLR_TENO_G(KP, MP, FQV, G)                  \
     do {                                            \
         if (small) MLR_TENO_S(KP, MP, FQV, G, 0);   \
         else MLR_TENO_L(KP, MP, FQV, G);            \
     } while (0)
     // the gated kernels exist where the central stencil has the network's 12 members or more
-#define MLR_TENO(KP, MP, FQV)                         \
-    do {                                              \
-        if (t.gate) MLR_TENO_G(KP, MP, FQV, true);    \
-        else MLR_TENO_G(KP, MP, FQV, false);          \
+#define MLR_TENO(KP, MP, FQV)                              \
+    do {                                                   \
+        if (!t.gate) MLR_TENO_G(KP, MP, FQV, 0);           \
+        else if (gate_reuse) MLR_TENO_G(KP, MP, FQV, 2);   \
+        else MLR_TENO_G(KP, MP, FQV, 1);                   \
     } while (0)
     if (t.gate && Mp < ind::kMembers - 1) return hipErrorInvalidValue;
     // the default stencil size (factor 2: M = 2K) is compiled with the member
@@ -1188,18 +1190,20 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
     // take the runtime loop
     if (FQ == 3) {
         switch (Kp) {
-        case 2: if (Mp == 5) MLR_TENO_G(2, 5, 3, false); else MLR_TENO_G(2, 0, 3, false); break;
+        case 2: if (Mp == 5) MLR_TENO_G(2, 5, 3, 0); else MLR_TENO_G(2, 0, 3, 0); break;
         case 5:
             if (Mp == 11 && small) {
-                if (t.gate) {
-                    if (occ == 5) MLR_TENO_S(5, 11, 3, true, 5);
-                    else if (occ == 6) MLR_TENO_S(5, 11, 3, true, 6);
-                    else MLR_TENO_S(5, 11, 3, true, 0);
-                } else {
-                    if (occ == 5) MLR_TENO_S(5, 11, 3, false, 5);
-                    else if (occ == 6) MLR_TENO_S(5, 11, 3, false, 6);
-                    else MLR_TENO_S(5, 11, 3, false, 0);
-                }
+                const int gm = !t.gate ? 0 : (gate_reuse ? 2 : 1);
+#define MLR_TENO_O(GMV)                                       \
+    do {                                                      \
+        if (occ == 5) MLR_TENO_S(5, 11, 3, GMV, 5);           \
+        else if (occ == 6) MLR_TENO_S(5, 11, 3, GMV, 6);      \
+        else MLR_TENO_S(5, 11, 3, GMV, 0);                    \
+    } while (0)
+                if (gm == 0) MLR_TENO_O(0);
+                else if (gm == 1) MLR_TENO_O(1);
+                else MLR_TENO_O(2);
+#undef MLR_TENO_O
             } else if (Mp == 11) {
                 MLR_TENO(5, 11, 3);
             } else {
@@ -1211,7 +1215,7 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
         }
     } else {
         switch (Kp) {
-        case 2: MLR_TENO_G(2, 0, 6, false); break;
+        case 2: MLR_TENO_G(2, 0, 6, 0); break;
         case 5: MLR_TENO(5, 0, 6); break;
         case 9: if (Mp == 19) MLR_TENO(9, 19, 6); else MLR_TENO(9, 0, 6); break;
         default: return hipErrorInvalidValue;
@@ -1264,35 +1268,37 @@ hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks,
     const dim3 grid(8 * per_xcd), block(256);
 #define MLR_TB_S(KP, MP, FQV, G, O)                                                                             \
     hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint8_t, 256, G, O>), grid, block, 256 * 32 + net_lds,   \
-                       stream, cases, n_cases, in_role, total_blocks, ph, per_xcd, gate_reuse)
+                       stream, cases, n_cases, in_role, total_blocks, ph, per_xcd)
 #define MLR_TB_G(KP, MP, FQV, G)                                                                               \
     do {                                                                                                       \
         if (small)                                                                                             \
             MLR_TB_S(KP, MP, FQV, G, 0);                                                                       \
         else                                                                                                   \
             hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint16_t, 1024, G, 0>), grid, block,              \
-                               1024 * 32 + net_lds, stream, cases, n_cases, in_role, total_blocks, ph, per_xcd, \
-                               gate_reuse);                                                             \
+                               1024 * 32 + net_lds, stream, cases, n_cases, in_role, total_blocks, ph, per_xcd); \
     } while (0)
-#define MLR_TB(KP, MP, FQV)                        \
-    do {                                           \
-        if (gated) MLR_TB_G(KP, MP, FQV, true);    \
-        else MLR_TB_G(KP, MP, FQV, false);         \
+#define MLR_TB(KP, MP, FQV)                              \
+    do {                                                 \
+        if (!gated) MLR_TB_G(KP, MP, FQV, 0);            \
+        else if (gate_reuse) MLR_TB_G(KP, MP, FQV, 2);   \
+        else MLR_TB_G(KP, MP, FQV, 1);                   \
     } while (0)
     if (gated && Mp < ind::kMembers - 1) return hipErrorInvalidValue;
     if (FQ == 3 && Kp == 5 && Mp == 11 && small) {
-        if (gated) {
-            if (occ == 5) MLR_TB_S(5, 11, 3, true, 5);
-            else if (occ == 6) MLR_TB_S(5, 11, 3, true, 6);
-            else MLR_TB_S(5, 11, 3, true, 0);
-        } else {
-            if (occ == 5) MLR_TB_S(5, 11, 3, false, 5);
-            else if (occ == 6) MLR_TB_S(5, 11, 3, false, 6);
-            else MLR_TB_S(5, 11, 3, false, 0);
-        }
+        const int gm = !gated ? 0 : (gate_reuse ? 2 : 1);
+#define MLR_TB_O(GMV)                                     \
+    do {                                                  \
+        if (occ == 5) MLR_TB_S(5, 11, 3, GMV, 5);         \
+        else if (occ == 6) MLR_TB_S(5, 11, 3, GMV, 6);    \
+        else MLR_TB_S(5, 11, 3, GMV, 0);                  \
+    } while (0)
+        if (gm == 0) MLR_TB_O(0);
+        else if (gm == 1) MLR_TB_O(1);
+        else MLR_TB_O(2);
+#undef MLR_TB_O
     } else if (FQ == 3 && Kp == 5 && Mp == 11) MLR_TB(5, 11, 3);
     else if (FQ == 3 && Kp == 5) MLR_TB(5, 0, 3);
-    else if (FQ == 3 && Kp == 2) MLR_TB_G(2, 0, 3, false);
+    else if (FQ == 3 && Kp == 2) MLR_TB_G(2, 0, 3, 0);
     else if (FQ == 6 && Kp == 9) MLR_TB(9, 0, 6);
     else return hipErrorInvalidValue;
 #undef MLR_TB_S

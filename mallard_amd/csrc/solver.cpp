@@ -648,7 +648,8 @@ void Solver::launch_teno(const double* U, bool first_stage) {
     // with the gate on, the network judges the cells on a step's first stage;
     // later stages reuse its decisions unless every stage evaluates it
     const int reuse = (gate_ && !gate_per_stage_ && !first_stage) ? 1 : 0;
-    HIP_OK(dev::teno_faces(geom(), teno(), U, fs_, dec_, ph_, tune_.teno_occ, reuse, stream_));
+    HIP_OK(dev::teno_faces(geom(), teno(), U, fs_, dec_, ph_, reuse ? tune_.teno_occ_reuse : tune_.teno_occ, reuse,
+                           stream_));
     mark(kStatTenoMs);
 }
 
@@ -764,7 +765,8 @@ void Solver::device_matrices(uint32_t cell, double* aplus, double* ta) const {
 void Solver::set_tune(const dev::Tune& t) {
     auto bs_ok = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
     if (!bs_ok(t.flux_block) || !bs_ok(t.spec_block)) fail("block sizes must be 64, 128 or 256 cells");
-    if (t.teno_occ != 0 && t.teno_occ != 5 && t.teno_occ != 6) fail("TENO occupancy target must be 0, 5 or 6");
+    auto occ_ok = [](int o) { return o == 0 || o == 5 || o == 6; };
+    if (!occ_ok(t.teno_occ) || !occ_ok(t.teno_occ_reuse)) fail("TENO occupancy target must be 0, 5 or 6");
     tune_ = t;
     tuned_ = true;
     drop_graph();
@@ -835,6 +837,16 @@ void Solver::autotune() {
         }
         tune_.teno_occ = best.teno_occ;
         HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, tune_.teno_occ, 0, stream_));
+        if (gate_ && !gate_per_stage_) {  // passes that reuse the decisions just recorded
+            bt = 1e30f;
+            for (int occ : {0, 5, 6}) {
+                if (occ != 0 && !occ_tunable) continue;
+                const float t = timed([&] {
+                    HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, occ, 1, stream_));
+                });
+                if (t < bt) { bt = t; best.teno_occ_reuse = occ; }
+            }
+        }
         if (saved) {
             HIP_OK(hipMemcpyAsync(gated_, saved, gated_slots_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
             synchronize();

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 EXTRAP = [(z, "extrapolation", {}) for z in ("left", "right", "top", "bottom")]
 RIEMANN2D = np.array([[1.5, 0.0, 0.0, 1.5], [0.5323, 1.206, 0.0, 0.3],
                       [0.138, 1.206, 1.206, 0.029], [0.5323, 0.0, 1.206, 0.3]])
-SHAPES = [(0, 256, 256), (5, 64, 128), (6, 128, 64)]
+SHAPES = [(0, 256, 256, 0), (5, 64, 128, 6), (6, 128, 64, 5)]
 
 
 def _num(mlr):
@@ -53,19 +53,19 @@ def test_autotune_leaves_state_and_counts(mlr):
     s = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", 40, 40, 1.0, 1.0, 0.2, 9), num=_num(mlr))
     s.init_quadrant(0.5, 0.5, RIEMANN2D)
     s.set_indicator(_half_gate(), 0.9)
-    s.set_tune(0, 256, 256)
+    s.set_tune(0, 256, 256, 0)
     s.run_steps(3)
     U0, g0 = s.state()[0], s.gated()
-    occ, fb, sb = s.autotune()
-    assert occ in (0, 5, 6) and fb in (64, 128, 256) and sb in (64, 128, 256)
-    assert s.tune() == (occ, fb, sb)
+    occ, fb, sb, occ2 = s.autotune()
+    assert occ in (0, 5, 6) and occ2 in (0, 5, 6) and fb in (64, 128, 256) and sb in (64, 128, 256)
+    assert s.tune() == (occ, fb, sb, occ2)
     assert np.array_equal(s.state()[0], U0)
     assert s.gated() == g0
     # and the steps after it equal the steps of an untuned twin
     t = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", 40, 40, 1.0, 1.0, 0.2, 9), num=_num(mlr))
     t.init_quadrant(0.5, 0.5, RIEMANN2D)
     t.set_indicator(_half_gate(), 0.9)
-    t.set_tune(0, 256, 256)
+    t.set_tune(0, 256, 256, 0)
     t.run_steps(3)
     s.run_steps(4)
     t.run_steps(4)
@@ -79,12 +79,12 @@ def test_ensemble_shapes_give_identical_results(mlr):
     gens = [mlr.mesh_gen("cartesian_tri", n, n, 1.0, 1.0, 0.2, 50 + n) for n in sizes]
     states = [ind.random_quadrant_states(rng) for _ in sizes]
     res = []
-    for occ, bs in [(0, 256), (6, 64), (5, 128)]:
+    for occ, bs, occ2 in [(0, 256, 0), (6, 64, 5), (5, 128, 6)]:
         ens = mlr.Ensemble(gens, _num(mlr))
         for k in range(len(gens)):
             ens.init_quadrant(k, 0.5, 0.5, states[k])
         ens.set_indicator(_half_gate(), 0.9)
-        ens.set_tune(occ, bs)
+        ens.set_tune(occ, bs, occ2)
         ens.run_steps(5)
         res.append([ens.state(k)[0] for k in range(len(gens))])
         res[-1].append(ens.gated())
@@ -96,5 +96,5 @@ def test_ensemble_shapes_give_identical_results(mlr):
     ens = mlr.Ensemble(gens, _num(mlr))
     for k in range(len(gens)):
         ens.init_quadrant(k, 0.5, 0.5, states[k])
-    occ, bs, bs2 = ens.autotune()
-    assert occ in (0, 5, 6) and bs in (64, 128, 256) and bs2 == bs
+    occ, bs, bs2, occ2 = ens.autotune()
+    assert occ in (0, 5, 6) and occ2 in (0, 5, 6) and bs in (64, 128, 256) and bs2 == bs
