@@ -72,6 +72,23 @@ def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes):
     return alg, layout
 
 
+def learned_roofline(learned, N, dev):
+    """The gated TENO kernel of the learned run against HBM peak. Algorithmic
+    bytes per launch: a gated cell needs only its central stencil (the
+    K x M pseudo-inverse, M ids and M areas), an ungated one all its
+    non-empty stencils, every cell its state; averaged with the measured gated
+    fraction."""
+    if learned is None:
+        return None
+    per_st = 8 * dev.K * dev.M + 12 * dev.M
+    g = learned["gated_fraction"]
+    alg = 32 * N + per_st * (g * N + (1.0 - g) * dev.n_stencils)
+    achieved = alg / (learned["teno_ms"] * 1e-3) / 1e9
+    return {"kernel": "teno_kernel (learned gate; network evaluated once per step)", "bound": "hbm",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "alg_bytes_per_launch": alg, "avg_launch_ms": learned["teno_ms"]}
+
+
 def check_state(dev, what):
     U, W = dev.state()
     bad = ~np.isfinite(U).all(axis=1) | (U[:, 0] <= 0) | (W[:, 2] <= 0)
@@ -466,6 +483,7 @@ def main():
             "stream_copy_GBs": stream_copy,
             "stream_read_GBs": stream_read,
         },
+        "roofline_learned": learned_roofline(learned, N, dev),
         "kernels_ms": {"teno": teno_ms, "flux_update": flux_ms,
                        "spectral_dt": st["dt_ms"] / max(st["dt_n"], 1),
                        "copy": st["copy_ms"] / max(st["copy_n"], 1) if st["copy_n"] else None},
