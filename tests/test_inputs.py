@@ -105,3 +105,26 @@ def test_expression_errors(mlr):
         mlr.eval_expr("var q := 1; z + q", [0.0], [0.0])
     with pytest.raises(mlr.MallardError):
         mlr.eval_expr("erfc(x)", [0.0], [0.0])
+
+
+def test_navier_stokes_keys(mlr, tmp_path):
+    """[physics] type = "navier-stokes" and the no-slip wall (extension keys)."""
+    base = open(os.path.join(HERE, "wedge.toml")).read()
+    ns = base.replace('type = "euler"', 'type = "navier-stokes"\nmu_ref = 1.8e-5\nsutherland_S = 110.4\nPr = 0.71')
+    ns = ns.replace('name = "bottom"\ntype = "symmetry"', 'name = "bottom"\ntype = "wall_noslip"')
+    assert ns != base
+    p = tmp_path / "ns.toml"
+    p.write_text(ns)
+    g, num, run, init = mlr.parse_config(p)
+    assert num.viscous == 1 and num.mu_ref == 1.8e-5 and num.sutherland_S == 110.4 and num.prandtl == 0.71
+    assert num.T_mu_ref == 298.15  # defaults to T_ref
+    b = {num.bc[i].zone.decode(): num.bc[i] for i in range(num.n_bc)}
+    assert b["bottom"].type == mlr.BC["wall_noslip"]
+    g, num, run, init = mlr.parse_config(os.path.join(HERE, "wedge.toml"))
+    assert num.viscous == 0
+    p.write_text(ns.replace("mu_ref = 1.8e-5\n", ""))
+    with pytest.raises(mlr.MallardError, match="Missing mu_ref for physics: navier-stokes."):
+        mlr.parse_config(p)
+    p.write_text(base.replace('type = "euler"', 'type = "stokes"'))
+    with pytest.raises(mlr.MallardError, match="Unknown physics type: stokes."):
+        mlr.parse_config(p)
