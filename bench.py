@@ -60,7 +60,7 @@ def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes):
     alg: each compulsory element counted once, per non-empty stencil the
     reference's K x M pseudo-inverse (8 B), M ids (4 B) and M transformed
     areas (8 B), plus the cell's own state (32 B).
-    layout: what this build's packed layout must move at least: the
+    layout: what the packed device layout must move at least: the
     (K-1) x (M-1) non-zero block of the pseudo-inverse, M-1 areas and in-tile
     positions, the tile lists (4 B per entry), the cell state, its info word,
     the face-point basis values, the face states written (S*Q points x 4

@@ -4,7 +4,8 @@
  * AoS Kokkos LayoutRight layout, atomics in the multi-threaded mode).
  *
  * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg as the checker and the timed CPU baseline.
+ * bench.py's cpu_baseline leg, as the reference the HIP path is compared
+ * with and as the timed CPU baseline.
  *
  * With one thread the residual is accumulated in the canonical order of a
  * one-thread reference run (interior faces in ascending id, then each boundary

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY. This is a clean-room FP64 restatement of the
  * reference algorithm (reference numbering, AoS state, atomics in the
- * OpenMP mode) used as the checker for the HIP path and as the timed CPU
+ * OpenMP mode) that the tests compare the HIP path with, and the timed CPU
  * baseline. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg may load it. The product library never links or calls it.
  *

@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg. It is the checker, never the product path.
+bench.py's cpu_baseline leg. It is what results are compared with, never
+the product path.
 """
 import ctypes as C
 import os
