@@ -150,3 +150,30 @@ def test_trained_indicator_gains_at_matched_l1(mlr, every_stage):
     assert np.isfinite(Ub).all()
     assert frac > 0.5
     assert l1.max() <= 1e-3
+
+
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("every_stage", [False, True])
+def test_gate_never_firing_is_plain_at_each_order(mlr, order, every_stage):
+    """The gated kernels of every stencil size (p = 2: 12 members, the
+    unrolled kernel; p = 3: 20 members, the runtime-loop kernel) leave the
+    solution bit-identical to plain TENO when the network never fires."""
+    num = mlr.numerics(recon="TENO", order=order, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
+    gen = mlr.mesh_gen("cartesian_tri", 24, 24, 1.0, 1.0, 0.2, 19)
+    a, b = mlr.Solver(gen=gen, num=num), mlr.Solver(gen=gen, num=num)
+    for s in (a, b):
+        s.init_quadrant(0.5, 0.5, RIEMANN2D)
+    th = np.zeros(mlr.N_PARAMS, np.float32)
+    th[ir.OFF_B3] = -50.0
+    b.set_indicator(th, 0.9, every_stage=every_stage)
+    a.run_steps(6)
+    b.run_steps(6)
+    assert np.array_equal(a.state()[0], b.state()[0])
+    assert b.gated() == 0
+
+
+def test_gate_needs_a_central_stencil_of_twelve(mlr):
+    num = mlr.numerics(recon="TENO", order=1, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
+    s = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", 8, 8, 1.0, 1.0, 0.2, 1), num=num)
+    with pytest.raises(mlr.MallardError, match="central stencils of at least 12 cells"):
+        s.set_indicator(np.zeros(mlr.N_PARAMS, np.float32), 0.9)
