@@ -117,3 +117,40 @@ def test_integrators_constant_rhs(oracle, integrator):
     rhs = np.arange(8, dtype=float)
     oracle.lib().orc_integrator_constant_rhs(oracle.INTEGRATOR[integrator], 0.1, oracle._p(U), 8, oracle._p(rhs))
     assert np.allclose(U, 1.1 * np.arange(8), atol=1e-6, rtol=0)
+
+
+def _phys_flux(n, rho, u, p, g):
+    un = u[0] * n[0] + u[1] * n[1]
+    E = p / (g - 1.0) / rho + 0.5 * (u[0] ** 2 + u[1] ** 2)
+    return np.array([rho * un, rho * u[0] * un + p * n[0], rho * u[1] * un + p * n[1], (rho * E + p) * un])
+
+
+def test_roe_textbook_properties(oracle):
+    """Roe's flux (no reference counterpart) pinned by properties of the
+    exact Roe linearisation: a stationary contact passes no mass or energy
+    and only its pressure as momentum; a supersonic pair whose waves all run
+    one way is the upwind side's physical flux; the flux is rotation
+    invariant (normal and tangential components follow the normal)."""
+    g = 1.4
+    # stationary contact: density jump, equal pressure, no velocity
+    F = oracle.riemann_flux("Roe", [1.0, 0.0], 1.0, [0.0, 0.0], 1.0, g, _h(g, 1.0, 1.0), 0.3, [0.0, 0.0], 1.0, g,
+                            _h(g, 0.3, 1.0))
+    assert abs(F[0]) < 1e-15 and abs(F[3]) < 1e-15 and abs(F[1] - 1.0) < 1e-15 and abs(F[2]) < 1e-15
+    # supersonic to the right (all eigenvalues > 0): the left state's flux
+    uL, uR = [3.5, 0.2], [3.2, -0.1]
+    F = oracle.riemann_flux("Roe", [1.0, 0.0], 1.0, uL, 1.0, g, _h(g, 1.0, 1.0), 0.8, uR, 0.7, g, _h(g, 0.8, 0.7))
+    assert np.allclose(F, _phys_flux([1.0, 0.0], 1.0, uL, 1.0, g), rtol=1e-12, atol=1e-13)
+    # ... and to the left: the right state's flux
+    uL, uR = [-3.5, 0.2], [-3.2, -0.1]
+    F = oracle.riemann_flux("Roe", [1.0, 0.0], 1.0, uL, 1.0, g, _h(g, 1.0, 1.0), 0.8, uR, 0.7, g, _h(g, 0.8, 0.7))
+    assert np.allclose(F, _phys_flux([1.0, 0.0], 0.8, uR, 0.7, g), rtol=1e-12, atol=1e-13)
+    # rotation invariance: rotate states and normal together
+    th = 0.7
+    c, s = np.cos(th), np.sin(th)
+    rot = lambda v: [c * v[0] - s * v[1], s * v[0] + c * v[1]]
+    uL, uR = [0.4, 0.1], [-0.2, 0.3]
+    F0 = oracle.riemann_flux("Roe", [1.0, 0.0], 1.0, uL, 1.0, g, _h(g, 1.0, 1.0), 0.5, uR, 0.4, g, _h(g, 0.5, 0.4))
+    F1 = oracle.riemann_flux("Roe", rot([1.0, 0.0]), 1.0, rot(uL), 1.0, g, _h(g, 1.0, 1.0), 0.5, rot(uR), 0.4, g,
+                             _h(g, 0.5, 0.4))
+    assert abs(F1[0] - F0[0]) < 1e-13 and abs(F1[3] - F0[3]) < 1e-13
+    assert np.allclose([F1[1], F1[2]], rot([F0[1], F0[2]]), atol=1e-13)
