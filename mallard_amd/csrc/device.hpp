@@ -76,6 +76,10 @@ struct Teno {
     ind::Net net;
     uint32_t* gated;
     uint8_t* gate_dec;  // per cell: 1 when the last network evaluation judged it smooth
+    // gated runs: the central stencil's linear reconstruction as one operator,
+    // lin[N/64][S*Q][M-1][64]: face point fq of cell c is
+    // u_c + sum_i lin(fq, i) (u_i - u_c) over the stencil's members i
+    const double* lin;
     double gamma;
 };
 
@@ -186,6 +190,10 @@ struct PreInput {
     double ipt[10];
 };
 bool aplus_supported(int K, int M);
+// lin(fq, i) = (sum_k fbasis(fq, k) A+(central, i, k)) ta(central, i), 0 where
+// a cell has no central stencil (the blocked layouts of the Teno tables)
+hipError_t teno_linear_operator(const Teno& t, const uint32_t* info, uint32_t N, int FQ, double* lin,
+                                hipStream_t stream);
 // A: [N/64][4][M-1][K-1][64], ta: [N/64][4][M-1][64]; err |= 1 singular
 // frame, |= 2 a stencil needing the full-matrix branch
 hipError_t teno_aplus(const PreInput& in, int K, int M, double* A, double* ta, int* err, hipStream_t stream);

@@ -400,8 +400,6 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
 
     const uint32_t info0 = g.info[cc];
     const D4 uc = ld4(U, cc);
-    double ws[4];
-    double at[4][KP];
     const uint32_t cl = c & 63u;
     // the member update a += A+_i (ta_i (u_i - u_c)) of stencil row0's member i
     auto member = [&](size_t row0, int i, double (&a)[KP][4]) {
@@ -422,44 +420,59 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         }
     };
 
-    if (GM != 0 && t.gate && __all(smooth || !valid)) {
-        // ---- every cell of the wave takes its central stencil: the quad
-        // splits its members (lane q: members q, q+4, q+8, ...), then a
-        // reduce-scatter over the quad leaves lane v the coefficients of
-        // variable v, summed as (lanes 0 + 2) + (lanes 1 + 3).
-        smooth = true;  // lanes past the end too: they store nothing
-        double a[KP][4];
+    const double ucv = sel4(uc.v, q);  // lane-dependent pick by selects, not a stack array
+    double f[FQ];
+    bool central = false;
+    // the central stencil's linear reconstruction through its precombined
+    // operator (gated cells): quad lane q sums face points q and q + 4 for
+    // all 4 variables over the members, a quad transpose hands lane v its
+    // variable's face points
+    auto linear_faces = [&]() {
+        double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+        const bool has0 = q < FQ, has1 = q + 4 < FQ;
+        if (valid && has0) {
+            const size_t row0 = (size_t)lb * 4 * Mp;  // central stencil rows of this block
+            const double* Lw = t.lin + ((size_t)lb * FQ) * Mp * 64 + cl;
+            auto add = [&](int i) {
+                const uint32_t pos = loc[(row0 + i) * 64 + cl];
+                const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
+                const double d[4] = {u01.x - uc.v[0], u01.y - uc.v[1], u23.x - uc.v[2], u23.y - uc.v[3]};
+                const double w0 = Lw[((size_t)q * Mp + i) * 64];
 #pragma unroll
-        for (int k = 0; k < KP; ++k)
+                for (int v = 0; v < 4; ++v) acc[0][v] = acc[0][v] + w0 * d[v];
+                if (has1) {
+                    const double w1 = Lw[((size_t)(q + 4) * Mp + i) * 64];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) a[k][v] = 0.0;
-        if (valid && (info0 >> 20) & 1u) {
-            const size_t row0 = (size_t)lb * 4 * Mp;
+                    for (int v = 0; v < 4; ++v) acc[1][v] = acc[1][v] + w1 * d[v];
+                }
+            };
             if constexpr (MP > 0) {
 #pragma unroll
-                for (int j = 0; j < (MP + 3) / 4; ++j)
-                    if (q + 4 * j < MP) member(row0, q + 4 * j, a);
+                for (int i = 0; i < MP; ++i) add(i);
             } else {
-                for (int i = q; i < Mp; i += 4) member(row0, i, a);
+                for (int i = 0; i < Mp; ++i) add(i);
             }
         }
-        const bool hi = (q & 2) != 0, odd = (q & 1) != 0;
+        double y[4];
+        quad_transpose(acc[0], y, q);
 #pragma unroll
-        for (int k = 0; k < KP; ++k) {
-            // lanes q, q^2: keep variables 2 (q >> 1) + {0, 1}, send the other two
-            const double k0 = hi ? a[k][2] : a[k][0], k1 = hi ? a[k][3] : a[k][1];
-            const double s0 = hi ? a[k][0] : a[k][2], s1 = hi ? a[k][1] : a[k][3];
-            const double p0 = k0 + qperm<kRot2>(s0), p1 = k1 + qperm<kRot2>(s1);
-            // lanes q, q^1: keep variable q
-            const double kk = odd ? p1 : p0, ss = odd ? p0 : p1;
-            at[0][k] = kk + qperm<kXor1>(ss);
+        for (int fq = 0; fq < (FQ < 4 ? FQ : 4); ++fq) f[fq] = ucv + y[fq];
+        if constexpr (FQ > 4) {
+            quad_transpose(acc[1], y, q);
 #pragma unroll
-            for (int st = 1; st < 4; ++st) at[st][k] = 0.0;
+            for (int fq = 4; fq < FQ; ++fq) f[fq] = ucv + y[fq - 4];
         }
-#pragma unroll
-        for (int st = 0; st < 4; ++st) ws[st] = 0.0;
+        central = true;
+    };
+
+    if (GM != 0 && t.gate && __all(smooth || !valid)) {
+        // every cell of the wave takes its central stencil
+        smooth = true;  // lanes past the end too: they store nothing
+        linear_faces();
+    } else if (GM != 0 && smooth) {
+        linear_faces();  // a gated cell in a wave with ungated ones
     } else {
-        const bool active = valid && ((info0 >> (20 + q)) & 1u) && !(smooth && q != 0);
+        const bool active = valid && ((info0 >> (20 + q)) & 1u);
 
         // ---- phase A: lane q = stencil
         double a[KP][4];
@@ -477,23 +490,23 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
             } else {
                 for (int i = 0; i < Mp; ++i) member(row0, i, a);
             }
-            if (!smooth) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    double si = 0.0;
+            for (int v = 0; v < 4; ++v) {
+                double si = 0.0;
 #pragma unroll
-                    for (int k = 0; k < KP; ++k) {
-                        double oia = 0.0;
+                for (int k = 0; k < KP; ++k) {
+                    double oia = 0.0;
 #pragma unroll
-                        for (int j = 0; j < KP; ++j) oia = oia + t.OI[k * KP + j] * a[j][v];
-                        si = si + a[k][v] * oia;
-                    }
-                    w[v] = 1.0 / pow6_cr(si + 1.0e-12);
+                    for (int j = 0; j < KP; ++j) oia = oia + t.OI[k * KP + j] * a[j][v];
+                    si = si + a[k][v] * oia;
                 }
+                w[v] = 1.0 / pow6_cr(si + 1.0e-12);
             }
         }
 
         // ---- exchange: lane q = variable
+        double ws[4];
+        double at[4][KP];
         quad_transpose(w, ws, q);
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
@@ -502,64 +515,56 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
 #pragma unroll
             for (int st = 0; st < 4; ++st) at[st][k] = y[st];
         }
-    }
 
-    // ---- phase B: cut-off (face_reconstruction.cpp:948-981)
+        // ---- phase B: cut-off (face_reconstruction.cpp:948-981)
+        constexpr int G = 4;
+        const double CT1 = 1.0e-7, CT2 = 1.0e-5;
+        double sd = 0.0;
+#pragma unroll
+        for (int st = 1; st < G; ++st) sd += ws[st];
+        if (ws[0] / (sd + ws[0]) > CT1) {
+            central = true;
+            ws[0] = 1.0;
+#pragma unroll
+            for (int st = 1; st < G; ++st) ws[st] = 0.0;
+        } else {
+            if (t.variant != 0) ws[0] = 0.0;
+#pragma unroll
+            for (int st = 1; st < G; ++st) {
+                if (ws[st] / sd > CT2) ws[st] = (1.0 / t.K);
+                else if (t.variant != 0) ws[st] = 0.0;
+            }
+            double s2 = 0.0;
+#pragma unroll
+            for (int st = 1; st < G; ++st) s2 += ws[st];
+#pragma unroll
+            for (int st = 1; st < G; ++st) ws[st] /= s2;
+        }
+        double pw[G][KP];
+#pragma unroll
+        for (int st = 0; st < G; ++st)
+#pragma unroll
+            for (int k = 0; k < KP; ++k) pw[st][k] = ws[st] * at[st][k];
+
+        // face points (face_reconstruction.cpp:985-1031)
+#pragma unroll
+        for (int fq = 0; fq < FQ; ++fq) {
+            const double* B = t.fbasis + (((size_t)lb * FQ + fq) * KP) * 64 + (c & 63u);
+            double bk[KP];
+#pragma unroll
+            for (int k = 0; k < KP; ++k) bk[k] = B[k * 64];
+            double acc = ucv;
+#pragma unroll
+            for (int st = 0; st < G; ++st) {
+                if (ws[st] == 0.0) continue;
+                acc = acc + (ws[st] * 0.0) * t.basis0;
+#pragma unroll
+                for (int k = 0; k < KP; ++k) acc = acc + pw[st][k] * bk[k];
+            }
+            f[fq] = acc;
+        }
+    }
     const int v = q;
-    constexpr int G = 4;
-    const double CT1 = 1.0e-7, CT2 = 1.0e-5;
-    double sd = 0.0;
-#pragma unroll
-    for (int st = 1; st < G; ++st) sd += ws[st];
-    bool central = false;
-    if (smooth) {
-        central = true;
-        ws[0] = 1.0;
-#pragma unroll
-        for (int st = 1; st < G; ++st) ws[st] = 0.0;
-    } else if (ws[0] / (sd + ws[0]) > CT1) {
-        central = true;
-        ws[0] = 1.0;
-#pragma unroll
-        for (int st = 1; st < G; ++st) ws[st] = 0.0;
-    } else {
-        if (t.variant != 0) ws[0] = 0.0;
-#pragma unroll
-        for (int st = 1; st < G; ++st) {
-            if (ws[st] / sd > CT2) ws[st] = (1.0 / t.K);
-            else if (t.variant != 0) ws[st] = 0.0;
-        }
-        double s2 = 0.0;
-#pragma unroll
-        for (int st = 1; st < G; ++st) s2 += ws[st];
-#pragma unroll
-        for (int st = 1; st < G; ++st) ws[st] /= s2;
-    }
-    double pw[G][KP];
-#pragma unroll
-    for (int st = 0; st < G; ++st)
-#pragma unroll
-        for (int k = 0; k < KP; ++k) pw[st][k] = ws[st] * at[st][k];
-
-    // face points (face_reconstruction.cpp:985-1031)
-    const double ucv = sel4(uc.v, v);  // lane-dependent pick by selects, not a stack array
-    double f[FQ];
-#pragma unroll
-    for (int fq = 0; fq < FQ; ++fq) {
-        const double* B = t.fbasis + (((size_t)lb * FQ + fq) * KP) * 64 + (c & 63u);
-        double bk[KP];
-#pragma unroll
-        for (int k = 0; k < KP; ++k) bk[k] = B[k * 64];
-        double acc = ucv;
-#pragma unroll
-        for (int st = 0; st < G; ++st) {
-            if (ws[st] == 0.0) continue;
-            acc = acc + (ws[st] * 0.0) * t.basis0;
-#pragma unroll
-            for (int k = 0; k < KP; ++k) acc = acc + pw[st][k] * bk[k];
-        }
-        f[fq] = acc;
-    }
 
     if (t.variant != 0) {
         // positivity scaling of the cell's face states (Zhang & Shu 2010),

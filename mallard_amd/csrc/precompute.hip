@@ -210,7 +210,43 @@ __global__ __launch_bounds__(256) void aplus_kernel(PreInput in, double* __restr
         for (int k = 0; k < KP; ++k) A[((row0 + i) * KP + k) * 64 + cl] = X[k][i];
 }
 
+// thread = (block of 64 cells, face point fq, cell in block); the member loop
+// writes lin(fq, i) for every member of the cell's central stencil
+__global__ __launch_bounds__(256) void linear_operator_kernel(Teno t, const uint32_t* __restrict__ info, uint32_t N,
+                                                              int FQ, uint64_t total, double* __restrict__ lin) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const uint32_t cl = (uint32_t)(gid & 63u);
+    const uint64_t rest = gid >> 6;
+    const int fq = (int)(rest % (uint64_t)FQ);
+    const uint64_t blk = rest / (uint64_t)FQ;
+    const uint32_t c = (uint32_t)(blk * 64 + cl);
+    const int KP = t.K - 1, Mp = t.M - 1;
+    const bool on = c < N && ((info[c] >> 20) & 1u);
+    const double* B = t.fbasis + ((blk * FQ + fq) * KP) * 64 + cl;
+    for (int i = 0; i < Mp; ++i) {
+        double w = 0.0;
+        if (on) {
+            const size_t row = blk * 4 * Mp + i;  // central stencil, member i
+            const double* Ai = t.A + row * (KP * 64) + cl;
+            double acc = 0.0;
+            for (int k = 0; k < KP; ++k) acc = acc + B[k * 64] * Ai[k * 64];
+            w = acc * t.ta[row * 64 + cl];
+        }
+        lin[((blk * FQ + fq) * Mp + i) * 64 + cl] = w;
+    }
+}
+
 }  // namespace
+
+hipError_t teno_linear_operator(const Teno& t, const uint32_t* info, uint32_t N, int FQ, double* lin,
+                                hipStream_t stream) {
+    if (FQ < 1 || FQ > 8 || t.M < 2) return hipErrorInvalidValue;
+    const uint64_t total = (uint64_t)((N + 63) / 64) * 64 * FQ;
+    hipLaunchKernelGGL(linear_operator_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, t, info,
+                       N, FQ, total, lin);
+    return hipGetLastError();
+}
 
 bool aplus_supported(int K, int M) { return (K == 3 && M == 6) || (K == 6 && M == 12) || (K == 10 && M == 20); }
 

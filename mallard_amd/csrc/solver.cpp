@@ -491,6 +491,7 @@ dev::Teno Solver::teno() const {
     t.gamma = ph_.gamma;
     t.gated = gated_;
     t.gate_dec = gate_dec_;
+    t.lin = lin_;
     t.gate = gate_ ? 1 : 0;
     if (gate_) t.net = dev::net_view(net_buf_, gate_thr_, nullptr);
     return t;
@@ -521,6 +522,12 @@ void Solver::set_indicator(const float* params, double threshold) {
         HIP_OK(hipMalloc((void**)&gate_dec_, L_.N));
         allocs_.push_back(gate_dec_);
         HIP_OK(hipMemset(gate_dec_, 0, L_.N));
+        // the central stencils' linear operators the gated cells use
+        const size_t nl = L_.n_pad() * (size_t)L_.S * L_.Q * (size_t)(L_.M - 1);
+        HIP_OK(hipMalloc((void**)&lin_, nl * sizeof(double)));
+        allocs_.push_back(lin_);
+        dev_bytes_ += nl * sizeof(double);
+        HIP_OK(dev::teno_linear_operator(teno(), info_, L_.N, (int)(L_.S * L_.Q), lin_, stream_));
     }
     HIP_OK(hipMemcpyAsync(theta_, params, kIndParams * sizeof(float), hipMemcpyHostToDevice, stream_));
     HIP_OK(dev::net_pack(theta_, net_buf_, stream_));

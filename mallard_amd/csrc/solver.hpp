@@ -207,6 +207,7 @@ private:
     int64_t* step_dev_ = nullptr;
     uint32_t* gated_ = nullptr;  // per-wave-slot smooth counts of the gated TENO kernel
     uint8_t* gate_dec_ = nullptr;  // per-cell decisions of the last network evaluation
+    double* lin_ = nullptr;        // central stencils' linear operators (gated cells)
     bool gate_per_stage_ = false;
     size_t gated_slots_ = 0;
     void* net_buf_ = nullptr;

@@ -111,18 +111,33 @@ def test_gate_that_never_fires_is_plain_teno(mlr, every_stage):
 
 def test_gate_that_always_fires_on_smooth_field_is_plain_teno(mlr):
     """Where plain TENO selects the central stencil everywhere, forcing it
-    changes nothing, bit for bit; every cell of every pass is counted."""
+    changes nothing but rounding: the gated cells evaluate the central
+    stencil through its precombined linear operator (face-point basis x A+ x
+    areas) instead of coefficients and basis. Every cell of every pass is
+    counted, and a uniform state stays bit-identical."""
     num = mlr.numerics(recon="TENO", order=2, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
     gen = mlr.mesh_gen("cartesian_tri", 32, 32, 1.0, 1.0, 0.2, 12)
     a, b = mlr.Solver(gen=gen, num=num), mlr.Solver(gen=gen, num=num)
-    for s in (a, b):
-        s.init_constant([0.4, -0.3], 1.0e5, 290.0)
     th = np.zeros(mlr.N_PARAMS, np.float32)
     th[ir.OFF_B3] = 50.0
     b.set_indicator(th, 0.9)
+    # uniform: the first stage sees exactly equal members
+    for s in (a, b):
+        s.init_constant([0.4, -0.3], 1.0e5, 290.0)
+    assert np.array_equal(a.calc_rhs(), b.calc_rhs())
+    # smooth: plain TENO takes the central stencil in every cell
+    lay = mlr.Layout(gen, mlr.numerics(recon="FO", bcs=EXTRAP))
+    cc = lay.mesh_arrays()["cell_coords"]
+    rho = 1.0 + 0.01 * np.sin(2 * np.pi * cc[:, 0]) * np.cos(2 * np.pi * cc[:, 1])
+    U = np.stack([rho, 0.3 * rho, -0.2 * rho, 1.0 / 0.4 + 0.5 * rho * 0.13], axis=1)
+    for s in (a, b):
+        s.set_state(U)
     a.run_steps(5)
+    b.gated(reset=True)
     b.run_steps(5)
-    assert np.array_equal(a.state()[0], b.state()[0])
+    assert (a.decisions() == 0xF).all()
+    Ua, Ub = a.state()[0], b.state()[0]
+    assert np.abs(Ua - Ub).max() <= 1e-13 * np.abs(Ua).max()
     assert b.gated() == a.n_cells * 3 * 5
 
 
