@@ -74,10 +74,12 @@ def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes):
 
 def learned_roofline(learned, N, dev):
     """The gated TENO kernel of the learned run against HBM peak. Algorithmic
-    bytes per launch: a gated cell needs only its central stencil (the
-    K x M pseudo-inverse, M ids and M areas), an ungated one all its
-    non-empty stencils, every cell its state; averaged with the measured gated
-    fraction."""
+    bytes per launch, in the learned form's accounting: a gated cell needs
+    only its central stencil (the K x M pseudo-inverse, M ids and M areas),
+    an ungated one all its non-empty stencils, every cell its state; averaged
+    with the measured gated fraction. (The kernel reads less for a gated
+    cell: its precombined linear operator, S*Q x (M-1) values, in place of
+    the pseudo-inverse, areas and face basis.)"""
     if learned is None:
         return None
     per_st = 8 * dev.K * dev.M + 12 * dev.M
