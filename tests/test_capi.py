@@ -49,7 +49,51 @@ def test_host_pow_tracks_glibc(mlr):
     g = np.array([math.pow(u, v) for u, v in zip(b, e)])
     ulps = np.abs(h.view(np.int64) - g.view(np.int64))
     assert ulps.max() <= 1 and np.mean(ulps > 0) < 0.005
-    # IEEE special operands follow the platform pow
-    sp = np.array([0.0, -2.0, np.inf, np.nan, 1.0])
-    assert np.array_equal(mlr.host_pow(sp, np.full(5, 2.5)), np.array([math.pow(v, 2.5) if v >= 0 or np.isnan(v) else np.nan
-                                                                      for v in sp]), equal_nan=True)
+
+
+def _glibc_pow(x, y):
+    """C pow from the C library itself (Python's math.pow raises where C99
+    returns infinities and NaNs)."""
+    import ctypes
+    import ctypes.util
+    libm = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+    f = libm.pow
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_double, ctypes.c_double]
+    return np.array([f(float(a), float(b)) for a, b in zip(x, y)])
+
+
+SPECIAL_X = np.array([0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 0.5, -0.5, np.inf, -np.inf, np.nan, 5e-324, -5e-324,
+                      1e-310, 1e300, -1e300, 3.0, -3.0])
+SPECIAL_Y = np.array([0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 3.0, -3.0, 2.5, -2.5, np.inf, -np.inf, np.nan, 1e300,
+                      -1e300, 2.0 ** 53, 2.0 ** 53 + 2, 7.000000000000001, 0.5, 1e-320])
+
+
+def special_pow_operands():
+    """Every pairing of zeros of both signs, +-1, infinities, NaN,
+    subnormals, huge values, odd, even and non-integer exponents."""
+    X, Y = np.meshgrid(SPECIAL_X, SPECIAL_Y)
+    return X.ravel(), Y.ravel()
+
+
+def test_host_pow_special_operands_follow_c99(mlr):
+    """Special operands are handled case by case (C99 Annex F.9.4.4) and give
+    glibc's results bit for bit, signed zeros, infinities and NaNs included;
+    subnormal bases, negative bases with integer exponents and results near
+    overflow stay within 1 ulp of glibc, results in the subnormal range
+    within 1 unit of 2^-1074."""
+    x, y = special_pow_operands()
+    h, g = mlr.host_pow(x, y), _glibc_pow(x, y)
+    same = (h.view(np.int64) == g.view(np.int64)) | (np.isnan(h) & np.isnan(g))
+    assert same.all(), list(zip(x[~same], y[~same], h[~same], g[~same]))
+    rng = np.random.default_rng(11)
+    cases = [(-np.exp(rng.uniform(-5, 5, 4000)), rng.integers(-40, 40, 4000).astype(float)),
+             (rng.uniform(1e-320, 2.2e-308, 4000), rng.uniform(0.01, 1.0, 4000)),
+             (np.exp(rng.uniform(700, 712, 4000) / 7), np.full(4000, 7.0)),
+             (np.exp(-rng.uniform(700, 750, 4000) / 7), np.full(4000, 7.0)),
+             (np.exp(rng.uniform(-700, 700, 4000)), rng.uniform(-3, 3, 4000))]
+    for bx, by in cases:
+        h, g = mlr.host_pow(bx, by), _glibc_pow(bx, by)
+        assert (np.isinf(h) == np.isinf(g)).all()
+        fin = np.isfinite(g)
+        assert np.abs(h[fin].view(np.int64) - g[fin].view(np.int64)).max() <= 1

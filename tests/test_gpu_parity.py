@@ -57,6 +57,12 @@ def test_device_math_matches_ieee(mlr):
     ulps = np.abs(d.view(np.int64) - g.view(np.int64))
     assert ulps.max() <= 1
     assert np.mean(ulps > 0) < 0.005
+    # special operands: the device takes the same case-by-case branches
+    from test_capi import special_pow_operands
+    sx, sy = special_pow_operands()
+    sd, sh = mlr.probe_math(2, sx, sy), mlr.host_pow(sx, sy)
+    assert np.array_equal(sd.view(np.int64), sh.view(np.int64)) or np.array_equal(sd, sh, equal_nan=True)
+    assert (np.isnan(sd) == np.isnan(sh)).all() and (np.signbit(sd) == np.signbit(sh))[~np.isnan(sd)].all()
     native = mlr.probe_math(4, b, e)
     print("pow vs glibc: double-double %.4f%%, platform pow %.4f%% differ by 1 ulp"
           % (100.0 * np.mean(ulps > 0), 100.0 * np.mean(native != g)))
