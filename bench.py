@@ -151,7 +151,7 @@ def learned_run(args, mlr, dev, U_plain, value_plain, wall_plain):
     N = dev.n_cells
 
     def timed(every_stage):
-        dev.set_indicator(params, args.threshold, every_stage=every_stage)
+        dev.set_indicator(params, args.threshold, every_stage=every_stage, split=bool(args.gate_split))
         dev.init_quadrant(0.5, 0.5, QUADRANT)
         tune = dev.autotune()  # the gated kernel's shapes, before the timed region
         dev.gated(reset=True)
@@ -176,7 +176,9 @@ def learned_run(args, mlr, dev, U_plain, value_plain, wall_plain):
     out["every_stage"] = {k: every[k] for k in ("value", "ms_per_step", "speedup", "l1_rel", "gated_fraction",
                                                 "teno_ms")}
     out.update({"train_s": train_s, "train_rows": int(len(y)), "train_loss": float(hist[-50:].mean()),
-                "threshold": args.threshold})
+                "threshold": args.threshold,
+                "gate_kernels": "split: gated cells, then the ungated cells' blocks" if args.gate_split
+                else "one kernel for gated and ungated cells"})
     return out
 
 
@@ -357,6 +359,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-indicator run")
     ap.add_argument("--train-cases", type=int, default=512)
     ap.add_argument("--threshold", type=float, default=0.9)
+    ap.add_argument("--gate-split", type=int, default=1, choices=[0, 1],
+                    help="1: gated passes run as two kernels (gated cells, then ungated); 0: one kernel")
     ap.add_argument("--save-indicator", default=None, help="save the trained indicator parameters (.npy)")
     ap.add_argument("--indicator-file", default=None,
                     help="use saved indicator parameters instead of training (profiling runs)")

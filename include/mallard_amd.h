@@ -308,6 +308,11 @@ int mlr_set_indicator(mlr_ctx* ctx, const float* params, double threshold);
  * the step's first stage, and the later stages reuse its decisions;
  * 1: every stage evaluates it */
 int mlr_indicator_cadence(mlr_ctx* ctx, int every_stage);
+/* 1 (default): a gated TENO pass runs as two kernels, one for the cells the
+ * network gated (no full-TENO code, so it keeps few registers) and one for
+ * the other cells, in the 64-cell blocks that hold any; 0: one kernel takes
+ * both kinds. The results are identical. */
+int mlr_indicator_split(mlr_ctx* ctx, int split);
 int mlr_ensemble_indicator_cadence(mlr_ensemble* ens, int every_stage);
 /* cells judged smooth (summed over every gated TENO pass) since the last reset */
 int mlr_indicator_gated(mlr_ctx* ctx, int reset, int64_t* gated);

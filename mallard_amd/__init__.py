@@ -109,6 +109,7 @@ _SIGS = {
     "mlr_ensemble_run_steps_profiled": (C.c_int, [_P, C.c_int64, _P]),
     "mlr_ensemble_autotune": (C.c_int, [_P, _P]),
     "mlr_indicator_cadence": (C.c_int, [_P, C.c_int]),
+    "mlr_indicator_split": (C.c_int, [_P, C.c_int]),
     "mlr_ensemble_indicator_cadence": (C.c_int, [_P, C.c_int]),
     "mlr_ensemble_get_volumes": (C.c_int, [_P, C.c_int, _P]),
     "mlr_ensemble_set_tune": (C.c_int, [_P, _P]),
@@ -433,10 +434,12 @@ class Solver:
         self._check(lib().mlr_write_vtu(self.h, os.fspath(path).encode(), names, len(variables)))
 
     # ---- learned troubled-cell indicator
-    def set_indicator(self, params, threshold=0.9, every_stage=False):
+    def set_indicator(self, params, threshold=0.9, every_stage=False, split=True):
         """Gate TENO with the network (params: 7361 FP32 values; None: off).
         every_stage False: the network judges the cells on each step's first
-        stage and the later stages reuse the decisions; True: every stage."""
+        stage and the later stages reuse the decisions; True: every stage.
+        split True: gated and ungated cells are reconstructed by two kernels;
+        False: by one (the results are identical)."""
         if params is None:
             self._check(lib().mlr_set_indicator(self.h, None, threshold))
         else:
@@ -445,6 +448,7 @@ class Solver:
                 raise ValueError("indicator parameters must have %d values" % N_PARAMS)
             self._check(lib().mlr_set_indicator(self.h, _p(p), threshold))
             self._check(lib().mlr_indicator_cadence(self.h, int(every_stage)))
+            self._check(lib().mlr_indicator_split(self.h, int(split)))
 
     def gated(self, reset=False):
         n = C.c_int64()

@@ -855,6 +855,11 @@ int mlr_indicator_cadence(mlr_ctx* ctx, int every_stage) {
     return guarded(ctx, [&] { ctx->solver.set_indicator_cadence(every_stage != 0); });
 }
 
+int mlr_indicator_split(mlr_ctx* ctx, int split) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.set_indicator_split(split != 0); });
+}
+
 int mlr_ensemble_indicator_cadence(mlr_ensemble* ens, int every_stage) {
     if (!ens) return 1;
     return guarded(nullptr, [&] { ens->e->set_indicator_cadence(every_stage != 0); });

@@ -81,6 +81,10 @@ struct Teno {
     // u_c + sum_i lin(fq, i) (u_i - u_c) over the stencil's members i
     const double* lin;
     double gamma;
+    // non-null: gated passes run split (teno_kernel modes 3/4, then 5 for
+    // the ungated cells); one byte per wave slot (block * 4 + wave), 1 when
+    // the wave holds a valid cell the network did not gate
+    uint8_t* blk_mixed;
 };
 
 // Stage programs of the fused residual + SSP-RK update.

@@ -315,19 +315,31 @@ __device__ inline void quad_gather(double f, double* out, int q) {
 // mapped so that each of the 8 XCDs walks one contiguous range of the
 // space-filling-curve order.
 // GM: 0 plain TENO; 1 the learned gate evaluates the network and records
-// its decisions; 2 the gate takes the recorded decisions (no network code)
+// its decisions; 2 the gate takes the recorded decisions (no network code).
+// Split passes (no full-TENO code in the gated kernels, so they keep few
+// registers): 3 evaluates like 1 and 4 reuses like 2, but both reconstruct
+// only the gated cells and mark the waves holding ungated ones; 5 then
+// reconstructs only the ungated cells, in the marked blocks.
 template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM>
 __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const double* __restrict__ U,
                                            double* __restrict__ fs, uint8_t* __restrict__ dec, const Physics& ph,
                                            uint32_t lb, double2* sh_u, unsigned char* sh_net) {
+    constexpr bool kEval = GM == 1 || GM == 3;
+    constexpr bool kReuse = GM == 2 || GM == 4 || GM == 5;
+    constexpr bool kLinOnly = GM == 3 || GM == 4;
+    constexpr bool kFullOnly = GM == 5;
     const uint32_t N = g.N;
     if (lb * 64u >= N) return;  // whole block past the end (uniform)
+    if constexpr (kFullOnly) {
+        // the 4 wave marks of the block as one word (uniform)
+        if (reinterpret_cast<const uint32_t*>(t.blk_mixed)[lb] == 0u) return;
+    }
     {
         const uint32_t t0 = t.blk_off[lb], nloc = t.blk_off[lb + 1] - t0;
         const double2* U2 = reinterpret_cast<const double2*>(U);
         for (uint32_t j = threadIdx.x; j < 2 * nloc; j += blockDim.x)
             sh_u[j] = U2[2 * (size_t)t.blk_ids[t0 + (j >> 1)] + (j & 1u)];
-        if (GM == 1 && t.gate) ind::stage_net(t.net, sh_net);  // the network, once per block
+        if (kEval && t.gate) ind::stage_net(t.net, sh_net);  // the network, once per block
     }
     __syncthreads();
     const IdxT* __restrict__ loc = sizeof(IdxT) == 1 ? reinterpret_cast<const IdxT*>(t.loc8)
@@ -342,12 +354,14 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
     // (features from the LDS tile, one MFMA column per cell); a smooth cell
     // skips its directional stencils and takes the central one.
     bool smooth = false;
-    if (GM == 2 && t.gate) {
+    if (kReuse && t.gate) {
         // the decisions the network recorded earlier in the step
         smooth = valid && t.gate_dec[c] != 0;
-        const uint64_t bal = __ballot(smooth && q == 0);
-        if ((threadIdx.x & 63) == 0 && bal) t.gated[lb * 4u + (threadIdx.x >> 6)] += (uint32_t)__popcll(bal);
-    } else if (GM == 1 && t.gate) {
+        if constexpr (!kFullOnly) {
+            const uint64_t bal = __ballot(smooth && q == 0);
+            if ((threadIdx.x & 63) == 0 && bal) t.gated[lb * 4u + (threadIdx.x >> 6)] += (uint32_t)__popcll(bal);
+        }
+    } else if (kEval && t.gate) {
         const int lane = threadIdx.x & 63, gq = lane >> 4;
         const uint32_t cm = lb * 64u + (threadIdx.x >> 6) * 16u + (lane & 15);
         const bool vm = cm < N;
@@ -395,6 +409,11 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         if (vm && gq == 0) t.gate_dec[cm] = (uint8_t)sm;
         const uint64_t bal = __ballot(sm && gq == 0);
         if (lane == 0 && bal) t.gated[lb * 4u + (threadIdx.x >> 6)] += (uint32_t)__popcll(bal);
+        if constexpr (GM == 3) {
+            // mark the wave when a valid cell of it was not gated
+            const uint64_t ung = __ballot(vm && !sm && gq == 0);
+            if (lane == 0) t.blk_mixed[lb * 4u + (threadIdx.x >> 6)] = ung ? 1 : 0;
+        }
         smooth = __shfl(sm, lane >> 2) != 0;
     }
 
@@ -465,13 +484,8 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         central = true;
     };
 
-    if (GM != 0 && t.gate && __all(smooth || !valid)) {
-        // every cell of the wave takes its central stencil
-        smooth = true;  // lanes past the end too: they store nothing
-        linear_faces();
-    } else if (GM != 0 && smooth) {
-        linear_faces();  // a gated cell in a wave with ungated ones
-    } else {
+    // full TENO for this cell (the plain path)
+    auto full_teno = [&]() {
         const bool active = valid && ((info0 >> (20 + q)) & 1u);
 
         // ---- phase A: lane q = stencil
@@ -563,6 +577,27 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
             }
             f[fq] = acc;
         }
+    };
+    // a cell another pass of the split reconstructs: the cell average
+    // stands in (positivity leaves it alone) and nothing is stored
+    auto skip = [&]() {
+#pragma unroll
+        for (int fq = 0; fq < FQ; ++fq) f[fq] = ucv;
+    };
+
+    if constexpr (kFullOnly) {
+        if (smooth) skip();
+        else full_teno();
+    } else if (GM != 0 && t.gate && __all(smooth || !valid)) {
+        // every cell of the wave takes its central stencil
+        smooth = true;  // lanes past the end too: they store nothing
+        linear_faces();
+    } else if (GM != 0 && smooth) {
+        linear_faces();  // a gated cell in a wave with ungated ones
+    } else if constexpr (kLinOnly) {
+        skip();
+    } else {
+        full_teno();
     }
     const int v = q;
 
@@ -602,7 +637,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
             for (int fq = 0; fq < FQ; ++fq) f[fq] = ucv + theta * (f[fq] - ucv);
     }
     const uint64_t cmask = __ballot(central);
-    if (valid) {
+    if (valid && (kLinOnly ? smooth : kFullOnly ? !smooth : true)) {
 #pragma unroll
         for (int fq = 0; fq < FQ; ++fq) fs[((size_t)fq * N + c) * 4 + v] = f[fq];
         if (dec && v == 0) dec[c] = (uint8_t)((cmask >> ((threadIdx.x & 63u) & ~3u)) & 0xFu);
@@ -1197,7 +1232,20 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
         switch (Kp) {
         case 2: if (Mp == 5) MLR_TENO_G(2, 5, 3, 0); else MLR_TENO_G(2, 0, 3, 0); break;
         case 5:
-            if (Mp == 11 && small) {
+            if (Mp == 11 && small && t.gate && t.blk_mixed) {
+                // split gated pass: the gated cells (modes 3 / 4, no
+                // full-TENO code), then the ungated ones of the marked
+                // blocks (mode 5, at the tuned occupancy target)
+                if (gate_reuse) MLR_TENO_S(5, 11, 3, 4, 0);
+                else MLR_TENO_S(5, 11, 3, 3, 0);
+                if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+                if (occ == 5) hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 5, 5>), grid, block, 256 * 32,
+                                                 stream, g, t, U, fs, dec, ph, per_xcd);
+                else if (occ == 6) hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 5, 6>), grid, block,
+                                                      256 * 32, stream, g, t, U, fs, dec, ph, per_xcd);
+                else hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 5, 0>), grid, block, 256 * 32, stream,
+                                        g, t, U, fs, dec, ph, per_xcd);
+            } else if (Mp == 11 && small) {
                 const int gm = !t.gate ? 0 : (gate_reuse ? 2 : 1);
 #define MLR_TENO_O(GMV)                                       \
     do {                                                      \

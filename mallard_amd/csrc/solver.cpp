@@ -492,6 +492,7 @@ dev::Teno Solver::teno() const {
     t.gated = gated_;
     t.gate_dec = gate_dec_;
     t.lin = lin_;
+    t.blk_mixed = (gate_ && gate_split_) ? blk_mixed_ : nullptr;
     t.gate = gate_ ? 1 : 0;
     if (gate_) t.net = dev::net_view(net_buf_, gate_thr_, nullptr);
     return t;
@@ -522,6 +523,11 @@ void Solver::set_indicator(const float* params, double threshold) {
         HIP_OK(hipMalloc((void**)&gate_dec_, L_.N));
         allocs_.push_back(gate_dec_);
         HIP_OK(hipMemset(gate_dec_, 0, L_.N));
+        // every wave marked until a network evaluation clears the marks
+        const size_t nmix = (size_t)(L_.N + 63) / 64 * 4;
+        HIP_OK(hipMalloc((void**)&blk_mixed_, nmix));
+        allocs_.push_back(blk_mixed_);
+        HIP_OK(hipMemset(blk_mixed_, 1, nmix));
         // the central stencils' linear operators the gated cells use
         const size_t nl = L_.n_pad() * (size_t)L_.S * L_.Q * (size_t)(L_.M - 1);
         HIP_OK(hipMalloc((void**)&lin_, nl * sizeof(double)));
@@ -541,6 +547,12 @@ void Solver::set_indicator(const float* params, double threshold) {
 void Solver::set_indicator_cadence(bool every_stage) {
     gate_per_stage_ = every_stage;
     drop_graph();  // the captured step carries the cadence in its kernel arguments
+}
+
+void Solver::set_indicator_split(bool split) {
+    gate_split_ = split;
+    drop_graph();
+    tuned_ = false;
 }
 
 uint64_t Solver::gated_cells(bool reset) {

@@ -142,6 +142,9 @@ public:
     // false (default): the network judges the cells once per step, on its
     // first stage, and the later stages reuse the decisions; true: every stage
     void set_indicator_cadence(bool every_stage);
+    // true (default): gated passes run split, the gated cells in one kernel
+    // and the ungated ones in another (identical results)
+    void set_indicator_split(bool split);
     uint64_t gated_cells(bool reset);      // cells judged smooth since the last reset
     void collect_features(Dataset& ds);    // features + TENO labels of the current state
     void indicator_features(uint16_t* X_ref, float* y_ref);  // same, reference numbering
@@ -209,6 +212,8 @@ private:
     uint8_t* gate_dec_ = nullptr;  // per-cell decisions of the last network evaluation
     double* lin_ = nullptr;        // central stencils' linear operators (gated cells)
     bool gate_per_stage_ = false;
+    bool gate_split_ = true;
+    uint8_t* blk_mixed_ = nullptr;  // split passes: per wave slot, 1 when it holds an ungated cell
     size_t gated_slots_ = 0;
     void* net_buf_ = nullptr;
     float* theta_ = nullptr;

@@ -187,6 +187,34 @@ def test_gate_never_firing_is_plain_at_each_order(mlr, order, every_stage):
     assert b.gated() == 0
 
 
+@pytest.mark.parametrize("every_stage", [False, True])
+def test_split_gated_passes_equal_one_kernel(mlr, every_stage):
+    """Gated passes split in two kernels (the gated cells; then the ungated
+    cells of the blocks holding any) give the one-kernel results bit for
+    bit: states, TENO decision bits, face values and gated counts, on a
+    mesh whose last 64-cell block is partial and a network that gates about
+    half of the cells, so most waves hold both kinds."""
+    rng = np.random.default_rng(4)
+    th = rng.normal(0.0, 0.2, mlr.N_PARAMS).astype(np.float32)
+    th[ir.OFF_B3] = 2.2
+    num = mlr.numerics(recon="TENO", order=2, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
+    gen = mlr.mesh_gen("cartesian_tri", 37, 37, 1.0, 1.0, 0.2, 23)
+    out = []
+    for split in (False, True):
+        s = mlr.Solver(gen=gen, num=num)
+        s.init_quadrant(0.5, 0.5, RIEMANN2D)
+        s.set_indicator(th, 0.9, every_stage=every_stage, split=split)
+        s.run_steps(7)
+        g = s.gated()
+        out.append((s.state()[0], s.decisions(), s.face_values(), g))
+        s.close()
+    assert 0 < out[0][3] < 37 * 37 * 2 * 3 * 7
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2], equal_nan=True)  # NaN: the missing side of boundary faces
+    assert out[0][3] == out[1][3]
+
+
 def test_gate_needs_a_central_stencil_of_twelve(mlr):
     num = mlr.numerics(recon="TENO", order=1, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
     s = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", 8, 8, 1.0, 1.0, 0.2, 1), num=num)
