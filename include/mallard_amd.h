@@ -313,6 +313,9 @@ int mlr_indicator_cadence(mlr_ctx* ctx, int every_stage);
  * the other cells, in the 64-cell blocks that hold any; 0: one kernel takes
  * both kinds. The results are identical. */
 int mlr_indicator_split(mlr_ctx* ctx, int split);
+/* the decisions of the network's last evaluation inside the TENO pass, per
+ * cell in reference numbering (1: gated, the central stencil alone) */
+int mlr_indicator_decisions(mlr_ctx* ctx, uint8_t* out);
 int mlr_ensemble_indicator_cadence(mlr_ensemble* ens, int every_stage);
 /* cells judged smooth (summed over every gated TENO pass) since the last reset */
 int mlr_indicator_gated(mlr_ctx* ctx, int reset, int64_t* gated);

@@ -110,6 +110,7 @@ _SIGS = {
     "mlr_ensemble_autotune": (C.c_int, [_P, _P]),
     "mlr_indicator_cadence": (C.c_int, [_P, C.c_int]),
     "mlr_indicator_split": (C.c_int, [_P, C.c_int]),
+    "mlr_indicator_decisions": (C.c_int, [_P, _P]),
     "mlr_ensemble_indicator_cadence": (C.c_int, [_P, C.c_int]),
     "mlr_ensemble_get_volumes": (C.c_int, [_P, C.c_int, _P]),
     "mlr_ensemble_set_tune": (C.c_int, [_P, _P]),
@@ -450,6 +451,12 @@ class Solver:
             self._check(lib().mlr_indicator_cadence(self.h, int(every_stage)))
             self._check(lib().mlr_indicator_split(self.h, int(split)))
 
+    def gate_decisions(self):
+        """The network's last decisions inside the TENO pass (1: gated), per cell."""
+        out = np.zeros(self.n_cells, np.uint8)
+        self._check(lib().mlr_indicator_decisions(self.h, _p(out)))
+        return out
+
     def gated(self, reset=False):
         n = C.c_int64()
         self._check(lib().mlr_indicator_gated(self.h, int(reset), C.byref(n)))
@@ -569,6 +576,12 @@ class Ensemble:
     def set_tune(self, teno_occ=0, block=256, teno_occ_reuse=0):
         t = np.array([teno_occ, block, block, teno_occ_reuse], np.int32)
         self._check(lib().mlr_ensemble_set_tune(self.h, _p(t)))
+
+    def gate_decisions(self):
+        """The network's last decisions inside the TENO pass (1: gated), per cell."""
+        out = np.zeros(self.n_cells, np.uint8)
+        self._check(lib().mlr_indicator_decisions(self.h, _p(out)))
+        return out
 
     def gated(self, reset=False):
         n = C.c_int64()

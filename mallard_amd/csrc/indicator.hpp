@@ -116,6 +116,25 @@ __device__ inline CellRef cell_ref(double gamma, const double* U) {
     return r;
 }
 
+// cell_ref evaluated by the 4 lanes of the cell's MFMA column (lanes col,
+// col + 16, col + 32, col + 48; gq = lane / 16): its two reciprocals 1 / a_c
+// and 1 / p_c are taken by lanes gq = 0 and 1 and shared, so each lane runs
+// one division fewer. The operations and operands are cell_ref's, so are the
+// results.
+__device__ inline CellRef cell_ref_col(double gamma, const double* U, int gq, int col) {
+    CellRef r;
+    r.ir = 1.0 / U[0];
+    r.u = U[1] * r.ir;
+    r.v = U[2] * r.ir;
+    const double p = (gamma - 1.0) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) * r.ir);
+    r.ok = U[0] > 0.0 && p > 0.0;
+    const double rec = 1.0 / (gq == 0 ? sqrt(gamma * p * r.ir) : p);
+    const double ra = __shfl(rec, col), rp = __shfl(rec, col + 16);
+    r.ia = r.ok ? ra : 1.0;
+    r.ip = rp;
+    return r;
+}
+
 // raw differences of member state Um against the cell; false when Um is not physical
 __device__ inline bool member_diff(double gamma, const CellRef& r, const double* Um, double* d) {
     const double rho = Um[0];

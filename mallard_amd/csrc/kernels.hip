@@ -366,7 +366,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         const uint32_t cm = lb * 64u + (threadIdx.x >> 6) * 16u + (lane & 15);
         const bool vm = cm < N;
         const D4 Uo = ld4(U, vm ? cm : 0);
-        const ind::CellRef r = ind::cell_ref(t.gamma, Uo.v);
+        const ind::CellRef r = ind::cell_ref_col(t.gamma, Uo.v, gq, lane & 15);
         const size_t crow = (size_t)lb * 4 * Mp;  // stencil 0 rows of this block
         // lane (column cell, group gq) holds members 4 kc + gq, kc = 0..2
         double d[3][4], smax[4] = {0.0, 0.0, 0.0, 0.0};
@@ -391,9 +391,11 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         int okc = ok ? 1 : 0;
         okc &= __shfl_xor(okc, 16);
         okc &= __shfl_xor(okc, 32);
+        // the 4 reciprocals of the maxima: lane gq takes variable gq's
         double is[4];
+        const double is_mine = ind::inv_scale(sel4(smax, gq));
 #pragma unroll
-        for (int v = 0; v < 4; ++v) is[v] = ind::inv_scale(smax[v]);
+        for (int v = 0; v < 4; ++v) is[v] = __shfl(is_mine, (lane & 15) + 16 * v);
         ind::v4s xb[3];
 #pragma unroll
         for (int kc = 0; kc < 3; ++kc) {
@@ -1079,7 +1081,17 @@ __device__ __forceinline__ void spectral_block(const Geom& g, const double* __re
 // block; every thread returns the result
 __device__ inline unsigned long long join_blocks(const unsigned long long* __restrict__ lam_blk, uint32_t n) {
     unsigned long long m = 0ull;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) m = lam_blk[i] > m ? lam_blk[i] : m;
+    const uint32_t st = blockDim.x;
+    uint32_t i = threadIdx.x;
+    // 8 loads in flight per thread (the max is order-free)
+    for (; i + 7 * st < n; i += 8 * st) {
+        unsigned long long v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = lam_blk[i + k * st];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) m = v[k] > m ? v[k] : m;
+    }
+    for (; i < n; i += st) m = lam_blk[i] > m ? lam_blk[i] : m;
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o2 = __shfl_xor(m, off);
         m = o2 > m ? o2 : m;
@@ -1235,16 +1247,13 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
             if (Mp == 11 && small && t.gate && t.blk_mixed) {
                 // split gated pass: the gated cells (modes 3 / 4, no
                 // full-TENO code), then the ungated ones of the marked
-                // blocks (mode 5, at the tuned occupancy target)
+                // blocks (mode 5); both at the compiler's occupancy (the
+                // occupancy target does not apply)
                 if (gate_reuse) MLR_TENO_S(5, 11, 3, 4, 0);
                 else MLR_TENO_S(5, 11, 3, 3, 0);
                 if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
-                if (occ == 5) hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 5, 5>), grid, block, 256 * 32,
-                                                 stream, g, t, U, fs, dec, ph, per_xcd);
-                else if (occ == 6) hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 5, 6>), grid, block,
-                                                      256 * 32, stream, g, t, U, fs, dec, ph, per_xcd);
-                else hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 5, 0>), grid, block, 256 * 32, stream,
-                                        g, t, U, fs, dec, ph, per_xcd);
+                hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 5, 0>), grid, block, 256 * 32, stream, g, t, U,
+                                   fs, dec, ph, per_xcd);
             } else if (Mp == 11 && small) {
                 const int gm = !t.gate ? 0 : (gate_reuse ? 2 : 1);
 #define MLR_TENO_O(GMV)                                       \

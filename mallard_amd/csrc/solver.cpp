@@ -845,7 +845,9 @@ void Solver::autotune() {
             HIP_OK(hipMalloc((void**)&saved, gated_slots_ * sizeof(uint32_t)));
             HIP_OK(hipMemcpyAsync(saved, gated_, gated_slots_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
         }
-        const bool occ_tunable = L_.K == 6 && L_.M == 12 && fq == 3 && loc8_ != nullptr;
+        // the occupancy targets exist for the p = 2 kernels, and not for
+        // split gated passes
+        const bool occ_tunable = L_.K == 6 && L_.M == 12 && fq == 3 && loc8_ != nullptr && !(gate_ && gate_split_);
         bt = 1e30f;
         for (int occ : {0, 5, 6}) {
             if (occ != 0 && !occ_tunable) continue;
@@ -1115,6 +1117,14 @@ void Solver::get_face_values(double* out) {
                 std::memcpy(&out[((f * (size_t)Q + q) * 2 + side) * 4], s, 32);
             }
         }
+}
+
+void Solver::get_gate_decisions(uint8_t* out) {
+    if (!gate_dec_) fail("the learned indicator has not been set");
+    std::vector<uint8_t> d(L_.N);
+    synchronize();
+    HIP_OK(hipMemcpy(d.data(), gate_dec_, d.size(), hipMemcpyDeviceToHost));
+    for (uint32_t j = 0; j < L_.N; ++j) out[L_.perm[j]] = d[j];
 }
 
 void Solver::get_decisions(uint8_t* out) {

@@ -145,6 +145,7 @@ public:
     // true (default): gated passes run split, the gated cells in one kernel
     // and the ungated ones in another (identical results)
     void set_indicator_split(bool split);
+    void get_gate_decisions(uint8_t* out);  // last network evaluation, reference numbering
     uint64_t gated_cells(bool reset);      // cells judged smooth since the last reset
     void collect_features(Dataset& ds);    // features + TENO labels of the current state
     void indicator_features(uint16_t* X_ref, float* y_ref);  // same, reference numbering

@@ -215,6 +215,34 @@ def test_split_gated_passes_equal_one_kernel(mlr, every_stage):
     assert out[0][3] == out[1][3]
 
 
+@pytest.mark.parametrize("split", [False, True])
+def test_fused_gate_decisions_equal_feature_and_mlp_kernels(mlr, split):
+    """The network evaluated inside the TENO pass (features from the LDS
+    tile, the network staged in LDS) decides exactly as the standalone
+    feature kernel followed by the inference kernel: gated where the logit
+    reaches logit(0.9) and the stencil is physical, for every cell of a
+    developed Riemann state."""
+    rng = np.random.default_rng(9)
+    th = rng.normal(0.0, 0.2, mlr.N_PARAMS).astype(np.float32)
+    th[ir.OFF_B3] = 0.0
+    num = mlr.numerics(recon="TENO", order=2, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
+    s = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", 45, 45, 1.0, 1.0, 0.2, 31), num=num)
+    s.init_quadrant(0.5, 0.5, RIEMANN2D)
+    s.run_steps(12)
+    X, y = s.indicator_features()
+    thr = np.float32(np.log(0.9 / 0.1))
+    # the output bias that puts the threshold at the median logit of the physical cells
+    z0 = mlr.mlp_forward(th, X)
+    th[ir.OFF_B3] = thr - np.median(z0[y >= 0])
+    z = mlr.mlp_forward(th, X)
+    want = ((z >= thr) & (y >= 0)).astype(np.uint8)
+    s.set_indicator(th, 0.9, split=split)
+    s.calc_rhs()
+    got = s.gate_decisions()
+    assert 0.05 < want.mean() < 0.95
+    assert np.array_equal(got, want)
+
+
 def test_gate_needs_a_central_stencil_of_twelve(mlr):
     num = mlr.numerics(recon="TENO", order=1, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
     s = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", 8, 8, 1.0, 1.0, 0.2, 1), num=num)
