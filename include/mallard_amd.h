@@ -317,6 +317,7 @@ int mlr_indicator_split(mlr_ctx* ctx, int split);
  * cell in reference numbering (1: gated, the central stencil alone) */
 int mlr_indicator_decisions(mlr_ctx* ctx, uint8_t* out);
 int mlr_ensemble_indicator_cadence(mlr_ensemble* ens, int every_stage);
+int mlr_ensemble_indicator_split(mlr_ensemble* ens, int split);
 /* cells judged smooth (summed over every gated TENO pass) since the last reset */
 int mlr_indicator_gated(mlr_ctx* ctx, int reset, int64_t* gated);
 /* cell volumes in reference numbering */

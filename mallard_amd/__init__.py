@@ -112,6 +112,7 @@ _SIGS = {
     "mlr_indicator_split": (C.c_int, [_P, C.c_int]),
     "mlr_indicator_decisions": (C.c_int, [_P, _P]),
     "mlr_ensemble_indicator_cadence": (C.c_int, [_P, C.c_int]),
+    "mlr_ensemble_indicator_split": (C.c_int, [_P, C.c_int]),
     "mlr_ensemble_get_volumes": (C.c_int, [_P, C.c_int, _P]),
     "mlr_ensemble_set_tune": (C.c_int, [_P, _P]),
     "mlr_ensemble_set_indicator": (C.c_int, [_P, _P, C.c_double]),
@@ -563,10 +564,12 @@ class Ensemble:
         return dict(total_ms=st[0], teno_ms=st[1], teno_n=int(st[2]), flux_ms=st[3], flux_n=int(st[4]),
                     dt_ms=st[5], dt_n=int(st[6]))
 
-    def set_indicator(self, params, threshold=0.9, every_stage=False):
+    def set_indicator(self, params, threshold=0.9, every_stage=False, split=True):
         p = None if params is None else _params(params)
         self._check(lib().mlr_ensemble_set_indicator(self.h, _p(p), threshold))
         self._check(lib().mlr_ensemble_indicator_cadence(self.h, int(every_stage)))
+        if p is not None:
+            self._check(lib().mlr_ensemble_indicator_split(self.h, int(split)))
 
     def autotune(self):
         t = np.zeros(4, np.int32)

@@ -860,6 +860,11 @@ int mlr_indicator_split(mlr_ctx* ctx, int split) {
     return guarded(ctx, [&] { ctx->solver.set_indicator_split(split != 0); });
 }
 
+int mlr_ensemble_indicator_split(mlr_ensemble* ens, int split) {
+    if (!ens) return 1;
+    return guarded(nullptr, [&] { ens->e->set_indicator_split(split != 0); });
+}
+
 int mlr_indicator_decisions(mlr_ctx* ctx, uint8_t* out) {
     if (!ctx || !out) return 1;
     return guarded(ctx, [&] { ctx->solver.get_gate_decisions(out); });

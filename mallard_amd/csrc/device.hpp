@@ -169,9 +169,11 @@ hipError_t viscous_grad(const Geom& g, const double* U, const Physics& ph, doubl
 // blk_* offsets are prefix sums (flux and spectral blocks of bs cells, TENO
 // blocks of 64); every case has the same K, M, face points, tile-position
 // width (small = 8-bit) and Riemann solver.
-// gated: some case runs the learned gate (the block stages the network in LDS)
+// gated: some case runs the learned gate (the block stages the network in LDS);
+// split: every case's gate runs split (Teno::blk_mixed set), as teno_faces
 hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
-                      bool small, bool gated, int occ, int gate_reuse, const Physics& ph, hipStream_t stream);
+                      bool small, bool gated, int occ, int gate_reuse, bool split, const Physics& ph,
+                      hipStream_t stream);
 hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const BatchStage& st, int riemann,
                       const Physics& ph, uint32_t bs, hipStream_t stream);
 hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const Physics& ph, uint32_t bs,

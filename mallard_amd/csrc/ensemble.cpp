@@ -71,9 +71,11 @@ void Ensemble::upload_table() {
     const uint32_t bs = tune_.flux_block;
     uint32_t bt = 0, bf = 0;
     gated_ = false;
+    split_ = true;
     for (const auto& c : cases_) {
         dev::CaseBufs b = c->case_bufs();
         gated_ = gated_ || b.t.gate != 0;  // the TENO launch then reserves LDS for the network
+        split_ = split_ && b.t.gate != 0 && b.t.blk_mixed != nullptr;
         b.blk_teno0 = bt;
         b.blk_flux0 = bf;
         bt += (c->L_.N + 63) / 64;
@@ -100,6 +102,13 @@ void Ensemble::set_indicator(const float* params, double threshold) {
 void Ensemble::set_indicator_cadence(bool every_stage) {
     gate_per_stage_ = every_stage;
     drop_graph();
+}
+
+void Ensemble::set_indicator_split(bool split) {
+    for (auto& c : cases_) c->set_indicator_split(split);
+    upload_table();
+    drop_graph();
+    tuned_ = false;
 }
 
 void Ensemble::set_tune(const dev::Tune& t) {
@@ -159,13 +168,13 @@ void Ensemble::autotune() {
                                       stream_), "hipMemcpyAsync");
             }
         }
-        const bool occ_tunable = K_ == 6 && M_ == 12 && FQ_ == 3 && small_;
+        const bool occ_tunable = K_ == 6 && M_ == 12 && FQ_ == 3 && small_ && !(gated_ && split_);
         bt = 1e30f;
         for (int occ : {0, 5, 6}) {
             if (occ != 0 && !occ_tunable) continue;
             const float t = timed([&] {
-                hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, dev::kBufUn, K_, M_, FQ_, small_, gated_, occ, 0, ph_,
-                                       stream_), "teno_batch");
+                hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, dev::kBufUn, K_, M_, FQ_, small_, gated_, occ, 0,
+                                       split_, ph_, stream_), "teno_batch");
             });
             if (t < bt) { bt = t; best.teno_occ = occ; }
         }
@@ -175,7 +184,7 @@ void Ensemble::autotune() {
                 if (occ != 0 && !occ_tunable) continue;
                 const float t = timed([&] {
                     hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, dev::kBufUn, K_, M_, FQ_, small_, gated_, occ,
-                                           1, ph_, stream_), "teno_batch");
+                                           1, split_, ph_, stream_), "teno_batch");
                 });
                 if (t < bt) { bt = t; best.teno_occ_reuse = occ; }
             }
@@ -223,7 +232,7 @@ void Ensemble::launch_step(bool prof) {
             const bool first = op == dev::kSSP1 || op == dev::kRK4_1;
             const int reuse = (gated_ && !gate_per_stage_ && !first) ? 1 : 0;
             hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, gated_,
-                                   reuse ? tune_.teno_occ_reuse : tune_.teno_occ, reuse, ph_, stream_),
+                                   reuse ? tune_.teno_occ_reuse : tune_.teno_occ, reuse, split_, ph_, stream_),
                    "teno_batch");
             mark(kStatTenoMs);
         }

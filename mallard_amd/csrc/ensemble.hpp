@@ -35,8 +35,9 @@ public:
 
     // the learned indicator on every case (params NULL: off)
     void set_indicator(const float* params, double threshold);
-    // as Solver::set_indicator_cadence
+    // as Solver::set_indicator_cadence / set_indicator_split
     void set_indicator_cadence(bool every_stage);
+    void set_indicator_split(bool split);
     // launch shapes (flux and spectral blocks share one size here), timed on
     // the cases' current states before the first steps unless pinned
     void autotune();
@@ -59,7 +60,7 @@ private:
     dev::CaseBufs* d_cases_ = nullptr;
     uint32_t teno_blocks_ = 0, flux_blocks_ = 0;
     int K_ = 0, M_ = 0, FQ_ = 0;
-    bool small_ = true, teno_ = false, gated_ = false, gate_per_stage_ = false;
+    bool small_ = true, teno_ = false, gated_ = false, gate_per_stage_ = false, split_ = false;
     dev::Physics ph_{};
     dev::Tune tune_{};
     bool tuned_ = false;

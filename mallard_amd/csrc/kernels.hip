@@ -10,6 +10,7 @@
 //   - spectral radius, dt: solver.cpp:580-742
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "ddpow.hpp"
@@ -1321,7 +1322,8 @@ hipError_t finalize_dt(const unsigned long long* lam_blk, uint32_t n_blk, double
 }
 
 hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
-                      bool small, bool gated, int occ, int gate_reuse, const Physics& ph, hipStream_t stream) {
+                      bool small, bool gated, int occ, int gate_reuse, bool split, const Physics& ph,
+                      hipStream_t stream) {
     const int Kp = K - 1, Mp = M - 1;
     const size_t net_lds = (gated && !gate_reuse) ? ind::kNetInferBytes : 0;
     if (n_cases < 1 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
@@ -1346,7 +1348,14 @@ hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks,
         else MLR_TB_G(KP, MP, FQV, 1);                   \
     } while (0)
     if (gated && Mp < ind::kMembers - 1) return hipErrorInvalidValue;
-    if (FQ == 3 && Kp == 5 && Mp == 11 && small) {
+    if (FQ == 3 && Kp == 5 && Mp == 11 && small && gated && split) {
+        // split gated pass (teno_faces' modes 3 / 4, then 5)
+        if (gate_reuse) MLR_TB_S(5, 11, 3, 4, 0);
+        else MLR_TB_S(5, 11, 3, 3, 0);
+        if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+        hipLaunchKernelGGL((teno_batch_kernel<5, 11, 3, uint8_t, 256, 5, 0>), grid, block, 256 * 32, stream, cases,
+                           n_cases, in_role, total_blocks, ph, per_xcd);
+    } else if (FQ == 3 && Kp == 5 && Mp == 11 && small) {
         const int gm = !gated ? 0 : (gate_reuse ? 2 : 1);
 #define MLR_TB_O(GMV)                                     \
     do {                                                  \
