@@ -109,13 +109,16 @@ def test_gate_that_never_fires_is_plain_teno(mlr, every_stage):
     assert b.gated() == 0
 
 
-def test_gate_that_always_fires_on_smooth_field_is_plain_teno(mlr):
+@pytest.mark.parametrize("order", [2, 3])
+def test_gate_that_always_fires_on_smooth_field_is_plain_teno(mlr, order):
     """Where plain TENO selects the central stencil everywhere, forcing it
     changes nothing but rounding: the gated cells evaluate the central
     stencil through its precombined linear operator (face-point basis x A+ x
     areas) instead of coefficients and basis. Every cell of every pass is
-    counted, and a uniform state stays bit-identical."""
-    num = mlr.numerics(recon="TENO", order=2, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
+    counted, and a uniform state stays bit-identical. p = 2 runs the split
+    kernels, p = 3 (two face points per face, 20-member stencils) the
+    one-kernel runtime-loop build."""
+    num = mlr.numerics(recon="TENO", order=order, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
     gen = mlr.mesh_gen("cartesian_tri", 32, 32, 1.0, 1.0, 0.2, 12)
     a, b = mlr.Solver(gen=gen, num=num), mlr.Solver(gen=gen, num=num)
     th = np.zeros(mlr.N_PARAMS, np.float32)
