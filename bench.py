@@ -72,6 +72,18 @@ def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes):
     return alg, layout
 
 
+def step_roofline(teno_alg, N, n_stages, wall):
+    """The plain path's whole step against HBM peak: algorithmic bytes of a
+    stage are the TENO launch's plus 224 B per cell outside it (the stage
+    input, step-start and output states, 3 x 32 B; the cell geometry, 68 B;
+    1.5 faces' normals, areas, cells and nodes, 60 B), over the timed steps'
+    wall time."""
+    per_stage = teno_alg + 224 * N
+    achieved = per_stage * n_stages / wall / 1e9
+    return {"alg_bytes_per_stage": per_stage, "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+            "frac": achieved / 8000.0}
+
+
 def learned_roofline(learned, N, dev):
     """The gated TENO kernel of the learned run against HBM peak. Algorithmic
     bytes per launch, in the learned form's accounting: a gated cell needs
@@ -472,7 +484,8 @@ def main():
                           "FP64 TENO" % args.threshold) if matched else "none (plain FP64 TENO)",
         },
         "plain": {"value": value_plain, "ms_per_step": wall / args.steps * 1e3,
-                  "ms_per_step_graph": wall_graph / args.steps * 1e3},
+                  "ms_per_step_graph": wall_graph / args.steps * 1e3,
+                  "whole_step": step_roofline(alg, N, stages * args.steps, wall)},
         "learned": learned,
         "roofline": {
             "kernel": "teno_kernel (plain FP64 TENO run)",
