@@ -190,8 +190,9 @@ def test_gate_never_firing_is_plain_at_each_order(mlr, order, every_stage):
     assert b.gated() == 0
 
 
-@pytest.mark.parametrize("every_stage", [False, True])
-def test_split_gated_passes_equal_one_kernel(mlr, every_stage):
+@pytest.mark.parametrize("integrator,every_stage", [("SSPRK3", False), ("SSPRK3", True), ("RK4", False),
+                                                    ("FE", False)])
+def test_split_gated_passes_equal_one_kernel(mlr, integrator, every_stage):
     """Gated passes split in two kernels (the gated cells; then the ungated
     cells of the blocks holding any) give the one-kernel results bit for
     bit: states, TENO decision bits, face values and gated counts, on a
@@ -200,8 +201,9 @@ def test_split_gated_passes_equal_one_kernel(mlr, every_stage):
     rng = np.random.default_rng(4)
     th = rng.normal(0.0, 0.2, mlr.N_PARAMS).astype(np.float32)
     th[ir.OFF_B3] = 2.2
-    num = mlr.numerics(recon="TENO", order=2, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
+    num = mlr.numerics(recon="TENO", order=2, teno_variant="corrected", cfl=0.4, bcs=EXTRAP, integrator=integrator)
     gen = mlr.mesh_gen("cartesian_tri", 37, 37, 1.0, 1.0, 0.2, 23)
+    stages = {"SSPRK3": 3, "RK4": 4, "FE": 1}[integrator]
     out = []
     for split in (False, True):
         s = mlr.Solver(gen=gen, num=num)
@@ -211,7 +213,7 @@ def test_split_gated_passes_equal_one_kernel(mlr, every_stage):
         g = s.gated()
         out.append((s.state()[0], s.decisions(), s.face_values(), g))
         s.close()
-    assert 0 < out[0][3] < 37 * 37 * 2 * 3 * 7
+    assert 0 < out[0][3] < 37 * 37 * 2 * stages * 7
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2], equal_nan=True)  # NaN: the missing side of boundary faces
