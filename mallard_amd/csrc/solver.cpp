@@ -547,6 +547,7 @@ void Solver::set_indicator(const float* params, double threshold) {
 void Solver::set_indicator_cadence(bool every_stage) {
     gate_per_stage_ = every_stage;
     drop_graph();  // the captured step carries the cadence in its kernel arguments
+    tuned_ = false;  // the reuse passes' shape is tuned only for per-step cadence
 }
 
 void Solver::set_indicator_split(bool split) {
@@ -800,7 +801,8 @@ void Solver::autotune() {
     static std::map<std::vector<int64_t>, dev::Tune> cache;
     const int fq = (int)(L_.S * L_.Q);
     const std::vector<int64_t> key = {(int64_t)L_.N, L_.K, L_.M, fq, loc8_ ? 1 : 0, gate_ ? 1 : 0,
-                                      (int64_t)cfg_.riemann, L_.teno ? 1 : 0};
+                                      (int64_t)cfg_.riemann, L_.teno ? 1 : 0, gate_split_ ? 1 : 0,
+                                      gate_per_stage_ ? 1 : 0, cfg_.viscous ? 1 : 0};
     {
         std::lock_guard<std::mutex> lk(mu);
         auto it = cache.find(key);
