@@ -195,9 +195,9 @@ def test_gate_never_firing_is_plain_at_each_order(mlr, order, every_stage):
 def test_split_gated_passes_equal_one_kernel(mlr, integrator, every_stage):
     """Gated passes split in two kernels (the gated cells; then the ungated
     cells of the blocks holding any) give the one-kernel results bit for
-    bit: states, TENO decision bits, face values and gated counts, on a
-    mesh whose last 64-cell block is partial and a network that gates about
-    half of the cells, so most waves hold both kinds."""
+    bit: time steps, states, TENO decision bits, face values and gated
+    counts, on a mesh whose last 64-cell block is partial and a network that
+    gates about half of the cells, so most waves hold both kinds."""
     rng = np.random.default_rng(4)
     th = rng.normal(0.0, 0.2, mlr.N_PARAMS).astype(np.float32)
     th[ir.OFF_B3] = 2.2
@@ -209,15 +209,17 @@ def test_split_gated_passes_equal_one_kernel(mlr, integrator, every_stage):
         s = mlr.Solver(gen=gen, num=num)
         s.init_quadrant(0.5, 0.5, RIEMANN2D)
         s.set_indicator(th, 0.9, every_stage=every_stage, split=split)
-        s.run_steps(7)
+        dts = s.run_steps(7)
         g = s.gated()
-        out.append((s.state()[0], s.decisions(), s.face_values(), g))
+        out.append((s.state()[0], s.decisions(), s.face_values(), g, dts, s.cfl_local()))
         s.close()
     assert 0 < out[0][3] < 37 * 37 * 2 * stages * 7
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2], equal_nan=True)  # NaN: the missing side of boundary faces
     assert out[0][3] == out[1][3]
+    assert np.array_equal(out[0][4], out[1][4])
+    assert np.array_equal(out[0][5], out[1][5])
 
 
 @pytest.mark.parametrize("split", [False, True])
