@@ -10,7 +10,6 @@
 //   - spectral radius, dt: solver.cpp:580-742
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
 #include <cstdint>
 
 #include "ddpow.hpp"
@@ -1016,7 +1015,8 @@ __global__ __launch_bounds__(256) void flux_batch_kernel(const CaseBufs* __restr
 
 // ------------------------------------------------------------ dt
 // solver.cpp:592-706: per-cell spectral radius over the cell's faces in
-// local order, max over the grid through a block max and one atomic.
+// local order; the block's maximum, joined over the blocks by
+// finalize_dt_kernel.
 __device__ __forceinline__ void spectral_block(const Geom& g, const double* __restrict__ Un,
                                                const double* __restrict__ pr, const Physics& ph,
                                                double* __restrict__ lam_cell, unsigned long long* __restrict__ lam_blk,
