@@ -22,7 +22,11 @@
  * normals, free-stream preservation, A+ A = I on the fitted subspace, SVD
  * pseudo-inverse agreement, linear-field offsets, conservation, SSPRK3
  * amplification). The Roe flux (no reference counterpart) is pinned only by
- * textbook properties: parity unpinned against the reference.
+ * textbook properties: parity unpinned against the reference. Also parity
+ * unpinned beyond those invariants: the order of equal-distance ring members
+ * (libstdc++ std::sort ties, Q15 of the hot-path notes) and the reference
+ * weighting's raw-weight and face-offset details (Q7/Q8), whose only checks
+ * are the offset being a cell constant and the weighting's known divergence.
  */
 #include "oracle.h"
 
