@@ -582,7 +582,8 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
             L.nx[o] = u0;
             L.ny[o] = u1;
             L.area[o] = m.face_area[f];
-            // spectral-radius length (solver.cpp:662-672, boundary hack Q17)
+            // spectral-radius length (solver.cpp:662-672; a boundary face
+            // takes the reference's own special-case length)
             double s0, s1;
             if (bnd) {
                 s0 = 2.0 * std::pow(m.cell_volume[c0], 1.0 / kDim);

@@ -24,9 +24,10 @@
  * amplification). The Roe flux (no reference counterpart) is pinned only by
  * textbook properties: parity unpinned against the reference. Also parity
  * unpinned beyond those invariants: the order of equal-distance ring members
- * (libstdc++ std::sort ties, Q15 of the hot-path notes) and the reference
- * weighting's raw-weight and face-offset details (Q7/Q8), whose only checks
- * are the offset being a cell constant and the weighting's known divergence.
+ * (libstdc++ std::sort ties, face_reconstruction.cpp:253-257) and the
+ * reference weighting's raw weights and +ipt/ta0 face offset
+ * (face_reconstruction.cpp:948-1031), whose only checks are the offset being
+ * a cell constant and the weighting's known divergence.
  */
 #include "oracle.h"
 
@@ -253,7 +254,7 @@ double legendre_d(int n, int p, double x) {
     return P.scale * acc;
 }
 // basis.h:67-78 (monomial; the derivative keeps the reference's (p-k+1)
-// factors and negative powers, Q10)
+// factors and negative powers)
 double monomial(int p, double x) { return std::pow(x, (double)p); }
 double monomial_d(int n, int p, double x) {
     double r = std::pow(x, (double)(p - n));
@@ -415,7 +416,7 @@ struct orc_mesh {
             cell_coords[2 * (size_t)c + 1] = y / k;
         }
     }
-    // mesh.cpp:229-261 (phantom faces: 0/0 -> NaN normals, Q2)
+    // mesh.cpp:229-261 (phantom faces: 0/0 -> NaN normals, as in the reference)
     void face_normals_() {
         face_normals.assign(2 * (size_t)n_faces, 0.0);
         for (uint32_t f = 0; f < n_faces; ++f) {
@@ -526,7 +527,8 @@ void init_cart(orc_mesh& m, uint32_t nx, uint32_t ny, double Lx, double Ly, doub
     m.face_normals_();
 }
 
-// mesh.cpp:559-826 (n_faces over-allocated, Q2; node order per triangle, Q12)
+// mesh.cpp:559-826 (n_faces over-allocated and the node order of each
+// triangle as in the reference)
 void init_cart_tri(orc_mesh& m, uint32_t nx, uint32_t ny, double Lx, double Ly, double jitter,
                    uint64_t seed) {
     m.nx = nx; m.ny = ny;
@@ -874,7 +876,7 @@ struct orc_solver {
     std::vector<uint8_t> poly;  // [n_dof][2]
     Quad qcell, qface;
     std::vector<uint32_t> off_groups, off_st, st;
-    std::vector<uint64_t> off_rm;  // 64-bit offsets (Q5 overflows uint32 at p=3)
+    std::vector<uint64_t> off_rm;  // 64-bit offsets (the reference's uint32 ones overflow at p=3 on 8M cells)
     std::vector<double> rm, ta, ipt, OI;
     // state
     std::vector<double> U, Utmp, prim, face, cfl_local;
@@ -1085,7 +1087,7 @@ struct orc_solver {
                     for (int k = 0; k < K; ++k) A[(size_t)nl * K + k] -= at[nl] * ipt[k];
                 bool col0 = true;
                 for (int nl = 0; nl < ssz; ++nl)
-                    if (A[(size_t)nl * K] > 1.0e-12) { col0 = false; break; }  // no fabs (Q6)
+                    if (A[(size_t)nl * K] > 1.0e-12) { col0 = false; break; }  // no fabs, as the reference
                 if (col0) {
                     std::vector<double> B((size_t)(ssz - 1) * (K - 1));
                     for (int nl = 1; nl < ssz; ++nl)

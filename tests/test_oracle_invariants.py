@@ -209,7 +209,7 @@ def test_corrected_reconstruction_exact_for_quadratics(oracle):
 
 
 def test_reference_face_offset_is_cell_constant(oracle):
-    """Q7: the reference evaluates psi_k + ipt_k/ta_0 instead of psi_k - ipt_k,
+    """The reference evaluates psi_k + ipt_k/ta_0 instead of psi_k - ipt_k,
     so in central-stencil cells its face values exceed the zero-mean ones by
     sum_k a_k (ipt_k/ta_0 + ipt_k), the same on every face of the cell."""
     m = oracle.Mesh("cartesian_tri", 8, 8, 1.0, 1.0, 0.15, 2)
