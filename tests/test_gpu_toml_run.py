@@ -1,0 +1,51 @@
+"""The drop-in path end to end on the GPU: Solver::init(path) and
+Solver::run (mlr_ctx_create_toml, mlr_run) on the reference's own Sod
+example input, against the CPU oracle on the same mesh and state."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_sod_example_runs_to_the_oracle_state(mlr, oracle, tmp_path):
+    """examples/sod/input.toml (1000 x 1 quads, FO, HLLC, SSPRK3, CFL 1,
+    t_stop 0.2, symmetry walls, analytical initial state, VTU output every
+    100 steps) run by mlr_run from the state the input defines: the run
+    stops at the first step that reaches t_stop (dt is not clipped), as the
+    oracle's time does after the same number of steps, and the write_data
+    table's files are written. The rarefaction's two-rarefaction estimates
+    take the device's double-double pow where the oracle calls glibc's,
+    which differ by one ulp on about 0.1% of arguments (here first at step
+    55), so state and time are compared to 1e-10 relative L-inf."""
+    src = open(os.path.join(HERE, "golden", "examples", "sod.toml")).read()
+    prefix = str(tmp_path / "sod_all")
+    assert 'prefix = "./solut/all/sod_all"' in src
+    p = tmp_path / "sod.toml"
+    p.write_text(src.replace('prefix = "./solut/all/sod_all"', 'prefix = "%s"' % prefix))
+    s = mlr.Solver(toml_path=p)
+    U0, W0 = s.state()  # the analytical initial state, as the input file defines it
+    s.run()
+    t, steps = s.time()
+    U = s.state()[0]
+    s.close()
+
+    m = oracle.Mesh("cartesian", 1000, 1, 1.0, 0.001, 0.0, 0)
+    orc = oracle.Solver(m, oracle.make_config(recon="FO", riemann="HLLC", integrator="SSPRK3"), 1)
+    orc.set_state(U0, W0)
+    dts = orc.run(steps, 1.0)
+    assert orc.time() >= 0.2 > orc.time() - dts[-1]  # the step that crossed t_stop was the last
+    assert t >= 0.2 and abs(t - orc.time()) <= 1e-12 * t
+    Uo = orc.state()[0]
+    assert np.isfinite(U).all()
+    scale = np.maximum(np.abs(Uo).max(axis=0), 1e-300)  # rho v is zero throughout
+    assert (np.abs(U - Uo).max(axis=0) / scale).max() <= 1e-10
+    files = sorted(os.listdir(tmp_path))
+    vtus = [f for f in files if f.startswith("sod_all_") and f.endswith(".vtu")]
+    assert len(vtus) >= steps // 100
+    head = open(tmp_path / vtus[0], "rb").read(4096)
+    for name in (b"CFL", b"RHO", b"RHOU_X", b"P", b"T", b"H"):
+        assert b'Name="' + name + b'"' in head
