@@ -85,3 +85,11 @@ def test_rejects_teno_on_quads(mlr):
 def test_rejects_unknown_boundary_zone(mlr):
     with pytest.raises(mlr.MallardError, match="not found"):
         _layout(mlr, "cartesian_tri", 4, 4, 0.0, 0, recon="FO", bcs=[("inlet", "symmetry", {})])
+
+
+@pytest.mark.parametrize("nx,ny,order", [(1, 1, 1), (1, 1, 2), (2, 1, 2)])
+def test_rejects_mesh_smaller_than_stencil(mlr, nx, ny, order):
+    """Too few cells for a central stencil of the order's size: a clear error,
+    where the oracle's ring search runs out of memory."""
+    with pytest.raises(mlr.MallardError, match="mesh too small"):
+        _layout(mlr, "cartesian_tri", nx, ny, 0.0, 0, recon="TENO", order=order, teno_variant="corrected")
