@@ -62,3 +62,25 @@ def test_ensemble_with_gate_equals_standalone(mlr, split):
         total += s.gated()
         assert np.array_equal(ens.state(k)[0], s.state()[0]), k
     assert 0 < gated == total
+
+
+@pytest.mark.parametrize("recon,sizes", [("TENO", [(3, 3), (40, 40), (8, 4), (16, 1)]),
+                                         ("FO", [(1, 1), (5, 1), (24, 24), (2, 3)])])
+def test_ensemble_tiny_cases_equal_standalone(mlr, recon, sizes):
+    """Cases of one partial tile or block (down to a single cell) beside a
+    larger one: each still equals its standalone run."""
+    from mallard_amd import indicator as ind
+    num = mlr.numerics(recon=recon, order=2, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
+    rng = np.random.default_rng(17)
+    gens = [mlr.mesh_gen("cartesian_tri", nx, ny, 1.0, 1.0, 0.2, 40 + k) for k, (nx, ny) in enumerate(sizes)]
+    states = [ind.random_quadrant_states(rng) for _ in sizes]
+    ens = mlr.Ensemble(gens, num)
+    for k in range(len(gens)):
+        ens.init_quadrant(k, 0.5, 0.5, states[k])
+    ens.run_steps(8)
+    for k, g in enumerate(gens):
+        s = mlr.Solver(gen=g, num=num)
+        s.init_quadrant(0.5, 0.5, states[k])
+        s.run_steps(8)
+        assert np.array_equal(ens.state(k)[0], s.state()[0]), k
+        assert ens.time(k) == s.time(), k
