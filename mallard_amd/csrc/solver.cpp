@@ -897,9 +897,18 @@ void Solver::autotune() {
     cache[key] = best;
 }
 
+// solver.cpp:580-590: a negative CFL number or fixed dt is refused with the
+// reference's message. A fixed dt of zero is refused too: the device's dt
+// pass reads a zero fixed dt as "use the CFL number".
+void Solver::check_dt_mode() const {
+    if (cfg_.use_cfl && cfg_.cfl < 0.0) fail("dt negative: " + std::to_string(cfg_.cfl) + ".");
+    if (!cfg_.use_cfl && cfg_.dt < 0.0) fail("dt negative: " + std::to_string(cfg_.dt) + ".");
+    if (!cfg_.use_cfl && cfg_.dt == 0.0) fail("a fixed dt must be positive (or give a CFL number)");
+}
+
 void Solver::run_steps(int64_t n, double* dts_out) {
     if (n <= 0) return;
-    if (cfg_.use_cfl && cfg_.cfl < 0.0) fail("dt negative: " + std::to_string(cfg_.cfl) + ".");
+    check_dt_mode();
     ensure_tuned();
     int64_t s0 = 0;
     HIP_OK(hipMemcpyAsync(&s0, step_dev_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
@@ -916,7 +925,7 @@ void Solver::run_steps(int64_t n, double* dts_out) {
 void Solver::run_steps_profiled(int64_t n, double* stats) {
     for (int k = 0; k < kStatCount; ++k) stats[k] = 0.0;
     if (n <= 0) return;
-    if (cfg_.use_cfl && cfg_.cfl < 0.0) fail("dt negative: " + std::to_string(cfg_.cfl) + ".");
+    check_dt_mode();
     ensure_tuned();
     int64_t s0 = 0;
     HIP_OK(hipMemcpyAsync(&s0, step_dev_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
@@ -966,6 +975,7 @@ void Solver::finish_steps(int64_t s0, int64_t n, double* dts_out) {
 }
 
 double Solver::calc_dt() {
+    check_dt_mode();
     if (cfg_.use_cfl) {
         HIP_OK(dev::spectral_radius(geom(), Un_, prims_, ph_, lam_cell_, lam_blk_, tune_.spec_block, stream_));
         HIP_OK(dev::finalize_dt(lam_blk_, spec_blocks(), cfg_.cfl, 0.0, dt_dev_, nullptr, 1, nullptr, stream_));

@@ -172,6 +172,7 @@ private:
     dev::StageArgs stage(int op, const double* Uin, double* Uout) const;
     void stage_pass(int op, const double* Uin, double* Uout);
     void dt_pass(double cfl, double fixed_dt);
+    void check_dt_mode() const;
     void mark(int tag);
     void drop_graph();
     dev::CaseBufs case_bufs() const;   // this solver as one case of an ensemble

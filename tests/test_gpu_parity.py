@@ -259,3 +259,34 @@ def test_gpu_precompute_matches_host(mlr, order):
         a_h, t_h, _ = L.cell_matrices(c)
         assert np.array_equal(a_d, a_h), c
         assert np.array_equal(t_d[:, 1:], t_h[:, 1:]), c
+
+
+def test_fixed_dt_step_equals_cfl_step(mlr, oracle):
+    """A fixed dt (cfl = 0, dt given; solver.cpp:580-590 skips the spectral
+    radius) takes exactly that dt every step, and its first step equals the
+    CFL run's first step when given the dt that run chose."""
+    dev, orc, m = pair(mlr, oracle, "cartesian_tri", 12, 10, jitter=0.2, seed=3, recon="TENO", order=2,
+                       teno_variant="corrected", cfl=0.4)
+    dev.init_quadrant(0.5, 0.5, RIEMANN2D)
+    orc.init_quadrant(0.5, 0.5, RIEMANN2D)
+    dt0 = dev.run_steps(1)[0]
+    assert dt0 == orc.run(1, 0.4)[0]
+    num = mlr.numerics(recon="TENO", order=2, teno_variant="corrected", cfl=0.0, dt=dt0)
+    fx = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", 12, 10, 1.0, 1.0, 0.2, 3), num=num)
+    fx.init_quadrant(0.5, 0.5, RIEMANN2D)
+    dts = fx.run_steps(1)
+    assert dts[0] == dt0
+    assert np.array_equal(fx.state()[0], dev.state()[0])
+    dts = fx.run_steps(4)
+    assert np.all(dts == dt0)
+    t, steps = fx.time()
+    assert steps == 5 and abs(t - 5 * dt0) <= 1e-15
+
+
+def test_negative_or_zero_fixed_dt_refused(mlr):
+    gen = mlr.mesh_gen("cartesian_tri", 6, 6, 1.0, 1.0, 0.0, 0)
+    for dt, msg in ((-1e-3, "dt negative"), (0.0, "fixed dt must be positive")):
+        s = mlr.Solver(gen=gen, num=mlr.numerics(recon="FO", cfl=0.0, dt=dt))
+        s.init_quadrant(0.5, 0.5, RIEMANN2D)
+        with pytest.raises(mlr.MallardError, match=msg):
+            s.run_steps(1)
