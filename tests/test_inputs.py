@@ -128,3 +128,33 @@ def test_navier_stokes_keys(mlr, tmp_path):
     p.write_text(base.replace('type = "euler"', 'type = "stokes"'))
     with pytest.raises(mlr.MallardError, match="Unknown physics type: stokes."):
         mlr.parse_config(p)
+
+
+@pytest.mark.parametrize("run,msg", [
+    ("n_steps = 5", "Either dt or cfl must be specified."),
+    ("cfl = 0.5\ndt = 1e-3\nn_steps = 5", "Only one of dt or cfl can be specified."),
+    ("cfl = 0.5", "Either n_steps, t_stop, or t_wall_stop must be specified."),
+])
+def test_run_section_errors(mlr, tmp_path, run, msg):
+    """The reference's [run] checks and messages (solver.cpp:246-258)."""
+    src = open(os.path.join(HERE, "sod.toml")).read()
+    head, tail = src.split("[run]", 1)
+    rest = tail.split("\n[", 1)
+    p = tmp_path / "in.toml"
+    p.write_text(head + "[run]\n" + run + "\n" + ("\n[" + rest[1] if len(rest) > 1 else ""))
+    with pytest.raises(mlr.MallardError) as e:
+        mlr.parse_config(p)
+    assert str(e.value) == msg
+
+
+def test_run_section_fixed_dt(mlr, tmp_path):
+    """dt instead of cfl selects the fixed step (solver.cpp:260-264); cfl
+    then reads 0 and n_steps keeps its UINT32_MAX default."""
+    src = open(os.path.join(HERE, "sod.toml")).read()
+    head, tail = src.split("[run]", 1)
+    rest = tail.split("\n[", 1)
+    p = tmp_path / "in.toml"
+    p.write_text(head + "[run]\ndt = 2.5e-4\nt_stop = 0.01\n" + ("\n[" + rest[1] if len(rest) > 1 else ""))
+    run = mlr.parse_config(p)[2]
+    assert run["cfl"] == 0.0 and run["dt"] == 2.5e-4
+    assert run["t_stop"] == 0.01 and run["n_steps"] == UINT32_MAX
