@@ -887,11 +887,13 @@ int mlr_get_volumes(const mlr_ctx* ctx, double* vol) {
 }
 
 int mlr_host_pow(const double* x, const double* y, double* out, int n) {
+    if (n < 0 || (n > 0 && (!x || !y || !out))) return 1;
     for (int i = 0; i < n; ++i) out[i] = mlr::ddpow::pow_dd(x[i], y[i]);
     return 0;
 }
 
 int mlr_host_pow6(const double* x, double* out, int n) {
+    if (n < 0 || (n > 0 && (!x || !out))) return 1;
     for (int i = 0; i < n; ++i) out[i] = mlr::pow6_cr(x[i]);
     return 0;
 }
