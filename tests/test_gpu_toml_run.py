@@ -49,3 +49,45 @@ def test_sod_example_runs_to_the_oracle_state(mlr, oracle, tmp_path):
     head = open(tmp_path / vtus[0], "rb").read(4096)
     for name in (b"CFL", b"RHO", b"RHOU_X", b"P", b"T", b"H"):
         assert b'Name="' + name + b'"' in head
+
+
+def _sod_with_run(tmp_path, run):
+    """sod.toml with its [run] table replaced and its output redirected."""
+    src = open(os.path.join(HERE, "golden", "examples", "sod.toml")).read()
+    src = src.replace('prefix = "./solut/all/sod_all"', 'prefix = "%s"' % str(tmp_path / "sod_all"))
+    head, tail = src.split("[run]", 1)
+    rest = tail.split("\n[", 1)
+    p = tmp_path / "sod.toml"
+    p.write_text(head + "[run]\n" + run + "\n" + ("\n[" + rest[1] if len(rest) > 1 else ""))
+    return p
+
+
+def test_sod_example_n_steps_stop(mlr, oracle, tmp_path):
+    """n_steps ends the run after exactly that many steps (solver.cpp:375-403),
+    at the oracle's time and state (1e-10, as above)."""
+    s = mlr.Solver(toml_path=_sod_with_run(tmp_path, "n_steps = 37\ncfl = 1.0"))
+    U0, W0 = s.state()
+    s.run()
+    t, steps = s.time()
+    U = s.state()[0]
+    s.close()
+    assert steps == 37
+    m = oracle.Mesh("cartesian", 1000, 1, 1.0, 0.001, 0.0, 0)
+    orc = oracle.Solver(m, oracle.make_config(recon="FO", riemann="HLLC", integrator="SSPRK3"), 1)
+    orc.set_state(U0, W0)
+    orc.run(37, 1.0)
+    assert abs(t - orc.time()) <= 1e-12 * t
+    Uo = orc.state()[0]
+    scale = np.maximum(np.abs(Uo).max(axis=0), 1e-300)
+    assert (np.abs(U - Uo).max(axis=0) / scale).max() <= 1e-10
+
+
+def test_sod_example_wall_clock_stop(mlr, tmp_path):
+    """t_wall_stop ends a run that no step or time limit would end."""
+    s = mlr.Solver(toml_path=_sod_with_run(tmp_path, "t_wall_stop = 0.5\ndt = 1.0e-6"))
+    s.run()
+    t, steps = s.time()
+    U = s.state()[0]
+    s.close()
+    assert steps > 0 and abs(t - steps * 1.0e-6) <= 1e-9
+    assert np.isfinite(U).all()
