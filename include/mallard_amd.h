@@ -55,8 +55,10 @@ typedef struct {
 
 /* [numerics], [physics] and [run] settings (solver.cpp:110-273,
  * physics.cpp:27-53, face_reconstruction.cpp:109-152). quad orders <= 0 take
- * the reference defaults (Dunavant p+1, Gauss-Legendre (p+1)/2). cfl > 0
- * selects CFL time steps, otherwise the fixed dt is used. */
+ * the reference defaults (Dunavant p+1, Gauss-Legendre (p+1)/2). cfl != 0
+ * selects CFL time steps (a negative CFL number is refused with "dt
+ * negative", as solver.cpp:580-590 refuses the negative dt it gives),
+ * cfl = 0 the fixed dt, which must be positive. */
 typedef struct {
     int32_t recon, basis, order;
     double stencil_factor;
@@ -166,6 +168,12 @@ int mlr_get_tune(mlr_ctx* ctx, int32_t* tune);
 /* Solver::run (solver.cpp:352-373) with the configured stop conditions. */
 int mlr_run(mlr_ctx* ctx);
 int mlr_set_cfl(mlr_ctx* ctx, double cfl);
+/* Length of the device ring that holds each step's dt until mlr_run_steps /
+ * mlr_run read it back (1 .. 2^20 steps, default 2^20; no reference
+ * counterpart: the reference reads dt on the host every step,
+ * solver.cpp:358-364). Longer runs are issued in chunks of at most this many
+ * steps. */
+int mlr_set_dt_history(mlr_ctx* ctx, int64_t steps);
 int mlr_get_time(const mlr_ctx* ctx, double* t, int64_t* step);
 /* Face states of the last residual, face_solution[n_faces][Q][2][4] as in
  * FaceReconstruction::calc_face_values (face_reconstruction.cpp:95-99,

@@ -78,6 +78,7 @@ _SIGS = {
     "mlr_get_tune": (C.c_int, [_P, _P]),
     "mlr_run": (C.c_int, [_P]),
     "mlr_set_cfl": (C.c_int, [_P, C.c_double]),
+    "mlr_set_dt_history": (C.c_int, [_P, C.c_int64]),
     "mlr_get_time": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mlr_get_face_values": (C.c_int, [_P, _P]),
     "mlr_get_decisions": (C.c_int, [_P, _P]),
@@ -405,6 +406,11 @@ class Solver:
 
     def set_cfl(self, cfl):
         self._check(lib().mlr_set_cfl(self.h, cfl))
+
+    def set_dt_history(self, steps):
+        """Length of the device ring of per-step dt values (runs are issued
+        in chunks of at most this many steps)."""
+        self._check(lib().mlr_set_dt_history(self.h, steps))
 
     def time(self):
         t, s = C.c_double(), C.c_int64()

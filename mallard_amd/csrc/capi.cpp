@@ -77,7 +77,10 @@ mlr::Config config_from_desc(const mlr_numerics_desc* d) {
     c.rho_ref = d->rho_ref;
     c.p_min = d->p_min;
     c.p_max = d->p_max;
-    c.use_cfl = d->cfl > 0;
+    // a nonzero CFL number selects CFL steps (a negative one is then refused
+    // with "dt negative" before any step, as solver.cpp:580-590 refuses the
+    // negative dt it gives); zero selects the fixed dt
+    c.use_cfl = d->cfl != 0.0;
     c.cfl = d->cfl;
     c.dt = d->dt;
     if (d->n_bc < 0 || d->n_bc > 14) fail("n_bc out of range");
@@ -370,6 +373,11 @@ int mlr_run(mlr_ctx* ctx) {
 int mlr_set_cfl(mlr_ctx* ctx, double cfl) {
     if (!ctx) return 1;
     return guarded(ctx, [&] { ctx->solver.set_cfl(cfl); });
+}
+
+int mlr_set_dt_history(mlr_ctx* ctx, int64_t steps) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.set_dt_history(steps); });
 }
 
 int mlr_get_time(const mlr_ctx* ctx, double* t, int64_t* step) {

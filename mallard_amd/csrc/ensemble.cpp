@@ -2,6 +2,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <functional>
 #include <string>
 
@@ -19,6 +20,7 @@ Ensemble::Ensemble(const Config& base, const std::vector<MeshSpec>& meshes) : cf
     if (meshes.empty()) fail("an ensemble needs at least one case");
     if (cfg_.integrator == Integrator::FE) fail("ensembles run the RK4 and SSPRK3 integrators");
     if (cfg_.viscous) fail("ensembles run the Euler equations");
+    check_dt_config(cfg_);  // before any case allocates
     cases_.reserve(meshes.size());
     for (const MeshSpec& ms : meshes) {
         Config c = cfg_;
@@ -256,10 +258,12 @@ void Ensemble::finish(int64_t s0, int64_t n) {
     for (auto& c : cases_) c->finish_steps(s0, n, nullptr);
 }
 
+// Issued in chunks of at most the dt ring's length (every case's ring has
+// the same length), each chunk's dt values read back before the next.
 void Ensemble::run_steps(int64_t n) {
     if (n <= 0) return;
+    check_dt_config(cfg_);
     if (!tuned_) autotune();
-    const int64_t s0 = cases_[0]->device_step();
     if (!exec_) {
         hip_ok(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
@@ -273,37 +277,49 @@ void Ensemble::run_steps(int64_t n) {
         hip_ok(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
         hip_ok(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
     }
-    for (int64_t i = 0; i < n; ++i) hip_ok(hipGraphLaunch(exec_, stream_), "hipGraphLaunch");
-    hip_ok(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    finish(s0, n);
+    const int64_t cap = cases_[0]->dt_hist_cap_;
+    for (int64_t done = 0; done < n;) {
+        const int64_t m = std::min<int64_t>(n - done, cap);
+        const int64_t s0 = cases_[0]->device_step();
+        for (int64_t i = 0; i < m; ++i) hip_ok(hipGraphLaunch(exec_, stream_), "hipGraphLaunch");
+        hip_ok(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        finish(s0, m);
+        done += m;
+    }
 }
 
 void Ensemble::run_steps_profiled(int64_t n, double* stats) {
     for (int k = 0; k < kStatCount; ++k) stats[k] = 0.0;
     if (n <= 0) return;
+    check_dt_config(cfg_);
     if (!tuned_) autotune();
-    const int64_t s0 = cases_[0]->device_step();
-    prof_on_ = true;
-    prof_used_ = 0;
-    try {
-        mark(-1);
-        for (int64_t i = 0; i < n; ++i) launch_step(true);
-    } catch (...) {
+    const int64_t cap = cases_[0]->dt_hist_cap_;
+    for (int64_t done = 0; done < n;) {
+        const int64_t m = std::min<int64_t>(n - done, cap);
+        const int64_t s0 = cases_[0]->device_step();
+        prof_on_ = true;
+        prof_used_ = 0;
+        try {
+            mark(-1);
+            for (int64_t i = 0; i < m; ++i) launch_step(true);
+        } catch (...) {
+            prof_on_ = false;
+            throw;
+        }
         prof_on_ = false;
-        throw;
+        hip_ok(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        for (size_t i = 1; i < prof_used_; ++i) {
+            float ms = 0.0f;
+            hip_ok(hipEventElapsedTime(&ms, prof_[i - 1].second, prof_[i].second), "hipEventElapsedTime");
+            stats[prof_[i].first] += ms;
+            stats[prof_[i].first + 1] += 1.0;
+        }
+        float tot = 0.0f;
+        hip_ok(hipEventElapsedTime(&tot, prof_[0].second, prof_[prof_used_ - 1].second), "hipEventElapsedTime");
+        stats[kStatTotalMs] += tot;
+        finish(s0, m);
+        done += m;
     }
-    prof_on_ = false;
-    hip_ok(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    for (size_t i = 1; i < prof_used_; ++i) {
-        float ms = 0.0f;
-        hip_ok(hipEventElapsedTime(&ms, prof_[i - 1].second, prof_[i].second), "hipEventElapsedTime");
-        stats[prof_[i].first] += ms;
-        stats[prof_[i].first + 1] += 1.0;
-    }
-    float tot = 0.0f;
-    hip_ok(hipEventElapsedTime(&tot, prof_[0].second, prof_[prof_used_ - 1].second), "hipEventElapsedTime");
-    stats[kStatTotalMs] = tot;
-    finish(s0, n);
 }
 
 }  // namespace mlr

@@ -897,29 +897,40 @@ void Solver::autotune() {
     cache[key] = best;
 }
 
-// solver.cpp:580-590: a negative CFL number or fixed dt is refused with the
-// reference's message. A fixed dt of zero is refused too: the device's dt
-// pass reads a zero fixed dt as "use the CFL number".
-void Solver::check_dt_mode() const {
-    if (cfg_.use_cfl && cfg_.cfl < 0.0) fail("dt negative: " + std::to_string(cfg_.cfl) + ".");
-    if (!cfg_.use_cfl && cfg_.dt < 0.0) fail("dt negative: " + std::to_string(cfg_.dt) + ".");
-    if (!cfg_.use_cfl && cfg_.dt == 0.0) fail("a fixed dt must be positive (or give a CFL number)");
+void check_dt_config(const Config& c) {
+    if (c.use_cfl && !(c.cfl >= 0.0)) fail("dt negative: " + std::to_string(c.cfl) + ".");
+    if (!c.use_cfl && c.dt < 0.0) fail("dt negative: " + std::to_string(c.dt) + ".");
+    if (!c.use_cfl && !(c.dt > 0.0)) fail("a fixed dt must be positive (or give a CFL number)");
 }
 
+void Solver::check_dt_mode() const { check_dt_config(cfg_); }
+
+int64_t Solver::stream_step() {
+    int64_t s0 = 0;
+    HIP_OK(hipMemcpyAsync(&s0, step_dev_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    return s0;
+}
+
+// The device writes each step's dt into a ring of dt_hist_cap_ entries, so a
+// run is issued in chunks of at most that many steps, each read back before
+// the next chunk overwrites the ring.
 void Solver::run_steps(int64_t n, double* dts_out) {
     if (n <= 0) return;
     check_dt_mode();
     ensure_tuned();
-    int64_t s0 = 0;
-    HIP_OK(hipMemcpyAsync(&s0, step_dev_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
-    synchronize();
     if (use_graphs_) record_step();
-    for (int64_t i = 0; i < n; ++i) {
-        if (use_graphs_) HIP_OK(hipGraphLaunch(step_exec_, stream_));
-        else launch_step();
+    for (int64_t done = 0; done < n;) {
+        const int64_t m = std::min<int64_t>(n - done, dt_hist_cap_);
+        const int64_t s0 = stream_step();
+        for (int64_t i = 0; i < m; ++i) {
+            if (use_graphs_) HIP_OK(hipGraphLaunch(step_exec_, stream_));
+            else launch_step();
+        }
+        synchronize();
+        finish_steps(s0, m, dts_out ? dts_out + done : nullptr);
+        done += m;
     }
-    synchronize();
-    finish_steps(s0, n, dts_out);
 }
 
 void Solver::run_steps_profiled(int64_t n, double* stats) {
@@ -927,51 +938,71 @@ void Solver::run_steps_profiled(int64_t n, double* stats) {
     if (n <= 0) return;
     check_dt_mode();
     ensure_tuned();
-    int64_t s0 = 0;
-    HIP_OK(hipMemcpyAsync(&s0, step_dev_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
-    synchronize();
-    prof_on_ = true;
-    prof_used_ = 0;
-    try {
-        mark(-1);
-        for (int64_t i = 0; i < n; ++i) launch_step();
-    } catch (...) {
+    for (int64_t done = 0; done < n;) {
+        const int64_t m = std::min<int64_t>(n - done, dt_hist_cap_);
+        const int64_t s0 = stream_step();
+        prof_on_ = true;
+        prof_used_ = 0;
+        try {
+            mark(-1);
+            for (int64_t i = 0; i < m; ++i) launch_step();
+        } catch (...) {
+            prof_on_ = false;
+            throw;
+        }
         prof_on_ = false;
-        throw;
+        synchronize();
+        // launch i ran between the events recorded before and after it
+        for (size_t i = 1; i < prof_used_; ++i) {
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, prof_[i - 1].second, prof_[i].second));
+            const int tag = prof_[i].first;
+            stats[tag] += ms;
+            stats[tag + 1] += 1.0;
+        }
+        float tot = 0.0f;
+        HIP_OK(hipEventElapsedTime(&tot, prof_[0].second, prof_[prof_used_ - 1].second));
+        stats[kStatTotalMs] += tot;
+        finish_steps(s0, m, nullptr);
+        done += m;
     }
-    prof_on_ = false;
-    synchronize();
-    // launch i ran between the events recorded before and after it
-    for (size_t i = 1; i < prof_used_; ++i) {
-        float ms = 0.0f;
-        HIP_OK(hipEventElapsedTime(&ms, prof_[i - 1].second, prof_[i].second));
-        const int tag = prof_[i].first;
-        stats[tag] += ms;
-        stats[tag + 1] += 1.0;
-    }
-    float tot = 0.0f;
-    HIP_OK(hipEventElapsedTime(&tot, prof_[0].second, prof_[prof_used_ - 1].second));
-    stats[kStatTotalMs] = tot;
-    finish_steps(s0, n, nullptr);
 }
 
+// The n <= dt_hist_cap_ dt values of the steps s0 .. s0 + n - 1, which the
+// device wrote at ring positions (s0 + i) % dt_hist_cap_.
 void Solver::finish_steps(int64_t s0, int64_t n, double* dts_out) {
-    // the n dt values of these steps (a ring of dt_hist_cap_ entries)
-    std::vector<double> d((size_t)std::min<int64_t>(n, dt_hist_cap_));
-    const int64_t first = s0 % dt_hist_cap_, cnt = (int64_t)d.size();
-    const int64_t part = std::min<int64_t>(cnt, dt_hist_cap_ - first);
+    if (n > dt_hist_cap_) fail("internal: more steps than the dt history holds");
+    std::vector<double> d((size_t)n);
+    const int64_t first = s0 % dt_hist_cap_;
+    const int64_t part = std::min<int64_t>(n, dt_hist_cap_ - first);
     HIP_OK(hipMemcpy(d.data(), dt_hist_ + first, sizeof(double) * part, hipMemcpyDeviceToHost));
-    if (cnt > part) HIP_OK(hipMemcpy(d.data() + part, dt_hist_, sizeof(double) * (cnt - part), hipMemcpyDeviceToHost));
+    if (n > part) HIP_OK(hipMemcpy(d.data() + part, dt_hist_, sizeof(double) * (n - part), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; ++i) {
-        // beyond the ring only the last dt_hist_cap_ values remain
-        const int64_t k = i - (n - cnt);
-        const double dt = k >= 0 ? d[(size_t)k] : d[0];
+        const double dt = d[(size_t)i];
         if (dts_out) dts_out[i] = dt;
         if (dt < 0.0) fail("dt negative: " + std::to_string(dt) + ".");
         t_ += dt;
         dt_ = dt;
     }
     step_ += n;
+}
+
+void Solver::set_dt_history(int64_t cap) {
+    if (cap < 1 || cap > kDtHistCap) fail("the dt history holds 1 to 1048576 steps");
+    synchronize();
+    drop_graph();  // the captured finalize kernel carries the ring's address and length
+    void* p = nullptr;
+    HIP_OK(hipMalloc(&p, sizeof(double) * cap));
+    HIP_OK(hipMemset(p, 0, sizeof(double) * cap));
+    for (void*& a : allocs_)
+        if (a == dt_hist_) {
+            HIP_OK(hipFree(a));
+            a = p;
+        }
+    dev_bytes_ += sizeof(double) * (uint64_t)cap;
+    dev_bytes_ -= sizeof(double) * (uint64_t)dt_hist_cap_;
+    dt_hist_ = static_cast<double*>(p);
+    dt_hist_cap_ = cap;
 }
 
 double Solver::calc_dt() {
@@ -1146,7 +1177,14 @@ void Solver::get_decisions(uint8_t* out) {
     for (uint32_t j = 0; j < L_.N; ++j) out[L_.perm[j]] = d[j];
 }
 
+// solver.cpp:580-590, :705: cfl_local = dt * lambda, filled only when the
+// step comes from the CFL number; with a fixed dt the reference never runs
+// the spectral radius and its (zero-initialised) cfl_local stays zero
 void Solver::get_cfl_local(double* out) {
+    if (!cfg_.use_cfl) {
+        std::fill(out, out + L_.N, 0.0);
+        return;
+    }
     std::vector<double> l(L_.N);
     synchronize();
     HIP_OK(hipMemcpy(l.data(), lam_cell_, l.size() * 8, hipMemcpyDeviceToHost));

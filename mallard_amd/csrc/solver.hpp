@@ -73,6 +73,13 @@ struct Config {
 // Parses a Mallard TOML input with the reference's keys and defaults.
 Config config_from_toml(const toml::Value& in);
 
+// The time-step mode of a configuration, checked before any step
+// (solver.cpp:580-590): a negative CFL number or fixed dt is refused with the
+// reference's "dt negative" message, and a fixed dt of zero too (the device's
+// dt pass reads a zero fixed dt as "use the CFL number"). Solvers and
+// ensembles share it.
+void check_dt_config(const Config& c);
+
 class Ensemble;
 
 class Solver {
@@ -105,6 +112,9 @@ public:
     // on the stream after each launch; stats receives the event-measured totals
     // (see StepStats)
     void run_steps_profiled(int64_t n, double* stats);
+    // length of the device ring of per-step dt values (default 2^20 steps):
+    // runs are issued in chunks of at most this many steps
+    void set_dt_history(int64_t cap);
     void get_face_values(double* out);              // [n_faces alloc][Q][2][4], reference numbering
     void get_decisions(uint8_t* out);               // [N] bit v = central stencil for variable v
     void get_cfl_local(double* out);
@@ -177,6 +187,7 @@ private:
     void drop_graph();
     dev::CaseBufs case_bufs() const;   // this solver as one case of an ensemble
     int64_t device_step();
+    int64_t stream_step();  // the device step counter, read on the context's stream
     void finish_steps(int64_t s0, int64_t n, double* dts_out);
 
     Config cfg_;

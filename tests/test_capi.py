@@ -32,6 +32,19 @@ def test_null_arguments_return_errors(mlr):
     assert L.mlr_calc_rhs(None, None) != 0
 
 
+def test_ensemble_refuses_bad_dt_mode_before_allocating(mlr):
+    """An ensemble checks its time-step mode when it is created, before any
+    case allocates device memory (so no GPU is touched here): a fixed dt of
+    zero or below (cfl = 0) and a negative CFL number are refused with the
+    reference's messages (solver.cpp:580-590)."""
+    import pytest
+    gens = [mlr.mesh_gen("cartesian_tri", 8, 8, 1.0, 1.0, 0.0, 0)]
+    for cfl, dt, msg in ((0.0, 0.0, "fixed dt must be positive"), (0.0, -1e-3, "dt negative"),
+                         (-0.5, 1e-3, "dt negative")):
+        with pytest.raises(mlr.MallardError, match=msg):
+            mlr.Ensemble(gens, mlr.numerics(recon="TENO", order=2, cfl=cfl, dt=dt))
+
+
 def test_host_pow6_is_correctly_rounded(mlr):
     from fractions import Fraction
     rng = np.random.default_rng(4)

@@ -284,9 +284,46 @@ def test_fixed_dt_step_equals_cfl_step(mlr, oracle):
 
 
 def test_negative_or_zero_fixed_dt_refused(mlr):
+    """A negative fixed dt, a zero one, and a negative CFL number (which the
+    C ABI takes as CFL mode, so the reference's negative-dt error applies,
+    solver.cpp:580-590) are refused before any step, also when a fixed dt is
+    given beside the negative CFL number."""
     gen = mlr.mesh_gen("cartesian_tri", 6, 6, 1.0, 1.0, 0.0, 0)
-    for dt, msg in ((-1e-3, "dt negative"), (0.0, "fixed dt must be positive")):
-        s = mlr.Solver(gen=gen, num=mlr.numerics(recon="FO", cfl=0.0, dt=dt))
+    for cfl, dt, msg in ((0.0, -1e-3, "dt negative"), (0.0, 0.0, "fixed dt must be positive"),
+                         (-0.5, 1e-3, "dt negative"), (-0.5, 0.0, "dt negative")):
+        s = mlr.Solver(gen=gen, num=mlr.numerics(recon="FO", cfl=cfl, dt=dt))
         s.init_quadrant(0.5, 0.5, RIEMANN2D)
+        U0 = s.state()[0]
         with pytest.raises(mlr.MallardError, match=msg):
             s.run_steps(1)
+        assert s.time() == (0.0, 0) and np.array_equal(s.state()[0], U0)
+
+
+def test_dt_history_ring_wraps(mlr):
+    """Runs longer than the device's dt ring are issued in chunks that each
+    read the ring back before the next overwrites it: with a 7-entry ring,
+    20 steps in one call, then calls of 3 and 11 steps that start and end
+    inside the ring, return the same dt values, time and state as a solver
+    with the default ring."""
+    num = mlr.numerics(recon="TENO", order=2, teno_variant="corrected", cfl=0.4)
+    gen = mlr.mesh_gen("cartesian_tri", 12, 10, 1.0, 1.0, 0.2, 5)
+    a = mlr.Solver(gen=gen, num=num)
+    b = mlr.Solver(gen=gen, num=num)
+    b.set_dt_history(7)
+    for s in (a, b):
+        s.init_quadrant(0.5, 0.5, RIEMANN2D)
+    dts = []
+    for n in (20, 3, 11):
+        da = a.run_steps(n)
+        db = b.run_steps(n)
+        assert np.array_equal(da, db), n
+        assert a.time() == b.time()
+        dts.extend(db)
+    assert b.time()[1] == 34 and len(set(dts)) > 1
+    t = 0.0
+    for d in dts:  # the host's running sum, step by step
+        t += d
+    assert b.time()[0] == t
+    assert np.array_equal(a.state()[0], b.state()[0])
+    with pytest.raises(mlr.MallardError, match="dt history"):
+        b.set_dt_history(0)

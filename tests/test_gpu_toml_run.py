@@ -83,11 +83,22 @@ def test_sod_example_n_steps_stop(mlr, oracle, tmp_path):
 
 
 def test_sod_example_wall_clock_stop(mlr, tmp_path):
-    """t_wall_stop ends a run that no step or time limit would end."""
-    s = mlr.Solver(toml_path=_sod_with_run(tmp_path, "t_wall_stop = 0.5\ndt = 1.0e-6"))
-    s.run()
-    t, steps = s.time()
-    U = s.state()[0]
-    s.close()
-    assert steps > 0 and abs(t - steps * 1.0e-6) <= 1e-9
-    assert np.isfinite(U).all()
+    """t_wall_stop ends a run that no step or time limit would end
+    (solver.cpp:394-397): the clock starts when the context is made, so the
+    run ends no earlier than t_wall_stop after the creation call began, and
+    within one step after the limit (here well under a few seconds). With
+    an n_steps limit the run cannot reach, the wall clock still decides."""
+    import time
+    for extra in ("", "n_steps = 1000000000\n"):
+        t0 = time.perf_counter()
+        s = mlr.Solver(toml_path=_sod_with_run(tmp_path, extra + "t_wall_stop = 0.5\ndt = 1.0e-6"))
+        t1 = time.perf_counter()
+        s.run()
+        t2 = time.perf_counter()
+        t, steps = s.time()
+        U = s.state()[0]
+        s.close()
+        assert t2 - t0 >= 0.5, extra
+        assert t2 - t1 <= 0.5 + 3.0, extra
+        assert 0 < steps < 1000000000 and abs(t - steps * 1.0e-6) <= 1e-9
+        assert np.isfinite(U).all()
