@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg. It is what results are compared with, never
 the product path.
 """
+import contextlib
 import ctypes as C
 import os
 
@@ -99,6 +100,20 @@ def lib():
     return _lib
 
 
+@contextlib.contextmanager
+def use_library(path):
+    """Binds another build of the oracle (the mutation tests' copies with a
+    planted mistake) for the duration of the block. Objects keep the library
+    they were made with, so they can be released after the block ends."""
+    global _lib, _LIB_PATH
+    saved = (_lib, _LIB_PATH)
+    _lib, _LIB_PATH = None, path
+    try:
+        yield lib()
+    finally:
+        _lib, _LIB_PATH = saved
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
@@ -111,7 +126,7 @@ class Mesh:
     """Reference-numbered mesh from the oracle's generators (mesh.cpp:305-848)."""
 
     def __init__(self, kind, nx, ny, Lx=1.0, Ly=1.0, jitter=0.0, seed=0):
-        L = lib()
+        L = self._L = lib()
         self.h = L.orc_mesh_create(MESH[kind], nx, ny, Lx, Ly, jitter, seed)
         if not self.h:
             raise RuntimeError(L.orc_last_error().decode())
@@ -145,8 +160,8 @@ class Mesh:
             self.zones[name.value.decode()] = buf
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.orc_mesh_destroy(self.h)
+        if getattr(self, "h", None):
+            self._L.orc_mesh_destroy(self.h)
             self.h = None
 
 
@@ -176,7 +191,7 @@ class Solver:
     """Oracle solver: one thread reproduces the canonical one-thread order."""
 
     def __init__(self, mesh, cfg, n_threads=1):
-        L = lib()
+        L = self._L = lib()
         self.mesh = mesh
         self.cfg = cfg
         self.h = L.orc_solver_create(mesh.h, C.byref(cfg), n_threads)
@@ -185,8 +200,8 @@ class Solver:
         self.nc = mesh.n_cells
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.orc_solver_destroy(self.h)
+        if getattr(self, "h", None):
+            self._L.orc_solver_destroy(self.h)
             self.h = None
 
     def teno(self):

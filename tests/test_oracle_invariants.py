@@ -8,6 +8,8 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
+from oracle_pins import central_stencil
+
 RIEMANN2D = np.array([[1.5, 0.0, 0.0, 1.5], [0.5323, 1.206, 0.0, 0.3],
                       [0.138, 1.206, 1.206, 0.029], [0.5323, 0.0, 1.206, 0.3]])
 
@@ -56,40 +58,6 @@ def test_mesh_invariants(oracle, kind, nx, ny, jit):
 
 
 # ------------------------------------------------------------------ stencils
-def _neighbors1(m, c):
-    out = {c}
-    for f in m.faces_of_cell[3 * c:3 * c + 3]:
-        a, b = m.cells_of_face[f]
-        if b >= 0:
-            out.add(int(b) if a == c else int(a))
-    return sorted(out)
-
-
-def _next_ring(m, rings, tgt):
-    old = set().union(*[set(r) for r in rings])
-    nr = []
-    for c in rings[-1]:
-        for x in _neighbors1(m, c):
-            if x not in old and x not in nr:
-                nr.append(x)
-    d = [((m.cell_coords[x, 0] - m.cell_coords[tgt, 0]) ** 2 + (m.cell_coords[x, 1] - m.cell_coords[tgt, 1]) ** 2, i, x)
-         for i, x in enumerate(nr)]
-    d.sort(key=lambda t: t[0])  # stable: equals the introsort result for rings of <= 16 cells
-    assert len(nr) <= 16
-    return [x for _, _, x in d]
-
-
-def _central(m, c, M):
-    st, rings = [c], [[c]]
-    while len(st) < M:
-        rings.append(_next_ring(m, rings, c))
-        for x in rings[-1]:
-            if len(st) == M:
-                break
-            st.append(x)
-    return st
-
-
 def test_central_stencils_brute_force(oracle):
     """Independent ring search (Python sets + stable sort) on small meshes."""
     for jit in (0.0, 0.25):
@@ -99,10 +67,12 @@ def test_central_stencils_brute_force(oracle):
         og, os_, st = T["offsets_stencil_groups"], T["offsets_stencils"], T["stencils"]
         for c in range(m.n_cells):
             a, b = os_[og[c]], os_[og[c] + 1]
-            assert list(st[a:b]) == _central(m, c, 12)
+            assert list(st[a:b]) == central_stencil(m, c, 12)
 
 
 def test_directional_stencil_properties(oracle):
+    """Sizes and membership rules of the directional stencils (the stencils
+    themselves are pinned by the brute-force restatement in oracle_pins)."""
     m = oracle.Mesh("cartesian_tri", 8, 8, 1.0, 1.0, 0.2, 5)
     s = oracle.Solver(m, oracle.make_config(order=2))
     T = s.teno()
