@@ -185,6 +185,60 @@ int mlr_get_decisions(mlr_ctx* ctx, uint8_t* out);
 /* cfl_local after the last dt (solver.cpp:584, :705). */
 int mlr_get_cfl_local(mlr_ctx* ctx, double* out);
 
+/* ---- The strategy interfaces on their own, for callers that keep their own
+ * Solver loop and swap in one GPU piece at a time. */
+
+/* FaceReconstruction::calc_face_values(solution, face_solution)
+ * (face_reconstruction.h:43-90; TENO face_reconstruction.cpp:1081-1106, FO
+ * :59-99) on a caller's state: U[N_cells][4] in reference numbering,
+ * face_solution[n_faces][Q][2][4] as mlr_get_face_values. The context's own
+ * state, time and dt are left as they were. */
+int mlr_calc_face_values(mlr_ctx* ctx, const double* U, double* face_solution);
+
+/* Face sums of the residual (solver_rhs.cpp:44-115): MLR_RHS_INTERIOR is
+ * calc_rhs_interior (the interior flux functor, flux_functor.h:106-178),
+ * MLR_RHS_BOUNDARY is calc_rhs_boundaries, i.e. Boundary::apply of every zone
+ * (boundary.h:47-126, boundary_*.cpp), MLR_RHS_ALL both. Each result is divided
+ * by the cell volume (DivideVolumeFunctor, solver_rhs.cpp:18-42), so the
+ * interior and boundary parts add up to the residual (to rounding: the full
+ * residual sums each cell's faces in one pass). */
+enum { MLR_RHS_INTERIOR = 1, MLR_RHS_BOUNDARY = 2, MLR_RHS_ALL = 3 };
+/* Solver::calc_rhs(solution, face_solution, rhs) on a caller's state:
+ * U[N][4] and rhs[N][4] host arrays in reference numbering. */
+int mlr_calc_rhs_of(mlr_ctx* ctx, const double* U, int parts, double* rhs);
+/* The same on DEVICE arrays U_dev[N][4], rhs_dev[N][4] in the context's
+ * numbering (cell j of these arrays is reference cell perm[j],
+ * mlr_get_permutation), queued on the context's stream without a host copy or
+ * a synchronisation: the form a device rhs function for
+ * mlr_integrator_take_step takes. */
+int mlr_calc_rhs_device(mlr_ctx* ctx, const double* U_dev, int parts, double* rhs_dev);
+
+/* RiemannSolver::calc_flux(flux, n_unit, rho_l, u_l, p_l, gamma_l, h_l, rho_r,
+ * u_r, p_r, gamma_r, h_r) (riemann_solver.h:42-94; HLLC :441-519, HLL
+ * :377-439, Rusanov :331-375; Roe has no reference counterpart) for n face
+ * points at once, host arrays: n_unit[n][2], left[n][6] and right[n][6] =
+ * (rho, u_x, u_y, p, gamma, h), flux[n][4]. type = MLR_RS_*. guard != 0 floors
+ * the two-rarefaction pressure terms at zero (the corrected TENO variant's
+ * vacuum guard); guard = 0 is the reference's formula. The points are staged
+ * through device memory allocated for the call. */
+int mlr_riemann_flux(int type, int guard, int64_t n, const double* n_unit, const double* left, const double* right,
+                     double* flux);
+
+/* TimeIntegrator::take_step(dt, solution_vec, face_solution, rhs_vec, calc_rhs)
+ * (time_integrator.h:41-99, time_integrator.cpp:57-163) on DEVICE vectors of n
+ * doubles: solution_vec[0] is the state, advanced in place; solution_vec[1]
+ * the stage state (RK4, SSPRK3); rhs_vec[0..] the stage residuals (FE 1, RK4 4,
+ * SSPRK3 3: get_n_solution_vectors / get_n_rhs_vectors). rhs(U, rhs, user) is
+ * called once per stage with device pointers and returns 0 on success. The
+ * copies and KokkosBlas axpy / axpby updates run as kernels in the reference's
+ * order on the context's stream (ctx may be NULL: the null stream), so an rhs
+ * function that queues its work on that stream, such as mlr_calc_rhs_device
+ * on the same context, needs no synchronisation; any other must finish its
+ * work before returning. The call returns after the step has finished. */
+typedef int (*mlr_rhs_fn)(const double* U, double* rhs, void* user);
+int mlr_integrator_take_step(mlr_ctx* ctx, int type, double dt, int64_t n, double* const* solution_vec,
+                             double* const* rhs_vec, mlr_rhs_fn rhs, void* user);
+
 /* Index parity hooks: new -> reference cell permutation, and one cell's
  * stencils / pseudo-inverses recomputed in reference numbering
  * (face_reconstruction.cpp:406-433, :477-741). ids[G*M], sizes[G],

@@ -57,6 +57,8 @@ class MlrMeshGen(C.Structure):
 
 # every entry point of include/mallard_amd.h with its ctypes signature
 _P = C.c_void_p
+# int (*mlr_rhs_fn)(const double* U, double* rhs, void* user), device pointers
+RHS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p)
 _SIGS = {
     "mlr_ctx_create_toml": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "mlr_ctx_create_generated": (C.c_int, [C.POINTER(MlrMeshGen), C.POINTER(MlrNumerics), C.POINTER(_P)]),
@@ -136,7 +138,15 @@ _SIGS = {
     "mlr_get_volumes": (C.c_int, [_P, _P]),
     "mlr_host_pow": (C.c_int, [_P, _P, _P, C.c_int]),
     "mlr_host_pow6": (C.c_int, [_P, _P, C.c_int]),
+    "mlr_calc_face_values": (C.c_int, [_P, _P, _P]),
+    "mlr_calc_rhs_of": (C.c_int, [_P, _P, C.c_int, _P]),
+    "mlr_calc_rhs_device": (C.c_int, [_P, _P, C.c_int, _P]),
+    "mlr_riemann_flux": (C.c_int, [C.c_int, C.c_int, C.c_int64, _P, _P, _P, _P]),
+    "mlr_integrator_take_step": (C.c_int, [_P, C.c_int, C.c_double, C.c_int64, _P, _P, RHS_FN, _P]),
 }
+
+# residual parts (mlr_calc_rhs_of / mlr_calc_rhs_device)
+RHS_INTERIOR, RHS_BOUNDARY, RHS_ALL = 1, 2, 3
 
 _lib = None
 
@@ -422,6 +432,29 @@ class Solver:
         self._check(lib().mlr_get_face_values(self.h, _p(out)))
         return out
 
+    def calc_face_values(self, U):
+        """FaceReconstruction::calc_face_values of a caller's state U[N][4]
+        (reference numbering) -> face_solution[n_faces][Q][2][4]; the
+        solver's own state is untouched."""
+        U = _f64(U)
+        out = np.zeros((self.n_faces, self.Q, 2, 4))
+        self._check(lib().mlr_calc_face_values(self.h, _p(U), _p(out)))
+        return out
+
+    def calc_rhs_of(self, U, parts=RHS_ALL):
+        """Residual of a caller's state over the interior faces (1), the
+        boundary faces (2) or both (3), divided by the volume."""
+        U = _f64(U)
+        r = np.zeros((self.n_cells, 4))
+        self._check(lib().mlr_calc_rhs_of(self.h, _p(U), parts, _p(r)))
+        return r
+
+    def calc_rhs_device(self, U_ptr, rhs_ptr, parts=RHS_ALL):
+        """Residual of the device array at U_ptr ([N][4], this solver's
+        numbering) into the device array at rhs_ptr, queued on the solver's
+        stream."""
+        self._check(lib().mlr_calc_rhs_device(self.h, C.c_void_p(U_ptr), parts, C.c_void_p(rhs_ptr)))
+
     def decisions(self):
         d = np.zeros(self.n_cells, np.uint8)
         self._check(lib().mlr_get_decisions(self.h, _p(d)))
@@ -665,6 +698,42 @@ def mlp_train(params, ds, pool, steps, batch, lr, seed=0, pos_w=1.0, neg_w=1.0):
     if lib().mlr_mlp_train(_p(p), ds.h, _p(pool), len(pool), steps, batch, lr, seed, pos_w, neg_w, _p(hist)):
         raise MallardError(lib().mlr_last_error(None).decode())
     return p, hist
+
+
+def riemann_flux(rs, n_unit, left, right, guard=0):
+    """RiemannSolver::calc_flux over n points on the GPU: n_unit[n][2],
+    left/right[n][6] = (rho, u_x, u_y, p, gamma, h) -> flux[n][4]."""
+    n_unit, left, right = _f64(n_unit), _f64(left), _f64(right)
+    n = n_unit.shape[0]
+    if n_unit.shape != (n, 2) or left.shape != (n, 6) or right.shape != (n, 6):
+        raise ValueError("riemann_flux: n_unit[n][2], left[n][6], right[n][6]")
+    out = np.zeros((n, 4))
+    t = RIEMANN[rs] if isinstance(rs, str) else int(rs)
+    if lib().mlr_riemann_flux(t, guard, n, _p(n_unit), _p(left), _p(right), _p(out)):
+        raise MallardError(lib().mlr_last_error(None).decode())
+    return out
+
+
+def integrator_take_step(integrator, dt, n, solution_ptrs, rhs_ptrs, rhs_fn, solver=None):
+    """TimeIntegrator::take_step on device vectors of n doubles.
+    solution_ptrs / rhs_ptrs: device addresses; rhs_fn(U_ptr, rhs_ptr) fills
+    the residual (device pointers). With a solver, the updates run on its
+    stream."""
+    t = INTEGRATOR[integrator] if isinstance(integrator, str) else int(integrator)
+    sol = (C.c_void_p * len(solution_ptrs))(*solution_ptrs)
+    rhs = (C.c_void_p * len(rhs_ptrs))(*rhs_ptrs)
+
+    def _cb(U, r, _user):
+        try:
+            rhs_fn(U, r)
+            return 0
+        except Exception:
+            return 1
+
+    cb = RHS_FN(_cb)
+    h = solver.h if solver is not None else None
+    if lib().mlr_integrator_take_step(h, t, dt, n, sol, rhs, cb, None):
+        raise MallardError(lib().mlr_last_error(h).decode())
 
 
 def probe_stream(kind, nbytes, reps=10):

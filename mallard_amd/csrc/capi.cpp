@@ -392,6 +392,84 @@ int mlr_get_face_values(mlr_ctx* ctx, double* out) {
     return guarded(ctx, [&] { ctx->solver.get_face_values(out); });
 }
 
+int mlr_calc_face_values(mlr_ctx* ctx, const double* U, double* face_solution) {
+    if (!ctx || !U || !face_solution) return 1;
+    return guarded(ctx, [&] { ctx->solver.calc_face_values_of(U, face_solution); });
+}
+
+int mlr_calc_rhs_of(mlr_ctx* ctx, const double* U, int parts, double* rhs) {
+    if (!ctx || !U || !rhs) return 1;
+    return guarded(ctx, [&] { ctx->solver.calc_rhs_of(U, parts, rhs); });
+}
+
+int mlr_calc_rhs_device(mlr_ctx* ctx, const double* U_dev, int parts, double* rhs_dev) {
+    if (!ctx || !U_dev || !rhs_dev) return 1;
+    return guarded(ctx, [&] { ctx->solver.calc_rhs_device(U_dev, parts, rhs_dev); });
+}
+
+int mlr_riemann_flux(int type, int guard, int64_t n, const double* n_unit, const double* left, const double* right,
+                     double* flux) {
+    return guarded(nullptr, [&] { mlr::dev::riemann_flux_host(type, guard, n, n_unit, left, right, flux); });
+}
+
+// time_integrator.cpp:57-163: the reference's sequence of rhs evaluations,
+// deep copies and BLAS-1 updates, each update one kernel on the stream
+int mlr_integrator_take_step(mlr_ctx* ctx, int type, double dt, int64_t n, double* const* solution_vec,
+                             double* const* rhs_vec, mlr_rhs_fn rhs, void* user) {
+    return guarded(ctx, [&] {
+        using mlr::fail;
+        if (type != MLR_TI_FE && type != MLR_TI_RK4 && type != MLR_TI_SSPRK3) fail("Unknown time integrator type.");
+        if (n < 0) fail("take_step: negative vector length");
+        if (!solution_vec || !rhs_vec || !rhs) fail("take_step: NULL vector list or rhs function");
+        const int n_sol = type == MLR_TI_FE ? 1 : 2, n_rhs = type == MLR_TI_FE ? 1 : (type == MLR_TI_RK4 ? 4 : 3);
+        for (int i = 0; i < n_sol; ++i)
+            if (!solution_vec[i]) fail("take_step: NULL solution vector");
+        for (int i = 0; i < n_rhs; ++i)
+            if (!rhs_vec[i]) fail("take_step: NULL rhs vector");
+        hipStream_t st = ctx ? ctx->solver.stream() : nullptr;
+        auto chk = [](hipError_t e) { if (e != hipSuccess) fail(hipGetErrorString(e)); };
+        auto calc = [&](const double* U, double* r) {
+            if (rhs(U, r, user) != 0) fail("take_step: the rhs function failed");
+        };
+        auto copy = [&](const double* x, double* y) { chk(mlr::dev::vec_update(0, n, 0.0, x, 0.0, y, st)); };
+        auto axpy = [&](double a, const double* x, double* y) { chk(mlr::dev::vec_update(1, n, a, x, 0.0, y, st)); };
+        auto axpby = [&](double a, const double* x, double b, double* y) {
+            chk(mlr::dev::vec_update(2, n, a, x, b, y, st));
+        };
+        double* U = solution_vec[0];
+        if (type == MLR_TI_FE) {
+            calc(U, rhs_vec[0]);
+            axpy(dt, rhs_vec[0], U);
+        } else if (type == MLR_TI_RK4) {
+            double* Ut = solution_vec[1];
+            const double c[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+            calc(U, rhs_vec[0]);
+            copy(U, Ut);
+            axpy(dt / 2.0, rhs_vec[0], Ut);
+            calc(Ut, rhs_vec[1]);
+            copy(U, Ut);
+            axpy(dt / 2.0, rhs_vec[1], Ut);
+            calc(Ut, rhs_vec[2]);
+            copy(U, Ut);
+            axpy(dt, rhs_vec[2], Ut);
+            calc(Ut, rhs_vec[3]);
+            for (int k = 0; k < 4; ++k) axpy(dt * c[k], rhs_vec[k], U);
+        } else {
+            double* Ut = solution_vec[1];
+            const double c[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+            calc(U, rhs_vec[0]);
+            copy(U, Ut);
+            axpy(dt, rhs_vec[0], Ut);
+            calc(Ut, rhs_vec[1]);
+            axpby(3.0 / 4.0, U, 1.0 / 4.0, Ut);
+            axpy(dt / 4.0, rhs_vec[1], Ut);
+            calc(Ut, rhs_vec[2]);
+            for (int k = 0; k < 3; ++k) axpy(dt * c[k], rhs_vec[k], U);
+        }
+        chk(hipStreamSynchronize(st));
+    });
+}
+
 int mlr_get_decisions(mlr_ctx* ctx, uint8_t* out) {
     if (!ctx || !out) return 1;
     return guarded(ctx, [&] { ctx->solver.get_decisions(out); });

@@ -153,6 +153,10 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
                       const Physics& ph, int occ, int gate_reuse, hipStream_t stream);
 // bs: cells (threads) per block
 hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, uint32_t bs, hipStream_t stream);
+// kRhsOnly residual over the faces `parts` selects: 1 interior, 2 boundary,
+// 3 both (the full residual, flux_update)
+hipError_t rhs_parts(const Geom& g, const StageArgs& a, const Physics& ph, int parts, uint32_t bs,
+                     hipStream_t stream);
 // spectral radii: lam_cell per cell and, per bs-cell block, the bit pattern
 // of the block's largest radius in lam_blk (NaN radii are ignored)
 hipError_t spectral_radius(const Geom& g, const double* Un, const double* prims, const Physics& ph,
@@ -181,6 +185,22 @@ hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blo
 hipError_t finalize_batch(const CaseBufs* cases, int n_cases, double cfl, double fixed_dt, int64_t hist_cap,
                           uint32_t bs, hipStream_t stream);
 hipError_t probe_math(const double* x, const double* y, double* out, int n, int which, hipStream_t stream);
+
+// Point-wise operators (operators.hip).
+// RiemannSolver::calc_flux of n face points: nrm[n][2] unit normals,
+// L, R[n][6] = (rho, u, v, p, gamma, h) left / right, F[n][4] fluxes; type is
+// the Riemann solver code, guard the vacuum guard of the two-rarefaction
+// estimate (0: the reference's formula)
+hipError_t riemann_points(int type, int guard, int64_t n, const double* nrm, const double* L, const double* R,
+                          double* F, hipStream_t stream);
+// vector updates of n doubles: kind 0 y = x (deep_copy), 1 y = y + a x
+// (axpy), 2 y = a x + b y (axpby)
+hipError_t vec_update(int kind, int64_t n, double a, const double* x, double b, double* y, hipStream_t stream);
+// The Riemann operator on HOST arrays (same layouts): stages the points
+// through device memory allocated for the call, on the null stream; throws
+// mlr::Error on failure
+void riemann_flux_host(int type, int guard, int64_t n, const double* nrm, const double* L, const double* R,
+                       double* F);
 
 // TENO pseudo-inverses on the GPU (precompute.hip), bit-identical to the host
 // precompute for the Legendre basis.

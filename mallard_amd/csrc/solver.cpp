@@ -1134,7 +1134,48 @@ void Solver::init_solution() {
     set_state(U.data(), W.data());
 }
 
-void Solver::get_face_values(double* out) {
+void Solver::get_face_values(double* out) { face_values_from(Un_, out); }
+
+void Solver::upload_state(const double* U_ref, double* dst_dev) {
+    if (!U_ref) fail("state is NULL");
+    const size_t N = L_.N;
+    std::vector<double> u(4 * N);
+    for (uint32_t j = 0; j < N; ++j) std::memcpy(&u[4 * (size_t)j], &U_ref[4 * (size_t)L_.perm[j]], 32);
+    synchronize();
+    HIP_OK(hipMemcpy(dst_dev, u.data(), u.size() * 8, hipMemcpyHostToDevice));
+}
+
+// The stage buffer Ua_ carries the caller's state: between steps it holds
+// nothing a later step reads before writing it.
+void Solver::calc_face_values_of(const double* U_ref, double* out) {
+    if (!out) fail("face_solution is NULL");
+    upload_state(U_ref, Ua_);
+    if (L_.teno) launch_teno(Ua_);
+    face_values_from(Ua_, out);
+}
+
+void Solver::calc_rhs_device(const double* U_dev, int parts, double* rhs_dev) {
+    if (parts < 1 || parts > 3) fail("calc_rhs: parts must be 1 (interior), 2 (boundaries) or 3 (both)");
+    if (!U_dev || !rhs_dev) fail("calc_rhs: NULL device array");
+    if (L_.teno) launch_teno(U_dev);
+    if (cfg_.viscous) HIP_OK(dev::viscous_grad(geom(), U_dev, ph_, vp_, vg_, tune_.flux_block, stream_));
+    dev::StageArgs a = stage(dev::kRhsOnly, U_dev, nullptr);
+    a.rhs = rhs_dev;
+    HIP_OK(dev::rhs_parts(geom(), a, ph_, parts, tune_.flux_block, stream_));
+    mark(kStatFluxMs);
+}
+
+void Solver::calc_rhs_of(const double* U_ref, int parts, double* rhs_ref) {
+    if (!rhs_ref) fail("rhs is NULL");
+    upload_state(U_ref, Ua_);
+    calc_rhs_device(Ua_, parts, rhs_);
+    std::vector<double> r(4 * (size_t)L_.N);
+    HIP_OK(hipMemcpyAsync(r.data(), rhs_, r.size() * 8, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    for (uint32_t j = 0; j < L_.N; ++j) std::memcpy(&rhs_ref[4 * (size_t)L_.perm[j]], &r[4 * (size_t)j], 32);
+}
+
+void Solver::face_values_from(const double* cells_dev, double* out) {
     const uint32_t Q = L_.Q, N = L_.N;
     const size_t nf = mesh_.n_faces;
     std::fill(out, out + nf * Q * 2 * 4, NAN);
@@ -1145,7 +1186,7 @@ void Solver::get_face_values(double* out) {
         HIP_OK(hipMemcpy(src.data(), fs_, src.size() * 8, hipMemcpyDeviceToHost));
     } else {
         src.resize(4 * (size_t)N);
-        HIP_OK(hipMemcpy(src.data(), Un_, src.size() * 8, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(src.data(), cells_dev, src.size() * 8, hipMemcpyDeviceToHost));
     }
     for (uint32_t f = 0; f < mesh_.n_faces_real; ++f)
         for (int side = 0; side < 2; ++side) {

@@ -116,6 +116,17 @@ public:
     // runs are issued in chunks of at most this many steps
     void set_dt_history(int64_t cap);
     void get_face_values(double* out);              // [n_faces alloc][Q][2][4], reference numbering
+    // FaceReconstruction::calc_face_values(solution, face_solution) on a
+    // caller's state (host, reference numbering) into out as get_face_values;
+    // the solver's state, time and dt are untouched
+    void calc_face_values_of(const double* U_ref, double* out);
+    // Solver::calc_rhs on a caller's state (host, reference numbering); parts:
+    // 1 interior faces (calc_rhs_interior), 2 boundary faces
+    // (calc_rhs_boundaries), 3 both, each divided by the volume
+    void calc_rhs_of(const double* U_ref, int parts, double* rhs_ref);
+    // the same on device arrays [N][4] in this solver's numbering, queued on
+    // its stream (no host copies, no synchronisation)
+    void calc_rhs_device(const double* U_dev, int parts, double* rhs_dev);
     void get_decisions(uint8_t* out);               // [N] bit v = central stencil for variable v
     void get_cfl_local(double* out);
     void check_fields();                            // solver.cpp:464-498
@@ -189,6 +200,8 @@ private:
     int64_t device_step();
     int64_t stream_step();  // the device step counter, read on the context's stream
     void finish_steps(int64_t s0, int64_t n, double* dts_out);
+    void face_values_from(const double* cells_dev, double* out);  // fs_ (TENO) or the cell states (FO)
+    void upload_state(const double* U_ref, double* dst_dev);     // reference numbering -> dst_dev
 
     Config cfg_;
     Mesh mesh_;

@@ -1002,111 +1002,144 @@ struct orc_solver {
         }
         return S;
     }
-    // face_reconstruction.cpp:406-475
+    // face_reconstruction.cpp:406-475. Each cell's stencils depend on the
+    // mesh alone, so cells are searched in parallel and concatenated in cell
+    // order (the same arrays as a serial search).
     void compute_stencils() {
+        const int64_t nc = m->n_cells;
+        std::vector<std::vector<std::vector<uint32_t>>> per((size_t)nc);
+        int not_full = 0;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1024) reduction(| : not_full)
+        for (int64_t ci = 0; ci < nc; ++ci) {
+            const uint32_t c = (uint32_t)ci;
+            auto& S = per[(size_t)ci];
+            S.push_back(stencil_centered(c));
+            for (auto& x : stencils_directional(c)) S.push_back(std::move(x));
+            for (auto& st_ : S)
+                if ((int)st_.size() < M && !st_.empty()) not_full = 1;
+        }
+        if (not_full) throw std::runtime_error("Stencil is not full.");
         off_groups.assign(1, 0);
         off_st.assign(1, 0);
+        size_t total = 0;
+        for (const auto& S : per)
+            for (const auto& x : S) total += x.size();
         st.clear();
-        for (uint32_t c = 0; c < m->n_cells; ++c) {
-            std::vector<std::vector<uint32_t>> S;
-            S.push_back(stencil_centered(c));
-            for (auto& x : stencils_directional(c)) S.push_back(x);
-            for (auto& s : S) {
-                if ((int)s.size() < M && !s.empty()) throw std::runtime_error("Stencil is not full.");
-                for (auto x : s) st.push_back(x);
+        st.reserve(total);
+        for (auto& S : per) {
+            for (auto& x : S) {
+                for (auto v : x) st.push_back(v);
                 off_st.push_back((uint32_t)st.size());
             }
             off_groups.push_back((uint32_t)off_st.size() - 1);
+            std::vector<std::vector<uint32_t>>().swap(S);
         }
     }
     double b2(int px, int py, double x, double y) const { return basis2(cfg.basis, px, py, x, y); }
-    // face_reconstruction.cpp:477-741
+    // face_reconstruction.cpp:477-741 for one cell: its stencils' pseudo-
+    // inverses into rm_out (K x size per stencil, stencils in order) and
+    // transformed member areas into ta_out. Cell 0's central stencil defines
+    // ipt (Q20), so cell 0 runs first with set_ipt.
+    void matrices_of_cell(uint32_t c, bool set_ipt, double* rm_out, double* ta_out) {
+        const int K = n_dof;
+        double J[4], Ji[4], w0[2], w1[2], w2[2];
+        for (int i = 0; i < ND; i++) {
+            w0[i] = m->nc(m->node_of_cell(c, 0), i);
+            w1[i] = m->nc(m->node_of_cell(c, 1), i);
+            w2[i] = m->nc(m->node_of_cell(c, 2), i);
+        }
+        tri_J(w0, w1, w2, J);
+        invert2(J, Ji);
+        const uint32_t s0 = off_groups[c];
+        const int gsz = off_groups[c + 1] - s0;
+        for (int sl = 0; sl < gsz; ++sl) {
+            const uint32_t si = s0 + sl;
+            const uint32_t soff = off_st[si];
+            const int ssz = off_st[si + 1] - soff;
+            if (ssz == 0) continue;
+            std::vector<double> at(ssz), A((size_t)ssz * K), R((size_t)ssz * K), Y((size_t)K * ssz);
+            for (int nl = 0; nl < ssz; ++nl) {
+                const uint32_t nb = st[soff + nl];
+                double v0[2], v1[2], v2[2], Jn[4];
+                for (int i = 0; i < ND; i++) {
+                    v0[i] = m->nc(m->node_of_cell(nb, 0), i);
+                    v1[i] = m->nc(m->node_of_cell(nb, 1), i);
+                    v2[i] = m->nc(m->node_of_cell(nb, 2), i);
+                }
+                tri_J(v0, v1, v2, Jn);
+                double t0[2], t1[2], t2[2];
+                for (int i = 0; i < ND; i++) { t0[i] = v0[i] - w0[i]; t1[i] = v1[i] - w0[i]; t2[i] = v2[i] - w0[i]; }
+                gemv2(Ji, t0, t0);
+                gemv2(Ji, t1, t1);
+                gemv2(Ji, t2, t2);
+                at[nl] = tri_area(t0, t1, t2);
+                std::vector<double> qp(2 * (size_t)qcell.n);
+                for (int q = 0; q < qcell.n; ++q) {
+                    double x[2] = {qcell.pts[2 * q], qcell.pts[2 * q + 1]};
+                    gemv2(Jn, x, x);
+                    for (int i = 0; i < ND; i++) x[i] += v0[i];
+                    for (int i = 0; i < ND; i++) x[i] -= w0[i];
+                    gemv2(Ji, x, x);
+                    qp[2 * q] = x[0]; qp[2 * q + 1] = x[1];
+                }
+                for (int k = 0; k < K; ++k) {
+                    const size_t ind = (size_t)nl * K + k;
+                    A[ind] = 0.0;
+                    for (int q = 0; q < qcell.n; ++q)
+                        A[ind] += qcell.w[q] * b2(poly[2 * k], poly[2 * k + 1], qp[2 * q], qp[2 * q + 1]);
+                    A[ind] *= at[nl];
+                }
+            }
+            if (set_ipt && sl == 0)
+                for (int k = 0; k < K; ++k) ipt[k] = A[k] / at[0];
+            for (int nl = 0; nl < ssz; ++nl)
+                for (int k = 0; k < K; ++k) A[(size_t)nl * K + k] -= at[nl] * ipt[k];
+            bool col0 = true;
+            for (int nl = 0; nl < ssz; ++nl)
+                if (A[(size_t)nl * K] > 1.0e-12) { col0 = false; break; }  // no fabs, as the reference
+            if (col0) {
+                std::vector<double> B((size_t)(ssz - 1) * (K - 1));
+                for (int nl = 1; nl < ssz; ++nl)
+                    for (int k = 1; k < K; ++k) B[(size_t)(nl - 1) * (K - 1) + (k - 1)] = A[(size_t)nl * K + k];
+                qr_noq(B.data(), R.data(), ssz - 1, K - 1);
+                forward_sub(R.data(), B.data(), Y.data(), K - 1, K - 1, ssz - 1, true, true);
+                back_sub(R.data(), Y.data(), B.data(), K - 1, K - 1, ssz - 1, false, false);
+                for (int k = 0; k < K; ++k)
+                    for (int nl = 0; nl < ssz; ++nl)
+                        A[(size_t)k * ssz + nl] = (k == 0 || nl == 0) ? 0.0 : B[(size_t)(k - 1) * (ssz - 1) + (nl - 1)];
+            } else {
+                qr_noq(A.data(), R.data(), ssz, K);
+                forward_sub(R.data(), A.data(), Y.data(), K, K, ssz, true, true);
+                back_sub(R.data(), Y.data(), A.data(), K, K, ssz, false, false);
+            }
+            std::memcpy(rm_out + (size_t)K * (soff - off_st[s0]), A.data(), A.size() * 8);
+            std::memcpy(ta_out + (soff - off_st[s0]), at.data(), (size_t)ssz * 8);
+        }
+    }
+    // face_reconstruction.cpp:477-741: the stencil matrices and member areas
+    // of every cell. Entries are laid out like the stencil ids (K per member,
+    // one per member), so every cell's slice is known in advance and the cells
+    // after cell 0 are computed in parallel.
     void compute_matrices() {
         const int K = n_dof;
-        off_rm.assign(1, 0);
-        rm.clear();
-        ta.clear();
-        ipt.assign(K, 0.0);
-        for (uint32_t c = 0; c < m->n_cells; ++c) {
+        const uint32_t nc = m->n_cells;
+        for (uint32_t c = 0; c < nc; ++c) {
             if (m->n_nodes_of_cell(c) != 3) throw std::runtime_error("TENO has only been implemented for triangular cells.");
-            double J[4], Ji[4], w0[2], w1[2], w2[2];
-            for (int i = 0; i < ND; i++) {
-                w0[i] = m->nc(m->node_of_cell(c, 0), i);
-                w1[i] = m->nc(m->node_of_cell(c, 1), i);
-                w2[i] = m->nc(m->node_of_cell(c, 2), i);
-            }
-            tri_J(w0, w1, w2, J);
-            invert2(J, Ji);
-            const uint32_t s0 = off_groups[c];
-            const int gsz = off_groups[c + 1] - s0;
-            for (int sl = 0; sl < gsz; ++sl) {
-                const uint32_t si = s0 + sl;
-                const uint32_t soff = off_st[si];
-                const int ssz = off_st[si + 1] - soff;
-                if (ssz == 0) {
-                    if (sl == 0) throw std::runtime_error("Central stencil is empty for cell " + std::to_string(c) + ".");
-                    off_rm.push_back(rm.size());
-                    continue;
-                }
-                std::vector<double> at(ssz), A((size_t)ssz * K), R((size_t)ssz * K), Y((size_t)K * ssz);
-                for (int nl = 0; nl < ssz; ++nl) {
-                    const uint32_t nb = st[soff + nl];
-                    double v0[2], v1[2], v2[2], Jn[4];
-                    for (int i = 0; i < ND; i++) {
-                        v0[i] = m->nc(m->node_of_cell(nb, 0), i);
-                        v1[i] = m->nc(m->node_of_cell(nb, 1), i);
-                        v2[i] = m->nc(m->node_of_cell(nb, 2), i);
-                    }
-                    tri_J(v0, v1, v2, Jn);
-                    double t0[2], t1[2], t2[2];
-                    for (int i = 0; i < ND; i++) { t0[i] = v0[i] - w0[i]; t1[i] = v1[i] - w0[i]; t2[i] = v2[i] - w0[i]; }
-                    gemv2(Ji, t0, t0);
-                    gemv2(Ji, t1, t1);
-                    gemv2(Ji, t2, t2);
-                    at[nl] = tri_area(t0, t1, t2);
-                    std::vector<double> qp(2 * (size_t)qcell.n);
-                    for (int q = 0; q < qcell.n; ++q) {
-                        double x[2] = {qcell.pts[2 * q], qcell.pts[2 * q + 1]};
-                        gemv2(Jn, x, x);
-                        for (int i = 0; i < ND; i++) x[i] += v0[i];
-                        for (int i = 0; i < ND; i++) x[i] -= w0[i];
-                        gemv2(Ji, x, x);
-                        qp[2 * q] = x[0]; qp[2 * q + 1] = x[1];
-                    }
-                    for (int k = 0; k < K; ++k) {
-                        const size_t ind = (size_t)nl * K + k;
-                        A[ind] = 0.0;
-                        for (int q = 0; q < qcell.n; ++q)
-                            A[ind] += qcell.w[q] * b2(poly[2 * k], poly[2 * k + 1], qp[2 * q], qp[2 * q + 1]);
-                        A[ind] *= at[nl];
-                    }
-                }
-                if (c == 0 && sl == 0)
-                    for (int k = 0; k < K; ++k) ipt[k] = A[k] / at[0];
-                for (int nl = 0; nl < ssz; ++nl)
-                    for (int k = 0; k < K; ++k) A[(size_t)nl * K + k] -= at[nl] * ipt[k];
-                bool col0 = true;
-                for (int nl = 0; nl < ssz; ++nl)
-                    if (A[(size_t)nl * K] > 1.0e-12) { col0 = false; break; }  // no fabs, as the reference
-                if (col0) {
-                    std::vector<double> B((size_t)(ssz - 1) * (K - 1));
-                    for (int nl = 1; nl < ssz; ++nl)
-                        for (int k = 1; k < K; ++k) B[(size_t)(nl - 1) * (K - 1) + (k - 1)] = A[(size_t)nl * K + k];
-                    qr_noq(B.data(), R.data(), ssz - 1, K - 1);
-                    forward_sub(R.data(), B.data(), Y.data(), K - 1, K - 1, ssz - 1, true, true);
-                    back_sub(R.data(), Y.data(), B.data(), K - 1, K - 1, ssz - 1, false, false);
-                    for (int k = 0; k < K; ++k)
-                        for (int nl = 0; nl < ssz; ++nl)
-                            A[(size_t)k * ssz + nl] = (k == 0 || nl == 0) ? 0.0 : B[(size_t)(k - 1) * (ssz - 1) + (nl - 1)];
-                } else {
-                    qr_noq(A.data(), R.data(), ssz, K);
-                    forward_sub(R.data(), A.data(), Y.data(), K, K, ssz, true, true);
-                    back_sub(R.data(), Y.data(), A.data(), K, K, ssz, false, false);
-                }
-                for (auto v : A) rm.push_back(v);
-                off_rm.push_back(rm.size());
-                for (int nl = 0; nl < ssz; ++nl) ta.push_back(at[nl]);
-            }
+            if (off_st[off_groups[c] + 1] == off_st[off_groups[c]])
+                throw std::runtime_error("Central stencil is empty for cell " + std::to_string(c) + ".");
+        }
+        ipt.assign(K, 0.0);
+        off_rm.resize(off_st.size());
+        for (size_t i = 0; i < off_st.size(); ++i) off_rm[i] = (uint64_t)K * off_st[i];
+        rm.assign((size_t)K * st.size(), 0.0);
+        ta.assign(st.size(), 0.0);
+        if (nc == 0) return;
+        matrices_of_cell(0, true, rm.data(), ta.data());
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1024)
+        for (int64_t ci = 1; ci < (int64_t)nc; ++ci) {
+            const uint32_t c = (uint32_t)ci;
+            const size_t o = off_st[off_groups[c]];
+            matrices_of_cell(c, false, rm.data() + (size_t)K * o, ta.data() + o);
         }
     }
     // face_reconstruction.cpp:743-788
@@ -1387,17 +1420,20 @@ struct orc_solver {
         }
     }
 
-    // solver_rhs.cpp:44-115
-    void calc_rhs(const double* Ux, double* r) {
+    // solver_rhs.cpp:44-115; parts: bit 0 calc_rhs_interior (:74-108),
+    // bit 1 calc_rhs_boundaries (:110-115)
+    void calc_rhs(const double* Ux, double* r, int parts = 3) {
         g_vacuum_guard = cfg.recon == ORC_RECON_TENO && cfg.teno_variant == ORC_TENO_CORRECTED;
         const size_t nc = m->n_cells;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
         for (int64_t i = 0; i < (int64_t)(nc * NC); ++i) r[i] = 0.0;
         if (cfg.recon == ORC_RECON_FO) fo_faces(Ux); else teno_faces(Ux);
         const auto& zi = m->zones[0].faces;
+        if (parts & 1) {
 #pragma omp parallel for num_threads(nthreads) schedule(static)
-        for (int64_t i = 0; i < (int64_t)zi.size(); ++i) face_flux(zi[i], -1, nullptr, r);
-        for (int b = 0; b < cfg.n_bc; ++b) {
+            for (int64_t i = 0; i < (int64_t)zi.size(); ++i) face_flux(zi[i], -1, nullptr, r);
+        }
+        for (int b = 0; b < cfg.n_bc && (parts & 2); ++b) {
             const auto& zf = m->zones[cfg.bc[b].zone].faces;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
             for (int64_t i = 0; i < (int64_t)zf.size(); ++i) face_flux(zf[i], cfg.bc[b].type, &cfg.bc[b], r);
@@ -1572,6 +1608,11 @@ void orc_get_state(const orc_solver* s, double* U, double* prim) {
     if (U) std::memcpy(U, s->U.data(), s->U.size() * 8);
     if (prim) std::memcpy(prim, s->prim.data(), s->prim.size() * 8);
 }
+int orc_calc_rhs_parts(orc_solver* s, const double* U, int parts, double* rhs) {
+    ORC_TRY(s->calc_rhs(U, rhs, parts), return 1);
+    return 0;
+}
+
 int orc_calc_rhs(orc_solver* s, const double* U, double* rhs) {
     ORC_TRY(s->calc_rhs(U, rhs), return 1);
     return 0;
@@ -1607,10 +1648,20 @@ void orc_teno_decisions(const orc_solver* s, uint8_t* out) {
     std::memcpy(out, s->decisions.data(), s->decisions.size());
 }
 
+// RiemannSolver::calc_flux as the reference writes it (no vacuum guard,
+// whatever the last residual evaluation used)
 void orc_riemann_flux(int type, double* F, const double* n, double rl, const double* ul, double pl,
                       double gl, double hl, double rr, const double* ur, double pr, double gr,
                       double hr) {
+    orc_riemann_flux_guarded(type, 0, F, n, rl, ul, pl, gl, hl, rr, ur, pr, gr, hr);
+}
+void orc_riemann_flux_guarded(int type, int guard, double* F, const double* n, double rl, const double* ul,
+                              double pl, double gl, double hl, double rr, const double* ur, double pr,
+                              double gr, double hr) {
+    const int saved = g_vacuum_guard;
+    g_vacuum_guard = guard != 0;
     flux_fn(type)(F, n, rl, ul, pl, gl, hl, rr, ur, pr, gr, hr);
+    g_vacuum_guard = saved;
 }
 void orc_physics_constants(double g, double pr, double Tr, double rr, double* o) {
     Euler e;

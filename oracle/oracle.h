@@ -94,6 +94,9 @@ void orc_set_state(orc_solver* s, const double* U, const double* prim);
 void orc_get_state(const orc_solver* s, double* U, double* prim);
 /* One residual evaluation R(U) into rhs (N_c x 4). */
 int orc_calc_rhs(orc_solver* s, const double* U, double* rhs);
+/* The residual over the interior faces (parts 1), the boundary zones (2) or
+   both (3), divided by the volume (solver_rhs.cpp:44-115). */
+int orc_calc_rhs_parts(orc_solver* s, const double* U, int parts, double* rhs);
 /* Face states of the last residual evaluation, [n_faces alloc][Q][2][4]. */
 void orc_get_face_values(const orc_solver* s, double* out);
 int orc_calc_dt(orc_solver* s, double cfl, double* dt_out, double* cfl_local);
@@ -107,10 +110,16 @@ double orc_time(const orc_solver* s);
 void orc_teno_decisions(const orc_solver* s, uint8_t* out);
 
 /* ---- pointwise routines pinned by the reference's unit tests ---- */
+/* RiemannSolver::calc_flux as the reference writes it (no vacuum guard) */
 void orc_riemann_flux(int type, double* flux, const double* n_unit, double rho_l,
                       const double* u_l, double p_l, double gamma_l, double h_l,
                       double rho_r, const double* u_r, double p_r, double gamma_r,
                       double h_r);
+/* the same with the corrected variant's vacuum guard on (guard != 0) or off */
+void orc_riemann_flux_guarded(int type, int guard, double* flux, const double* n_unit, double rho_l,
+                              const double* u_l, double p_l, double gamma_l, double h_l,
+                              double rho_r, const double* u_r, double p_r, double gamma_r,
+                              double h_r);
 void orc_physics_constants(double gamma, double p_ref, double T_ref, double rho_ref,
                            double* R_cp_cv);
 void orc_primitives(const double* consts /* gamma,R,cp,cv,p_min,p_max */,

@@ -68,6 +68,7 @@ def lib():
         L.orc_set_state.argtypes = [P, C.c_void_p, C.c_void_p]
         L.orc_get_state.argtypes = [P, C.c_void_p, C.c_void_p]
         L.orc_calc_rhs.argtypes = [P, C.c_void_p, C.c_void_p]
+        L.orc_calc_rhs_parts.argtypes = [P, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_get_face_values.argtypes = [P, C.c_void_p]
         L.orc_calc_dt.argtypes = [P, C.c_double, dp, C.c_void_p]
         L.orc_take_step.argtypes = [P, C.c_double]
@@ -75,6 +76,10 @@ def lib():
         L.orc_time.restype = C.c_double
         L.orc_time.argtypes = [P]
         L.orc_teno_decisions.argtypes = [P, C.c_void_p]
+        L.orc_riemann_flux_guarded.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                               C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                               C.c_double, C.c_void_p, C.c_double, C.c_double,
+                                               C.c_double]
         L.orc_riemann_flux.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p,
                                        C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_void_p, C.c_double, C.c_double, C.c_double]
@@ -246,6 +251,14 @@ class Solver:
             raise RuntimeError(lib().orc_last_error().decode())
         return r
 
+    def calc_rhs_parts(self, U, parts):
+        """Residual over the interior faces (1), the boundary zones (2) or both (3)."""
+        U = _f64(U)
+        r = np.zeros((self.nc, 4))
+        if lib().orc_calc_rhs_parts(self.h, _p(U), parts, _p(r)):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return r
+
     def face_values(self, nq):
         out = np.zeros((self.mesh.n_faces, nq, 2, 4))
         lib().orc_get_face_values(self.h, _p(out))
@@ -279,9 +292,9 @@ class Solver:
         return d
 
 
-def riemann_flux(kind, n, rho_l, u_l, p_l, g_l, h_l, rho_r, u_r, p_r, g_r, h_r):
+def riemann_flux(kind, n, rho_l, u_l, p_l, g_l, h_l, rho_r, u_r, p_r, g_r, h_r, guard=0):
     F = np.zeros(4)
     n, ul, ur = _f64(n), _f64(u_l), _f64(u_r)
-    lib().orc_riemann_flux(RIEMANN[kind], _p(F), _p(n), rho_l, _p(ul), p_l, g_l, h_l, rho_r,
-                           _p(ur), p_r, g_r, h_r)
+    lib().orc_riemann_flux_guarded(RIEMANN[kind], guard, _p(F), _p(n), rho_l, _p(ul), p_l, g_l, h_l, rho_r,
+                                   _p(ur), p_r, g_r, h_r)
     return F

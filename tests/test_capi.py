@@ -110,3 +110,26 @@ def test_host_pow_special_operands_follow_c99(mlr):
         assert (np.isinf(h) == np.isinf(g)).all()
         fin = np.isfinite(g)
         assert np.abs(h[fin].view(np.int64) - g[fin].view(np.int64)).max() <= 1
+
+
+def test_operator_entry_points_refuse_bad_arguments_on_the_host(mlr):
+    """The operator entry points check their arguments before touching the
+    GPU: unknown Riemann solver or integrator codes are refused with the
+    reference's messages, NULL arrays with an error status."""
+    import ctypes as C
+    L = mlr.lib()
+    one = np.zeros(8)
+    p = one.ctypes.data_as(C.c_void_p)
+    assert L.mlr_riemann_flux(7, 0, 1, p, p, p, p) != 0
+    assert "Unknown Riemann solver type" in L.mlr_last_error(None).decode()
+    assert L.mlr_riemann_flux(2, 0, -1, p, p, p, p) != 0
+    assert L.mlr_riemann_flux(2, 0, 1, None, p, p, p) != 0
+    assert L.mlr_riemann_flux(2, 0, 0, None, None, None, None) == 0  # nothing to do
+    cb = mlr.RHS_FN(lambda u, r, user: 0)
+    vecs = (C.c_void_p * 4)(p, p, p, p)
+    assert L.mlr_integrator_take_step(None, 5, 0.1, 8, vecs, vecs, cb, None) != 0
+    assert "Unknown time integrator type" in L.mlr_last_error(None).decode()
+    assert L.mlr_integrator_take_step(None, 2, 0.1, 8, None, vecs, cb, None) != 0
+    assert L.mlr_calc_face_values(None, p, p) != 0
+    assert L.mlr_calc_rhs_of(None, p, 3, p) != 0
+    assert L.mlr_calc_rhs_device(None, p, 3, p) != 0

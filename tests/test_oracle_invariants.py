@@ -388,3 +388,16 @@ def test_pseudo_inverse_matches_numpy_svd(oracle, order, jit):
             err = np.abs(got[1:, 1:] - ref).max() / np.abs(ref).max()
             worst = max(worst, err)
     assert worst < 1e-10, worst
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_precompute_independent_of_thread_count(oracle, order):
+    """The oracle searches stencils and builds the pseudo-inverses cell by
+    cell in parallel; every array equals the one-thread build bit for bit."""
+    m = oracle.Mesh("cartesian_tri", 30, 26, 1.0, 1.0, 0.2, 41)
+    cfg = oracle.make_config(order=order, teno_variant="corrected")
+    one = oracle.Solver(m, cfg, 1).teno()
+    many = oracle.Solver(m, cfg, 8).teno()
+    for k, v in one.items():
+        if isinstance(v, np.ndarray):
+            assert np.array_equal(v, many[k]), k
