@@ -224,6 +224,18 @@ int mlr_calc_rhs_device(mlr_ctx* ctx, const double* U_dev, int parts, double* rh
 int mlr_riemann_flux(int type, int guard, int64_t n, const double* n_unit, const double* left, const double* right,
                      double* flux);
 
+/* mlr_riemann_flux without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The points are staged through one fixed device array of
+ * mlr_riemann_flux_noalloc_capacity() points (65,536 points, 9.4 MB), which
+ * every GPU the library runs on holds until the process ends; larger batches
+ * are refused with an error. Calls on one GPU share that array, so calls that
+ * overlap on one GPU (from several host threads) race. */
+int mlr_riemann_flux_noalloc(int type, int guard, int64_t n, const double* n_unit, const double* left,
+                             const double* right, double* flux);
+int64_t mlr_riemann_flux_noalloc_capacity(void);
+
 /* TimeIntegrator::take_step(dt, solution_vec, face_solution, rhs_vec, calc_rhs)
  * (time_integrator.h:41-99, time_integrator.cpp:57-163) on DEVICE vectors of n
  * doubles: solution_vec[0] is the state, advanced in place; solution_vec[1]

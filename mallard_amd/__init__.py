@@ -142,6 +142,8 @@ _SIGS = {
     "mlr_calc_rhs_of": (C.c_int, [_P, _P, C.c_int, _P]),
     "mlr_calc_rhs_device": (C.c_int, [_P, _P, C.c_int, _P]),
     "mlr_riemann_flux": (C.c_int, [C.c_int, C.c_int, C.c_int64, _P, _P, _P, _P]),
+    "mlr_riemann_flux_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int64, _P, _P, _P, _P]),
+    "mlr_riemann_flux_noalloc_capacity": (C.c_int64, []),
     "mlr_integrator_take_step": (C.c_int, [_P, C.c_int, C.c_double, C.c_int64, _P, _P, RHS_FN, _P]),
 }
 
@@ -712,6 +714,24 @@ def riemann_flux(rs, n_unit, left, right, guard=0):
     if lib().mlr_riemann_flux(t, guard, n, _p(n_unit), _p(left), _p(right), _p(out)):
         raise MallardError(lib().mlr_last_error(None).decode())
     return out
+
+
+def riemann_flux_noalloc(rs, n_unit, left, right, guard=0):
+    """riemann_flux staged through the library's fixed device scratch (no
+    device allocation; at most riemann_flux_noalloc_capacity() points)."""
+    n_unit, left, right = _f64(n_unit), _f64(left), _f64(right)
+    n = n_unit.shape[0]
+    if n_unit.shape != (n, 2) or left.shape != (n, 6) or right.shape != (n, 6):
+        raise ValueError("riemann_flux_noalloc: n_unit[n][2], left[n][6], right[n][6]")
+    out = np.zeros((n, 4))
+    t = RIEMANN[rs] if isinstance(rs, str) else int(rs)
+    if lib().mlr_riemann_flux_noalloc(t, guard, n, _p(n_unit), _p(left), _p(right), _p(out)):
+        raise MallardError(lib().mlr_last_error(None).decode())
+    return out
+
+
+def riemann_flux_noalloc_capacity():
+    return int(lib().mlr_riemann_flux_noalloc_capacity())
 
 
 def integrator_take_step(integrator, dt, n, solution_ptrs, rhs_ptrs, rhs_fn, solver=None):

@@ -412,6 +412,13 @@ int mlr_riemann_flux(int type, int guard, int64_t n, const double* n_unit, const
     return guarded(nullptr, [&] { mlr::dev::riemann_flux_host(type, guard, n, n_unit, left, right, flux); });
 }
 
+int mlr_riemann_flux_noalloc(int type, int guard, int64_t n, const double* n_unit, const double* left,
+                             const double* right, double* flux) {
+    return guarded(nullptr, [&] { mlr::dev::riemann_flux_noalloc(type, guard, n, n_unit, left, right, flux); });
+}
+
+int64_t mlr_riemann_flux_noalloc_capacity(void) { return mlr::dev::riemann_flux_noalloc_capacity(); }
+
 // time_integrator.cpp:57-163: the reference's sequence of rhs evaluations,
 // deep copies and BLAS-1 updates, each update one kernel on the stream
 int mlr_integrator_take_step(mlr_ctx* ctx, int type, double dt, int64_t n, double* const* solution_vec,

@@ -201,6 +201,12 @@ hipError_t vec_update(int kind, int64_t n, double a, const double* x, double b, 
 // mlr::Error on failure
 void riemann_flux_host(int type, int guard, int64_t n, const double* nrm, const double* L, const double* R,
                        double* F);
+// The same without allocating: the points are staged through a fixed device
+// array of riemann_flux_noalloc_capacity() points; larger batches are refused.
+// Calls that overlap on one device share that array and race.
+void riemann_flux_noalloc(int type, int guard, int64_t n, const double* nrm, const double* L, const double* R,
+                          double* F);
+int64_t riemann_flux_noalloc_capacity();
 
 // TENO pseudo-inverses on the GPU (precompute.hip), bit-identical to the host
 // precompute for the Legendre basis.

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "common.hpp"
@@ -105,6 +106,50 @@ void run_points(int type, int guard, int64_t n, const double* nrm, const double*
 }
 
 }  // namespace
+
+namespace {
+
+// Fixed staging area of the allocation-free Riemann entry point: room for
+// kNoAllocPoints face points (18 doubles each, 9.4 MB per device), held by
+// every device the library runs on until the process ends.
+constexpr int64_t kNoAllocPoints = int64_t(1) << 16;
+__device__ __align__(256) double scratch[kNoAllocPoints * kPointDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current. In host code the array's name is only a
+// stand-in: kernels and copies take this address instead.
+double* scratch_address() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "riemann flux: device query");
+    if (dev < 0 || dev >= kMaxDevices) fail("riemann flux: device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "riemann flux: scratch address");
+        if (!p) fail("riemann flux: scratch address is NULL");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+int64_t riemann_flux_noalloc_capacity() { return kNoAllocPoints; }
+
+void riemann_flux_noalloc(int type, int guard, int64_t n, const double* nrm, const double* L, const double* R,
+                          double* F) {
+    check_points(type, n, nrm, L, R, F);
+    if (n > kNoAllocPoints)
+        fail("riemann flux (no allocation): " + std::to_string(n) + " points exceed the fixed scratch of " +
+             std::to_string(kNoAllocPoints));
+    double* dev = scratch_address();
+    if (n == 0) return;
+    run_points(type, guard, n, nrm, L, R, F, dev);
+}
 
 void riemann_flux_host(int type, int guard, int64_t n, const double* nrm, const double* L, const double* R,
                        double* F) {
