@@ -104,7 +104,10 @@ struct StageArgs {
     int32_t op;
     int32_t riemann;
     int32_t fo;           // 1: face states are the cell averages
-    int32_t vacuum_guard;
+    // 1: the corrected TENO variant's flux rules: the vacuum guard of the
+    // two-rarefaction estimate, and extrapolation boundaries whose ghost state
+    // is the cell mean (a zero-gradient ghost cell) instead of the face state
+    int32_t corrected;
     const double* Uin;    // stage input state (gathered), [N][4]
     const double* Un;     // step start state
     double* Uout;
@@ -132,7 +135,7 @@ struct CaseBufs {
 };
 enum BufRole : int32_t { kBufUn = 0, kBufUa = 1, kBufUb = 2, kBufNone = 3 };
 struct BatchStage {
-    int32_t op, fo, vacuum_guard;
+    int32_t op, fo, corrected;
     int32_t in, out;  // BufRole of the stage input and output
 };
 

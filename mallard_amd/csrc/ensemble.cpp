@@ -145,9 +145,9 @@ void Ensemble::autotune() {
         return ms;
     };
     dev::Tune best = tune_;
-    const int vac = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
+    const int corrected = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
     const int op = cfg_.integrator == Integrator::RK4 ? dev::kRK4_1 : dev::kSSP1;
-    const dev::BatchStage st{op, teno_ ? 0 : 1, vac, dev::kBufUn, dev::kBufUa};
+    const dev::BatchStage st{op, teno_ ? 0 : 1, corrected, dev::kBufUn, dev::kBufUa};
     float bt = 1e30f;
     for (uint32_t bs : {64u, 128u, 256u}) {
         tune_.flux_block = tune_.spec_block = bs;
@@ -228,7 +228,7 @@ void Ensemble::launch_step(bool prof) {
     hip_ok(dev::finalize_batch(d_cases_, n, cfg_.cfl, cfg_.use_cfl ? 0.0 : cfg_.dt, cap, bs, stream_),
            "finalize_batch");
     mark(kStatDtMs);
-    const int vac = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
+    const int corrected = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
     auto stage = [&](int op, int in, int out) {
         if (teno_) {
             const bool first = op == dev::kSSP1 || op == dev::kRK4_1;
@@ -238,7 +238,7 @@ void Ensemble::launch_step(bool prof) {
                    "teno_batch");
             mark(kStatTenoMs);
         }
-        const dev::BatchStage st{op, teno_ ? 0 : 1, vac, in, out};
+        const dev::BatchStage st{op, teno_ ? 0 : 1, corrected, in, out};
         hip_ok(dev::flux_batch(d_cases_, n, flux_blocks_, st, (int)cfg_.riemann, ph_, bs, stream_), "flux_batch");
         mark(kStatFluxMs);
     };

@@ -687,9 +687,13 @@ __device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, con
                 oth = a.fo ? ld4(a.Uin, nbc) : ld4(a.fs, (size_t)(nbs * Q + q) * N + nbc);
                 const double* L = side ? oth.v : own.v;
                 const double* R = side ? own.v : oth.v;
-                point_flux(ph, g.bc, RS, a.vacuum_guard, false, 0, n0, n1, L, R, Ft);
+                point_flux(ph, g.bc, RS, a.corrected, false, 0, n0, n1, L, R, Ft);
             } else {
-                point_flux(ph, g.bc, RS, a.vacuum_guard, true, (inf >> 4) & 0xFu, n0, n1, own.v, own.v, Ft);
+                // the ghost state an extrapolation boundary copies: the face
+                // state itself (boundary_extrapolation.cpp:38-55), or in the
+                // corrected variant the cell mean
+                const D4 ghost = a.corrected ? ld4(a.Uin, c) : own;
+                point_flux(ph, g.bc, RS, a.corrected, true, (inf >> 4) & 0xFu, n0, n1, own.v, ghost.v, Ft);
             }
             const double wq = g.qw[q];
 #pragma unroll
@@ -809,7 +813,7 @@ __global__ __launch_bounds__(256) void flux_batch_kernel(const CaseBufs* __restr
     a.op = st.op;
     a.riemann = RS;
     a.fo = st.fo;
-    a.vacuum_guard = st.vacuum_guard;
+    a.corrected = st.corrected;
     a.Uin = case_buf(cs, st.in);
     a.Un = cs.Un;
     a.Uout = case_buf_w(cs, st.out);

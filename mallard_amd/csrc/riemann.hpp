@@ -179,6 +179,8 @@ __device__ inline void riemann_flux(int type, int guard, double* F, double n0, d
 
 // Flux through one face point. Boundary states follow boundary_*.cpp; the
 // wall returns the pressure-only flux (boundary_wall_adiabatic.cpp:38-70).
+// On a boundary face Cl is the face state and Cr the state an extrapolation
+// boundary copies into its ghost (the face state itself in the reference).
 __device__ inline void point_flux(const Physics& ph, const BCParam* __restrict__ bct, int riemann, int guard,
                                   bool bnd, int bc, double n0,
                                   double n1, const double* Cl, const double* Cr, double* F) {
@@ -196,9 +198,9 @@ __device__ inline void point_flux(const Physics& ph, const BCParam* __restrict__
             Wr[0] = Wl[0] - 2.0 * un * n0;
             Wr[1] = Wl[1] - 2.0 * un * n1;
             rr = Cl[0];
-        } else if (b.type == 1) {  // extrapolation
-            for (int i = 0; i < 5; i++) Wr[i] = Wl[i];
-            rr = Cl[0];
+        } else if (b.type == 1) {  // extrapolation: the ghost is the state Cr
+            prims(ph, Cr, Wr);
+            rr = Cr[0];
         } else if (b.type == 2 || b.type == 5) {  // adiabatic wall (slip; no-slip adds viscous terms)
             F[0] = 0.0; F[1] = Wl[2] * n0; F[2] = Wl[2] * n1; F[3] = 0.0;
             return;

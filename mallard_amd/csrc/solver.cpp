@@ -637,7 +637,7 @@ dev::StageArgs Solver::stage(int op, const double* Uin, double* Uout) const {
     a.op = op;
     a.riemann = (int32_t)cfg_.riemann;
     a.fo = L_.teno ? 0 : 1;
-    a.vacuum_guard = (L_.teno && L_.tspec.variant == TenoVariant::Corrected) ? 1 : 0;
+    a.corrected = (L_.teno && L_.tspec.variant == TenoVariant::Corrected) ? 1 : 0;
     a.Uin = Uin;
     a.Un = Un_;
     a.Uout = Uout;

@@ -1340,6 +1340,45 @@ struct orc_solver {
         }
     }
 
+    // Canonical mode: each face's area-weighted flux goes to ff, and every
+    // cell then adds its faces' contributions in the order of a one-thread run
+    // (interior faces in zone order, then each boundary zone in input order),
+    // so any thread count gives the one-thread result bit for bit. The default
+    // mode adds them with atomics as they come, like the reference.
+    bool canonical = false;
+    std::vector<double> ff;                 // [n_faces][NC], area * F
+    std::vector<uint32_t> cf_off;           // per cell: its entries in cf
+    std::vector<int64_t> cf;                // face id, negated plus one for side 0, tagged boundary below
+    std::vector<uint8_t> cf_bnd;            // 1 when the entry is a boundary face
+    void build_canonical_order() {
+        const size_t nc = m->n_cells;
+        std::vector<std::vector<std::pair<int64_t, int64_t>>> per(nc);  // (rank, signed face)
+        int64_t rank = 0;
+        auto add_zone = [&](const std::vector<uint32_t>& faces, bool bnd) {
+            for (uint32_t f : faces) {
+                const int32_t c0 = m->cof(f, 0), c1 = m->cof(f, 1);
+                per[c0].push_back({rank, -(int64_t)f - 1});
+                if (!bnd && c1 >= 0) per[c1].push_back({rank, (int64_t)f});
+                ++rank;
+            }
+        };
+        add_zone(m->zones[0].faces, false);
+        const int64_t first_bnd = rank;
+        for (int b = 0; b < cfg.n_bc; ++b) add_zone(m->zones[cfg.bc[b].zone].faces, true);
+        cf_off.assign(nc + 1, 0);
+        cf.clear();
+        cf_bnd.clear();
+        for (size_t c = 0; c < nc; ++c) {
+            std::sort(per[c].begin(), per[c].end());
+            for (auto& e : per[c]) {
+                cf.push_back(e.second);
+                cf_bnd.push_back(e.first >= first_bnd ? 1 : 0);
+            }
+            cf_off[c + 1] = (uint32_t)cf.size();
+        }
+        ff.assign((size_t)m->n_faces * NC, 0.0);
+    }
+
     inline void add_rhs(double* r, uint32_t c, int v, double x) {
         double& dst = r[(size_t)c * NC + v];
         if (nthreads > 1) {
@@ -1376,9 +1415,20 @@ struct orc_solver {
                 const double un = dot2(ul, nu);
                 for (int i = 0; i < ND; i++) Wr[i] = ul[i] - 2.0 * un * nu[i];
             } else if (bc_type == ORC_BC_EXTRAPOLATION) {
-                for (int v = 0; v < NC; v++) Cr[v] = Cl[v];
+                // boundary_extrapolation.cpp:38-55: the ghost copies the face
+                // state. The CORRECTED variant's ghost is the cell mean (a
+                // zero-gradient ghost cell): with the face state the
+                // reconstruction's boundary values feed back without upwinding
+                // and the four-quadrant problem grows without bound there.
                 phys.prims(Wl, Cl);
-                for (int i = 0; i < NP; i++) Wr[i] = Wl[i];
+                if (cfg.recon == ORC_RECON_TENO && cfg.teno_variant == ORC_TENO_CORRECTED) {
+                    const int32_t c0 = m->cof(f, 0);
+                    for (int v = 0; v < NC; v++) Cr[v] = Ucur[(size_t)c0 * NC + v];
+                    phys.prims(Wr, Cr);
+                } else {
+                    for (int v = 0; v < NC; v++) Cr[v] = Cl[v];
+                    for (int i = 0; i < NP; i++) Wr[i] = Wl[i];
+                }
             } else if (bc_type == ORC_BC_WALL_ADIABATIC) {
                 phys.prims(Wl, Cl);
                 Ft[0] = 0.0; Ft[1] = Wl[2] * nu[0]; Ft[2] = Wl[2] * nu[1]; Ft[3] = 0.0;
@@ -1414,6 +1464,10 @@ struct orc_solver {
         for (int v = 0; v < NC; v++) F[v] *= 0.5;
         const double a = m->face_area[f];
         const int32_t c0 = m->cof(f, 0), c1 = m->cof(f, 1);
+        if (canonical) {  // -a * F and a * F differ only in sign
+            for (int v = 0; v < NC; v++) ff[(size_t)f * NC + v] = a * F[v];
+            return;
+        }
         for (int v = 0; v < NC; v++) {
             add_rhs(r, c0, v, -a * F[v]);
             if (bc_type < 0 && c1 >= 0) add_rhs(r, c1, v, a * F[v]);
@@ -1422,7 +1476,9 @@ struct orc_solver {
 
     // solver_rhs.cpp:44-115; parts: bit 0 calc_rhs_interior (:74-108),
     // bit 1 calc_rhs_boundaries (:110-115)
+    const double* Ucur = nullptr;  // the state of the residual being evaluated
     void calc_rhs(const double* Ux, double* r, int parts = 3) {
+        Ucur = Ux;
         g_vacuum_guard = cfg.recon == ORC_RECON_TENO && cfg.teno_variant == ORC_TENO_CORRECTED;
         const size_t nc = m->n_cells;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
@@ -1437,6 +1493,19 @@ struct orc_solver {
             const auto& zf = m->zones[cfg.bc[b].zone].faces;
 #pragma omp parallel for num_threads(nthreads) schedule(static)
             for (int64_t i = 0; i < (int64_t)zf.size(); ++i) face_flux(zf[i], cfg.bc[b].type, &cfg.bc[b], r);
+        }
+        if (canonical) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (int64_t c = 0; c < (int64_t)nc; ++c)
+                for (uint32_t e = cf_off[c]; e < cf_off[c + 1]; ++e) {
+                    if (!(parts & (cf_bnd[e] ? 2 : 1))) continue;
+                    const int64_t sf = cf[e];
+                    const size_t f = (size_t)(sf < 0 ? -sf - 1 : sf);
+                    for (int v = 0; v < NC; ++v) {
+                        const double x = ff[f * NC + v];
+                        r[(size_t)c * NC + v] += sf < 0 ? -x : x;
+                    }
+                }
         }
 #pragma omp parallel for num_threads(nthreads) schedule(static)
         for (int64_t c = 0; c < (int64_t)nc; ++c)
@@ -1566,6 +1635,11 @@ orc_solver* orc_solver_create(orc_mesh* m, const orc_config* cfg, int n_threads)
     return s.release();
 }
 void orc_solver_destroy(orc_solver* s) { delete s; }
+void orc_set_canonical(orc_solver* s, int on) {
+    s->canonical = on != 0;
+    if (s->canonical) s->build_canonical_order();
+    else { std::vector<double>().swap(s->ff); s->cf.clear(); s->cf_off.clear(); s->cf_bnd.clear(); }
+}
 const char* orc_last_error(void) { return g_err.c_str(); }
 void orc_teno_sizes(const orc_solver* s, int64_t* o) {
     o[0] = s->n_dof; o[1] = s->M; o[2] = (int64_t)s->st.size(); o[3] = (int64_t)s->off_st.size();

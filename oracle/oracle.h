@@ -76,6 +76,10 @@ int64_t orc_mesh_zone(const orc_mesh* m, int zone, uint32_t* faces_out, char* na
 /* ---- solver ---- */
 orc_solver* orc_solver_create(orc_mesh* m, const orc_config* cfg, int n_threads);
 void orc_solver_destroy(orc_solver* s);
+/* on != 0: the residual's face contributions are gathered per cell in the
+   one-thread order instead of added with atomics, so every thread count gives
+   the one-thread result bit for bit (off by default). */
+void orc_set_canonical(orc_solver* s, int on);
 const char* orc_last_error(void);
 /* TENO sizes: out[0..4] = n_dof, max_cells_per_stencil, n stencils entries,
    n offsets_stencils, n reconstruction-matrix entries */

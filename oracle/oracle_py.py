@@ -60,6 +60,7 @@ def lib():
         L.orc_solver_create.restype = P
         L.orc_solver_create.argtypes = [P, C.POINTER(OrcConfig), C.c_int]
         L.orc_solver_destroy.argtypes = [P]
+        L.orc_set_canonical.argtypes = [P, C.c_int]
         L.orc_last_error.restype = C.c_char_p
         L.orc_teno_sizes.argtypes = [P, C.c_void_p]
         L.orc_teno_arrays.argtypes = [P] + [C.c_void_p] * 9
@@ -195,7 +196,10 @@ def make_config(recon="TENO", basis="legendre", order=2, factor=2.0, qcell=0, qf
 class Solver:
     """Oracle solver: one thread reproduces the canonical one-thread order."""
 
-    def __init__(self, mesh, cfg, n_threads=1):
+    def __init__(self, mesh, cfg, n_threads=1, canonical=False):
+        """canonical: gather each cell's face contributions in the one-thread
+        order (bit-identical to one thread at any thread count) instead of
+        adding them with atomics."""
         L = self._L = lib()
         self.mesh = mesh
         self.cfg = cfg
@@ -203,6 +207,8 @@ class Solver:
         if not self.h:
             raise RuntimeError(L.orc_last_error().decode())
         self.nc = mesh.n_cells
+        if canonical:
+            L.orc_set_canonical(self.h, 1)
 
     def __del__(self):
         if getattr(self, "h", None):
