@@ -17,9 +17,13 @@ number of steps replayed from the captured hipGraph is timed afterwards and
 reported as ms_per_step_graph.
 
 cpu_baseline: the CPU oracle (the restatement of the reference's OpenMP path:
-AoS state, per-face atomics) timed on this host on a bounded sample: the same
-numerics on a 512x512 (524,288-triangle) mesh for about --cpu-seconds
-(at most the workload's 100 steps).
+AoS state, per-face atomics) timed on this host's cores on bounded samples of
+the same numerics: a 1024x1024 (2,097,152-triangle) mesh, whose figure is
+`value`, and a 512x512 mesh beside it, each for about --cpu-seconds (at most
+the workload's 100 steps), with OMP_PROC_BIND=close OMP_PLACES=cores; and
+the serial canonical form (one core, one-thread face order) on a 256x256
+mesh. The report names the host CPU, nproc, the cores this process may use
+and the OpenMP threads used.
 
 Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 3]
 """
@@ -34,6 +38,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the OpenMP runtime reads its placement when the first library using it
+# loads, so it is set before anything is imported
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = "cell-updates/s per RK stage, learned indicator vs plain TENO at matched L1"
 QUADRANT = np.array([[1.5, 0.0, 0.0, 1.5],        # upper right
@@ -112,31 +120,66 @@ def check_state(dev, what):
     log(f"[bench] {what}: state physical, min rho {U[:, 0].min():.3g}, min p {W[:, 2].min():.3g}")
 
 
-def cpu_baseline(args):
-    """Oracle (OpenMP, atomics) on a bounded sample of the same workload."""
-    from oracle import oracle_py as orc
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    n = args.cpu_nx
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_sample(orc, args, n, threads, canonical, seconds):
+    """Steps of the oracle on an n x n sample of the workload: cell-updates/s
+    per RK stage over steps 2.. (the first step, with setup, is not timed)."""
     m = orc.Mesh("cartesian_tri", n, n, 1.0, 1.0, args.jitter, args.seed)
     bcs = [(z, args.bc, {}) for z in ("left", "right", "top", "bottom")]
-    s = orc.Solver(m, orc.make_config(order=args.order, teno_variant=args.variant, bcs=bcs), n_threads=threads)
+    s = orc.Solver(m, orc.make_config(order=args.order, teno_variant=args.variant, bcs=bcs), n_threads=threads,
+                   canonical=canonical)
     s.init_quadrant(0.5, 0.5, QUADRANT)
     t = time.perf_counter()
     s.run(1, 0.4)
     one = time.perf_counter() - t
     # at most the workload's 100 steps (1 + 99) from the initial state
-    k = min(99, max(2, int(args.cpu_seconds / max(one, 1e-3))))
+    k = min(99, max(2, int(seconds / max(one, 1e-3))))
     t = time.perf_counter()
     s.run(k, 0.4)
     el = time.perf_counter() - t
     U, _ = s.state()
     if not np.isfinite(U).all():
         raise RuntimeError("oracle sample produced non-finite state")
-    return {"value": m.n_cells * 3 * k / el, "unit": "cell-updates/s per RK stage", "cores": threads,
-            "kind": "port",
-            "sample": f"CPU oracle (OpenMP, per-face atomics) on {n}x{n} quads split into {m.n_cells} "
-                      f"jittered triangles, same numerics, {k} SSP-RK3 steps in {el:.2f} s"}
+    return m.n_cells, k, el, m.n_cells * 3 * k / el
+
+
+def cpu_baseline(args):
+    """Oracle (OpenMP, atomics) on bounded samples of the same workload, on
+    every core this process may use (the GPU's share of the host where a
+    scheduler grants one: OMP_NUM_THREADS caps it), plus one serial core."""
+    from oracle import oracle_py as orc
+    nproc = os.cpu_count() or 1
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    omp_cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(allowed, omp_cap) if omp_cap > 0 else allowed
+    sizes = []
+    for n in (args.cpu_nx, args.cpu_nx // 2):
+        cells, k, el, v = cpu_sample(orc, args, n, threads, False, args.cpu_seconds)
+        sizes.append({"mesh": f"{n}x{n}", "cells": cells, "steps": k, "seconds": round(el, 3), "value": v})
+        log(f"[bench] CPU oracle {n}x{n} on {threads} threads: {v / 1e6:.2f} M cell-updates/s ({k} steps, {el:.1f} s)")
+    n1 = max(16, args.cpu_nx // 4)
+    cells, k, el, v1 = cpu_sample(orc, args, n1, 1, True, args.cpu_seconds * 2.0 / 3.0)
+    log(f"[bench] CPU oracle {n1}x{n1} serial canonical: {v1 / 1e6:.3f} M cell-updates/s ({k} steps, {el:.1f} s)")
+    big = sizes[0]
+    per_cell_spread = max(x["value"] for x in sizes) / min(x["value"] for x in sizes)
+    return {"value": big["value"], "unit": "cell-updates/s per RK stage", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (OpenMP, per-face atomics, OMP_PROC_BIND=close OMP_PLACES=cores) on "
+                      f"{big['mesh']} quads split into {big['cells']} jittered triangles, same numerics, "
+                      f"{big['steps']} SSP-RK3 steps in {big['seconds']:.2f} s",
+            "cpu_model": cpu_model(), "nproc": nproc, "cores_allowed": allowed,
+            "omp_num_threads": omp_cap or None, "sizes": sizes, "size_spread": per_cell_spread,
+            "serial": {"value": v1, "cores": 1, "mesh": f"{n1}x{n1}", "cells": cells, "steps": k,
+                       "seconds": round(el, 3), "form": "one thread, canonical face order"}}
 
 
 def learned_run(args, mlr, dev, U_plain, value_plain, wall_plain):
@@ -365,7 +408,8 @@ def main():
     ap.add_argument("--integrator", default="SSPRK3", choices=["SSPRK3", "FE"],
                     help="FE adds one plain state copy per step (a known-size kernel for "
                          "calibrating HBM byte counters); the metric is defined for SSPRK3")
-    ap.add_argument("--cpu-nx", type=int, default=512)
+    ap.add_argument("--cpu-nx", type=int, default=1024,
+                    help="quads per side of the CPU baseline's main sample (also timed at half that)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-indicator run")
