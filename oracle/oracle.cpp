@@ -1605,6 +1605,45 @@ orc_mesh* orc_mesh_create(int type, uint32_t nx, uint32_t ny, double Lx, double 
     }
     return m.release();
 }
+// A caller's mesh in the reference's array layout (mesh.h:228-253; the
+// arrays Mesh::copy_host_to_device uploads), with the geometry computed as
+// for the generated meshes (mesh.cpp:167-261).
+orc_mesh* orc_mesh_create_arrays(uint32_t n_nodes, const double* node_coords, uint32_t n_cells,
+                                 const uint32_t* offsets_nodes_of_cell, const uint32_t* nodes_of_cell,
+                                 const uint32_t* faces_of_cell, uint32_t n_faces, const uint32_t* nodes_of_face,
+                                 const int32_t* cells_of_face, uint32_t n_zones, const char* const* zone_names,
+                                 const uint32_t* zone_offsets, const uint32_t* zone_faces) {
+    auto m = std::make_unique<orc_mesh>();
+    try {
+        if (n_zones < 1 || std::string(zone_names[0]) != "interior")
+            throw std::runtime_error("the first zone must be the interior");
+        m->type = 0;
+        m->n_nodes = n_nodes;
+        m->n_cells = n_cells;
+        m->n_faces = m->n_faces_real = n_faces;
+        m->node_coords.assign(node_coords, node_coords + 2 * (size_t)n_nodes);
+        std::vector<std::vector<uint32_t>> noc(n_cells), foc(n_cells), nof(n_faces);
+        for (uint32_t c = 0; c < n_cells; ++c)
+            for (uint32_t k = offsets_nodes_of_cell[c]; k < offsets_nodes_of_cell[c + 1]; ++k) {
+                noc[c].push_back(nodes_of_cell[k]);
+                foc[c].push_back(faces_of_cell[k]);
+            }
+        for (uint32_t f = 0; f < n_faces; ++f) nof[f] = {nodes_of_face[2 * (size_t)f], nodes_of_face[2 * (size_t)f + 1]};
+        m->build_csr(noc, foc, nof);
+        m->cells_of_face.assign(cells_of_face, cells_of_face + 2 * (size_t)n_faces);
+        for (uint32_t z = 0; z < n_zones; ++z)
+            m->zones.push_back({zone_names[z], z == 0,
+                                std::vector<uint32_t>(zone_faces + zone_offsets[z], zone_faces + zone_offsets[z + 1])});
+        m->face_areas();
+        m->cell_volumes();
+        m->cell_centroids();
+        m->face_normals_();
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+    return m.release();
+}
 void orc_mesh_destroy(orc_mesh* m) { delete m; }
 void orc_mesh_counts(const orc_mesh* m, int64_t* c) {
     c[0] = m->n_cells; c[1] = m->n_nodes; c[2] = m->n_faces; c[3] = m->n_faces_real;

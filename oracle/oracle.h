@@ -60,6 +60,13 @@ typedef struct {
 /* ---- mesh ---- */
 orc_mesh* orc_mesh_create(int type, uint32_t nx, uint32_t ny, double Lx, double Ly,
                           double jitter, uint64_t seed);
+/* A caller's mesh in the reference's array layout: 2 nodes per face,
+   faces_of_cell sharing offsets_nodes_of_cell, zone 0 named "interior". */
+orc_mesh* orc_mesh_create_arrays(uint32_t n_nodes, const double* node_coords, uint32_t n_cells,
+                                 const uint32_t* offsets_nodes_of_cell, const uint32_t* nodes_of_cell,
+                                 const uint32_t* faces_of_cell, uint32_t n_faces, const uint32_t* nodes_of_face,
+                                 const int32_t* cells_of_face, uint32_t n_zones, const char* const* zone_names,
+                                 const uint32_t* zone_offsets, const uint32_t* zone_faces);
 void orc_mesh_destroy(orc_mesh* m);
 /* counts[0..5] = n_cells, n_nodes, n_faces (allocated), n_faces_real,
    nodes_of_cell size, n_zones */

@@ -53,6 +53,10 @@ def lib():
         L.orc_mesh_create.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                       C.c_double, C.c_uint64]
         L.orc_mesh_destroy.argtypes = [P]
+        L.orc_mesh_create_arrays.restype = P
+        L.orc_mesh_create_arrays.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_mesh_counts.argtypes = [P, C.c_void_p]
         L.orc_mesh_arrays.argtypes = [P] + [C.c_void_p] * 12
         L.orc_mesh_zone.restype = C.c_int64
@@ -131,9 +135,23 @@ def _f64(x):
 class Mesh:
     """Reference-numbered mesh from the oracle's generators (mesh.cpp:305-848)."""
 
-    def __init__(self, kind, nx, ny, Lx=1.0, Ly=1.0, jitter=0.0, seed=0):
+    def __init__(self, kind, nx=0, ny=0, Lx=1.0, Ly=1.0, jitter=0.0, seed=0, arrays=None):
+        """A generated mesh (kind, nx, ny, ...) or, with kind "arrays", the
+        caller's arrays: the mesh dictionary mallard_amd.meshgen returns."""
         L = self._L = lib()
-        self.h = L.orc_mesh_create(MESH[kind], nx, ny, Lx, Ly, jitter, seed)
+        if kind == "arrays":
+            a = {k: np.ascontiguousarray(arrays[k], dt) for k, dt in (
+                ("node_coords", np.float64), ("offsets_nodes_of_cell", np.uint32), ("nodes_of_cell", np.uint32),
+                ("faces_of_cell", np.uint32), ("nodes_of_face", np.uint32), ("cells_of_face", np.int32),
+                ("zone_offsets", np.uint32), ("zone_faces", np.uint32))}
+            names = (C.c_char_p * len(arrays["zone_names"]))(*[z.encode() for z in arrays["zone_names"]])
+            self.h = L.orc_mesh_create_arrays(
+                a["node_coords"].size // 2, _p(a["node_coords"]), len(a["offsets_nodes_of_cell"]) - 1,
+                _p(a["offsets_nodes_of_cell"]), _p(a["nodes_of_cell"]), _p(a["faces_of_cell"]),
+                a["nodes_of_face"].size // 2, _p(a["nodes_of_face"]), _p(a["cells_of_face"]),
+                len(arrays["zone_names"]), names, _p(a["zone_offsets"]), _p(a["zone_faces"]))
+        else:
+            self.h = L.orc_mesh_create(MESH[kind], nx, ny, Lx, Ly, jitter, seed)
         if not self.h:
             raise RuntimeError(L.orc_last_error().decode())
         c = np.zeros(6, np.int64)
@@ -174,7 +192,8 @@ class Mesh:
 def make_config(recon="TENO", basis="legendre", order=2, factor=2.0, qcell=0, qface=0,
                 riemann="HLLC", integrator="SSPRK3", gamma=1.4, p_ref=101325.0, T_ref=298.15,
                 rho_ref=1.225, p_min=-1e20, p_max=1e20, bcs=None,
-                teno_variant="reference"):
+                teno_variant="reference", zones=None):
+    """zones: the mesh's zone names in order (generated meshes: ZONES)."""
     cfg = OrcConfig()
     cfg.recon, cfg.basis, cfg.order = RECON[recon], BASIS[basis], order
     cfg.stencil_factor, cfg.quad_order_cell, cfg.quad_order_face = factor, qcell, qface
@@ -186,7 +205,7 @@ def make_config(recon="TENO", basis="legendre", order=2, factor=2.0, qcell=0, qf
     cfg.n_bc = len(bcs)
     for i, (zone, kind, par) in enumerate(bcs):
         b = cfg.bc[i]
-        b.zone, b.type = ZONES.index(zone), BC[kind]
+        b.zone, b.type = (zones or ZONES).index(zone), BC[kind]
         u = par.get("u", (0.0, 0.0))
         b.u[0], b.u[1] = u
         b.p, b.T = par.get("p", 0.0), par.get("T", 0.0)
