@@ -18,16 +18,19 @@
  * pinned by the reference's own unit-test values. Mesh generation, stencils,
  * pseudo-inverses, OI, the TENO kernel, boundary fluxes, dt and whole
  * trajectories have no reference-held outputs; they are pinned by closed
- * forms and invariants (counts, face-index formulas, volume sums, closed
- * normals, free-stream preservation, A+ A = I on the fitted subspace, SVD
- * pseudo-inverse agreement, linear-field offsets, conservation, SSPRK3
- * amplification). The Roe flux (no reference counterpart) is pinned only by
- * textbook properties: parity unpinned against the reference. Also parity
- * unpinned beyond those invariants: the order of equal-distance ring members
- * (libstdc++ std::sort ties, face_reconstruction.cpp:253-257) and the
- * reference weighting's raw weights and +ipt/ta0 face offset
- * (face_reconstruction.cpp:948-1031), whose only checks are the offset being
- * a cell constant and the weighting's known divergence.
+ * forms, textbook formulas and brute-force restatements: counts, face-index
+ * formulas, volume sums, closed normals, free-stream preservation, A+ A = I
+ * on the fitted subspace, SVD pseudo-inverse agreement, linear-field offsets,
+ * conservation, SSPRK3 amplification, per-face spectral radii and dt on
+ * unjittered meshes, the ENO branch on a straight jump (both weightings,
+ * raw weights and the +ipt/ta0 offset included), the directional stencils
+ * against a brute-force Type-4 search, wall / fixed-state / p_out residuals,
+ * and Toro's HLL flux; each pin fails on planted mistakes
+ * (tests/test_oracle_mutations.py). The Roe flux (no reference counterpart)
+ * is pinned only by textbook properties: parity unpinned against the
+ * reference. Also parity unpinned beyond the invariants above: the order of
+ * equal-distance ring members (libstdc++ std::sort ties,
+ * face_reconstruction.cpp:253-257).
  */
 #include "oracle.h"
 
