@@ -38,6 +38,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the cores this process may use, read before the OpenMP runtime binds the
+# main thread to its first place
+ALLOWED_CORES = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
 # the OpenMP runtime reads its placement when the first library using it
 # loads, so it is set before anything is imported
 os.environ.setdefault("OMP_PROC_BIND", "close")
@@ -159,7 +162,7 @@ def cpu_baseline(args):
     scheduler grants one: OMP_NUM_THREADS caps it), plus one serial core."""
     from oracle import oracle_py as orc
     nproc = os.cpu_count() or 1
-    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    allowed = ALLOWED_CORES
     omp_cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
     threads = min(allowed, omp_cap) if omp_cap > 0 else allowed
     sizes = []
