@@ -14,7 +14,11 @@ The timed region launches every kernel on the solver's stream with a HIP
 event after each launch (mlr_run_steps_profiled), so the same run yields the
 whole-step throughput and the average duration of each kernel. The same
 number of steps replayed from the captured hipGraph is timed afterwards and
-reported as ms_per_step_graph.
+reported as ms_per_step_graph. The plain and learned runs use the exact
+TENO tables (bit-identical to the oracle). Afterwards the same steps are
+timed on compact tables (plain_compact_tables: member areas folded into the
+pseudo-inverses, face basis per reference-triangle edge), with their largest
+relative difference from the exact tables' run (--exact-tables skips it).
 
 cpu_baseline: the CPU oracle (the restatement of the reference's OpenMP path:
 AoS state, per-face atomics) timed on this host's cores on bounded samples of
@@ -65,7 +69,7 @@ def numerics_kw(args):
                 teno_variant=args.variant, riemann="HLLC", integrator=args.integrator, cfl=0.4, bcs=bcs)
 
 
-def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes):
+def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes, compact=False):
     """Algorithmic HBM bytes of one TENO face-state launch.
 
     alg: each compulsory element counted once, per non-empty stencil the
@@ -75,11 +79,13 @@ def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes):
     (K-1) x (M-1) non-zero block of the pseudo-inverse, M-1 areas and in-tile
     positions, the tile lists (4 B per entry), the cell state, its info word,
     the face-point basis values, the face states written (S*Q points x 4
-    variables) and the decision byte.
+    variables) and the decision byte. Compact tables read neither the areas
+    (folded into the pseudo-inverse) nor the per-cell face basis.
     """
     alg = n_stencils * (8 * K * M + 12 * M) + 32 * n_cells
-    layout = (n_stencils * (8 * (K - 1) * (M - 1) + (8 + pos_bytes) * (M - 1)) + 4 * tile_entries +
-              n_cells * (32 + 4 + 8 * S * Q * (K - 1) + 8 * S * Q * 4 + 1))
+    area_b, basis_b = (0, 0) if compact else (8, 8 * S * Q * (K - 1))
+    layout = (n_stencils * (8 * (K - 1) * (M - 1) + (area_b + pos_bytes) * (M - 1)) + 4 * tile_entries +
+              n_cells * (32 + 4 + basis_b + 8 * S * Q * 4 + 1))
     return alg, layout
 
 
@@ -416,6 +422,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-indicator run")
+    ap.add_argument("--exact-tables", action="store_true",
+                    help="skip the extra timing on compact TENO tables")
     ap.add_argument("--train-cases", type=int, default=512)
     ap.add_argument("--threshold", type=float, default=0.9)
     ap.add_argument("--gate-split", type=int, default=1, choices=[0, 1],
@@ -465,22 +473,24 @@ def main():
         dev.run_steps(args.warmup)
     log(f"[bench] warmup {args.warmup} steps done; timing {args.steps} steps")
 
-    # every timed block is the workload itself: steps from the initial state
-    dev.init_quadrant(0.5, 0.5, QUADRANT)
-    # timed region: kernel-by-kernel launches with HIP events on the solver's stream
-    t = time.perf_counter()
-    st = dev.run_steps_profiled(args.steps)   # returns after the stream has drained
-    wall = time.perf_counter() - t
-    log(f"[bench] timed: {wall:.3f} s wall, {st['total_ms']:.1f} ms by events")
-    check_state(dev, "timed steps")
+    def plain_run(what):
+        # every timed block is the workload itself: steps from the initial state
+        dev.init_quadrant(0.5, 0.5, QUADRANT)
+        # timed region: kernel-by-kernel launches with HIP events on the solver's stream
+        t = time.perf_counter()
+        st = dev.run_steps_profiled(args.steps)   # returns after the stream has drained
+        wall = time.perf_counter() - t
+        log(f"[bench] timed ({what}): {wall:.3f} s wall, {st['total_ms']:.1f} ms by events")
+        check_state(dev, "timed steps")
+        dev.init_quadrant(0.5, 0.5, QUADRANT)
+        t = time.perf_counter()
+        dev.run_steps(args.steps)
+        wall_graph = time.perf_counter() - t
+        check_state(dev, "graph-replayed steps")
+        return st, wall, wall_graph, dev.state()[0]
 
-    dev.init_quadrant(0.5, 0.5, QUADRANT)
-    t = time.perf_counter()
-    dev.run_steps(args.steps)
-    wall_graph = time.perf_counter() - t
-    check_state(dev, "graph-replayed steps")
-
-    U_plain = dev.state()[0]
+    st, wall, wall_graph, U_plain = plain_run("exact tables")
+    stages = {"SSPRK3": 3, "FE": 1}[args.integrator]
     teno_ms = st["teno_ms"] / max(st["teno_n"], 1)
     flux_ms = st["flux_ms"] / max(st["flux_n"], 1)
     alg, lay = teno_bytes(N, dev.n_stencils, dev.K, dev.M, dev.S, dev.Q, dev.tile_entries, dev.tile_pos_bytes)
@@ -488,24 +498,50 @@ def main():
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # summary, used only when it was measured on this workload and on the
     # kernel source being run
-    traffic = None
+    traffic = traffic_compact = None
     if os.path.exists(TRAFFIC_FILE):
         tr = json.load(open(TRAFFIC_FILE))
         src = open(os.path.join(ROOT, "mallard_amd", "csrc", "kernels.hip"), "rb").read()
         if (tr.get("n_cells") == N and tr.get("order") == args.order and
                 tr.get("kernels_sha256") == hashlib.sha256(src).hexdigest()[:16]):
             traffic = tr["bytes_per_launch"]
+            traffic_compact = tr.get("compact_bytes_per_launch")
 
     # what HBM delivers to plain 16-byte streaming on this GPU (2 GiB buffers)
     nb = 2 << 30
     stream_copy = 2 * nb / (mlr.probe_stream(0, nb, 10) * 1e-3) / 1e9
     stream_read = nb / (mlr.probe_stream(1, nb, 10) * 1e-3) / 1e9
 
-    stages = {"SSPRK3": 3, "FE": 1}[args.integrator]
     value_plain = N * stages * args.steps / wall
     learned = None
     if not args.no_learned and args.integrator == "SSPRK3" and args.order == 2:
         learned = learned_run(args, mlr, dev, U_plain, value_plain, wall)
+
+    compact = None
+    if not args.exact_tables and args.variant == "corrected" and args.order == 2:
+        # the same steps on compact tables: member areas folded into A+ and the
+        # face basis per reference-triangle edge (same quantities in exact
+        # arithmetic, 472 fewer bytes per cell in the TENO pass, rounding
+        # that the TENO cut-off can amplify); the tables change for good
+        dev.set_indicator(None)
+        dev.compact_tables()
+        dev.autotune()
+        if args.warmup > 0:
+            dev.run_steps(args.warmup)
+        stc, wallc, wall_graphc, Uc = plain_run("compact tables")
+        sc = np.maximum(np.abs(U_plain).max(axis=0), 1e-300)
+        dev_c = float((np.abs(Uc - U_plain).max(axis=0) / sc).max())
+        teno_c = stc["teno_ms"] / max(stc["teno_n"], 1)
+        alg_c, lay_c = teno_bytes(N, dev.n_stencils, dev.K, dev.M, dev.S, dev.Q, dev.tile_entries,
+                                  dev.tile_pos_bytes, compact=True)
+        ach_c = alg_c / (teno_c * 1e-3) / 1e9
+        compact = {"value": N * stages * args.steps / wallc, "ms_per_step": wallc / args.steps * 1e3,
+                   "ms_per_step_graph": wall_graphc / args.steps * 1e3, "teno_ms": teno_c,
+                   "rel_linf_vs_exact_tables": dev_c,
+                   "roofline": {"achieved": ach_c, "frac": ach_c / HBM_PEAK_GBS, "traffic": traffic_compact,
+                                "layout_bytes_per_launch": lay_c}}
+        log(f"[bench] compact tables: {compact['ms_per_step']:.2f} ms per step (TENO {teno_c:.3f} ms), relative "
+            f"L-inf against the exact tables after {args.steps} steps {dev_c:.3g}")
 
     matched = learned is not None and learned["matched_l1"]
     value = learned["value"] if matched else value_plain
@@ -532,7 +568,9 @@ def main():
         },
         "plain": {"value": value_plain, "ms_per_step": wall / args.steps * 1e3,
                   "ms_per_step_graph": wall_graph / args.steps * 1e3,
+                  "tables": "exact (bit-identical to the oracle)",
                   "whole_step": step_roofline(alg, N, stages * args.steps, wall)},
+        "plain_compact_tables": compact,
         "learned": learned,
         "roofline": {
             "kernel": "teno_kernel (plain FP64 TENO run)",

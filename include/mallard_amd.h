@@ -153,6 +153,19 @@ int mlr_run_steps(mlr_ctx* ctx, int64_t n, double* dts);
  *              TENO face-state ms, launches,  flux+stage-update ms, launches,
  *              spectral-radius+dt ms, launches, copy ms, launches }. */
 int mlr_run_steps_profiled(mlr_ctx* ctx, int64_t n, double* stats);
+/* Compact TENO tables (corrected weighting, p = 2 with the default stencil
+ * size; no reference counterpart). The member areas are folded into the
+ * pseudo-inverses, a = (A+ diag(ta)) (u_i - u_c), where the reference forms
+ * b_i = ta_i (u_i - u_c) first (face_reconstruction.cpp:900-925), and the
+ * face-point basis values are those of the reference triangle's edges
+ * instead of each cell's face points mapped through its inverse Jacobian
+ * (:985-1031). Both are the same quantities in exact arithmetic; the TENO
+ * pass reads 472 fewer bytes per cell. Results differ from the exact tables'
+ * by rounding in one residual, which the TENO cut-off can amplify over many
+ * steps (4.4e-8 relative after 100 steps of the 8.4M-cell four-quadrant
+ * problem), so this is an optional fast path, not the parity path. One way:
+ * the exact tables are released. */
+int mlr_compact_tables(mlr_ctx* ctx);
 /* Launch shapes; no reference counterpart (the reference leaves them to
  * Kokkos' defaults). tune[4] = { TENO occupancy target in waves per SIMD
  * (0: the compiler's choice, 5, 6; used at p = 2 with the default stencil

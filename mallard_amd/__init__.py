@@ -139,6 +139,7 @@ _SIGS = {
     "mlr_host_pow": (C.c_int, [_P, _P, _P, C.c_int]),
     "mlr_host_pow6": (C.c_int, [_P, _P, C.c_int]),
     "mlr_calc_face_values": (C.c_int, [_P, _P, _P]),
+    "mlr_compact_tables": (C.c_int, [_P]),
     "mlr_calc_rhs_of": (C.c_int, [_P, _P, C.c_int, _P]),
     "mlr_calc_rhs_device": (C.c_int, [_P, _P, C.c_int, _P]),
     "mlr_riemann_flux": (C.c_int, [C.c_int, C.c_int, C.c_int64, _P, _P, _P, _P]),
@@ -336,9 +337,15 @@ class Solver:
         s = np.zeros(15, np.int64)
         L.mlr_get_sizes(self.h, _p(s))
         self.n_cells, self.n_faces, self.n_faces_real, self.S, self.Q, self.K, self.M, self.G = map(int, s[:8])
-        self.device_bytes = int(s[8])
         self.n_stencils = int(s[12])
         self.tile_entries, self.tile_pos_bytes = int(s[13]), int(s[14])
+
+    @property
+    def device_bytes(self):
+        """Device memory the context holds now."""
+        s = np.zeros(15, np.int64)
+        lib().mlr_get_sizes(self.h, _p(s))
+        return int(s[8])
 
     def close(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -433,6 +440,12 @@ class Solver:
         out = np.zeros((self.n_faces, self.Q, 2, 4))
         self._check(lib().mlr_get_face_values(self.h, _p(out)))
         return out
+
+    def compact_tables(self):
+        """Fold the member areas into A+ and take the face-point basis values
+        from the reference triangle's edges (same values in exact arithmetic,
+        fewer bytes per TENO pass); one way."""
+        self._check(lib().mlr_compact_tables(self.h))
 
     def calc_face_values(self, U):
         """FaceReconstruction::calc_face_values of a caller's state U[N][4]

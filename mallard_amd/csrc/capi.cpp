@@ -477,6 +477,11 @@ int mlr_integrator_take_step(mlr_ctx* ctx, int type, double dt, int64_t n, doubl
     });
 }
 
+int mlr_compact_tables(mlr_ctx* ctx) {
+    if (!ctx) return 1;
+    return guarded(ctx, [&] { ctx->solver.compact_tables(); });
+}
+
 int mlr_get_decisions(mlr_ctx* ctx, uint8_t* out) {
     if (!ctx || !out) return 1;
     return guarded(ctx, [&] { ctx->solver.get_decisions(out); });

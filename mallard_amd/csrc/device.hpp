@@ -85,6 +85,13 @@ struct Teno {
     // the ungated cells); one byte per wave slot (block * 4 + wave), 1 when
     // the wave holds a valid cell the network did not gate
     uint8_t* blk_mixed;
+    // compact tables (corrected weighting, one face point per face): A holds
+    // A+ times the member areas (ta unused) and the face-point basis values
+    // come from fbc[e][K-1], one row per reference-triangle edge e (the
+    // midpoint of nodes 0-1, 1-2 or 2-0), e of slot l in bits 24 + 2 l of the
+    // slot-0 info word
+    int32_t compact;
+    const double* fbc;
 };
 
 // Stage programs of the fused residual + SSP-RK update.
@@ -232,6 +239,9 @@ hipError_t teno_linear_operator(const Teno& t, const uint32_t* info, uint32_t N,
 // A: [N/64][4][M-1][K-1][64], ta: [N/64][4][M-1][64]; err |= 1 singular
 // frame, |= 2 a stencil needing the full-matrix branch
 hipError_t teno_aplus(const PreInput& in, int K, int M, double* A, double* ta, int* err, hipStream_t stream);
+// A[row][k] *= ta[row] over the blocked tables (rows = n_blocks * G * (M-1)):
+// the member areas folded into the pseudo-inverses (compact tables)
+hipError_t teno_fold_areas(double* A, const double* ta, uint64_t rows, int KP, hipStream_t stream);
 
 // Learned indicator (indicator.hip). The network lives in one device buffer of
 // net_bytes() that net_pack fills from the FP32 master parameters.

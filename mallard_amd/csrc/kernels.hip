@@ -115,7 +115,10 @@ __device__ inline void quad_gather(double f, double* out, int q) {
 // registers): 3 evaluates like 1 and 4 reuses like 2, but both reconstruct
 // only the gated cells and mark the waves holding ungated ones; 5 then
 // reconstructs only the ungated cells, in the marked blocks.
-template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM>
+// CT: compact tables (corrected weighting): A carries the member areas and
+// the face-point basis values come from the small per-edge table t.fbc, so
+// neither ta nor the per-cell face basis is read.
+template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, bool CT = false>
 __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const double* __restrict__ U,
                                            double* __restrict__ fs, uint8_t* __restrict__ dec, const Physics& ph,
                                            uint32_t lb, double2* sh_u, unsigned char* sh_net) {
@@ -221,12 +224,17 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
     auto member = [&](size_t row0, int i, double (&a)[KP][4]) {
         const size_t o = (row0 + i) * 64 + cl;
         const uint32_t pos = loc[o];
-        const double tt = t.ta[o];
         const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
         const D4 un = {{u01.x, u01.y, u23.x, u23.y}};
         double b[4];
+        if constexpr (CT) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) b[v] = tt * (un.v[v] - uc.v[v]);
+            for (int v = 0; v < 4; ++v) b[v] = un.v[v] - uc.v[v];
+        } else {
+            const double tt = t.ta[o];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b[v] = tt * (un.v[v] - uc.v[v]);
+        }
         const double* Ai = t.A + (row0 + i) * (KP * 64) + cl;
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
@@ -358,12 +366,26 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
             for (int k = 0; k < KP; ++k) pw[st][k] = ws[st] * at[st][k];
 
         // face points (face_reconstruction.cpp:985-1031)
+        if constexpr (CT) {
+            // keep the small table's loads here: hoisted to the kernel's
+            // start (their addresses are known from info0) they would hold
+            // 30 registers through the stencil loop
+            __asm__ volatile("" ::: "memory");
+        }
 #pragma unroll
         for (int fq = 0; fq < FQ; ++fq) {
-            const double* B = t.fbasis + (((size_t)lb * FQ + fq) * KP) * 64 + (c & 63u);
             double bk[KP];
+            if constexpr (CT) {
+                static_assert(FQ == 3, "compact tables: one face point per face");
+                // the face's reference edge, from the slot-0 info word
+                const double* B = t.fbc + ((info0 >> (24 + 2 * fq)) & 3u) * KP;
 #pragma unroll
-            for (int k = 0; k < KP; ++k) bk[k] = B[k * 64];
+                for (int k = 0; k < KP; ++k) bk[k] = B[k];
+            } else {
+                const double* B = t.fbasis + (((size_t)lb * FQ + fq) * KP) * 64 + (c & 63u);
+#pragma unroll
+                for (int k = 0; k < KP; ++k) bk[k] = B[k * 64];
+            }
             double acc = ucv;
 #pragma unroll
             for (int st = 0; st < G; ++st) {
@@ -443,7 +465,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
 
 // OCC: waves per SIMD the register allocation targets (0: the compiler's
 // own choice); an autotuner picks it per mesh, results are identical
-template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, int OCC>
+template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, int OCC, bool CT = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC == 0 ? 1 : OCC, OCC == 0 ? 8 : OCC)))
 void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
                                                    double* __restrict__ fs, uint8_t* __restrict__ dec,
@@ -451,8 +473,8 @@ void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
     // dynamic LDS: the tile's states (TILE cells), then the network when gated
     extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
-    teno_block<KP, MP, FQ, IdxT, TILE, GM>(g, t, U, fs, dec, ph, lb, reinterpret_cast<double2*>(teno_smem),
-                                           teno_smem + TILE * 32);
+    teno_block<KP, MP, FQ, IdxT, TILE, GM, CT>(g, t, U, fs, dec, ph, lb, reinterpret_cast<double2*>(teno_smem),
+                                               teno_smem + TILE * 32);
 }
 
 // case of global block gb: the last case whose first block is <= gb
@@ -1051,6 +1073,38 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
         else MLR_TENO_G(KP, MP, FQV, 1);                   \
     } while (0)
     if (t.gate && Mp < ind::kMembers - 1) return hipErrorInvalidValue;
+    if (t.compact) {
+        // compact tables: p = 2 with the default stencil size, 8-bit tile
+        // positions, the compiler's occupancy
+        if (!(FQ == 3 && Kp == 5 && Mp == 11 && small) || t.fbc == nullptr) return hipErrorInvalidValue;
+        const size_t lds = 256 * 32;
+        if (t.gate && t.blk_mixed) {
+            // split gated pass: the gated cells (linear operators, no tables),
+            // then the ungated cells of the marked blocks on compact tables
+            if (gate_reuse) MLR_TENO_S(5, 11, 3, 4, 0);
+            else MLR_TENO_S(5, 11, 3, 3, 0);
+            if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+            hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 5, 0, true>), grid, block, lds, stream, g, t, U,
+                               fs, dec, ph, per_xcd);
+        } else if (!t.gate) {
+            if (occ == 5)
+                hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 0, 5, true>), grid, block, lds, stream, g, t,
+                                   U, fs, dec, ph, per_xcd);
+            else if (occ == 6)
+                hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 0, 6, true>), grid, block, lds, stream, g, t,
+                                   U, fs, dec, ph, per_xcd);
+            else
+                hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 0, 0, true>), grid, block, lds, stream, g, t,
+                                   U, fs, dec, ph, per_xcd);
+        } else if (gate_reuse) {
+            hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 2, 0, true>), grid, block, lds, stream, g, t, U,
+                               fs, dec, ph, per_xcd);
+        } else {
+            hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 1, 0, true>), grid, block, lds + net_lds, stream,
+                               g, t, U, fs, dec, ph, per_xcd);
+        }
+        return hipGetLastError();
+    }
     // the default stencil size (factor 2: M = 2K) is compiled with the member
     // loop unrolled, and at p = 2 in three occupancy targets; other sizes
     // take the runtime loop

@@ -639,6 +639,7 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
     L.G = (int)S + 1;
     L.Q = (uint32_t)ctx.qf.n;
     L.qface_w = ctx.qf.w;
+    L.qface_pts = ctx.qf.pts;
     L.poly = ctx.poly;
     L.ipt = ctx.ipt;
     L.OI = ctx.oscillation_indicator();
@@ -715,6 +716,23 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
             for (uint32_t l = 0; l < S; ++l) {
                 const uint32_t f = m.face_of_cell(c, l);
                 const uint32_t a = m.node_of_face(f, 0), b = m.node_of_face(f, 1);
+                // the face's edge of the reference triangle: the target frame
+                // maps the cell's nodes 0, 1, 2 to (0, 0), (1, 0), (0, 1), so
+                // the face's midpoint there is the midpoint of edge e = 0
+                // (nodes 0-1), 1 (1-2) or 2 (2-0); slot 0's info word holds
+                // e of slot l in bits 24 + 2 l
+                {
+                    int ia = -1, ib = -1;
+                    for (int r = 0; r < 3; ++r) {
+                        if (m.node_of_cell(c, r) == a) ia = r;
+                        if (m.node_of_cell(c, r) == b) ib = r;
+                    }
+                    if (ia >= 0 && ib >= 0 && ia != ib && l < 3) {
+                        const int lo = ia < ib ? ia : ib, hi = ia < ib ? ib : ia;
+                        const uint32_t e = (lo == 0 && hi == 1) ? 0u : (lo == 1 && hi == 2) ? 1u : 2u;
+                        L.info[j] |= e << (24 + 2 * l);
+                    }
+                }
                 double x0[2], x1[2];
                 for (int i = 0; i < 2; i++) { x0[i] = m.nc(a, i) - w0[i]; x1[i] = m.nc(b, i) - w0[i]; }
                 gemv2(Ji, x0, x0);

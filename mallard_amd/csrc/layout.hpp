@@ -47,6 +47,7 @@ struct Layout {
     uint32_t S = 0;        // face slots per cell (3 triangles, 4 quads)
     uint32_t Q = 1;        // face quadrature points
     std::vector<double> qface_w;  // face quadrature weights
+    std::vector<double> qface_pts;  // face quadrature points on [-1, 1]
     std::vector<uint32_t> perm;   // new -> old
     std::vector<uint32_t> iperm;  // old -> new
     std::vector<uint32_t> face_slot;  // old face id -> (new cell << 2 | slot) of side 0, or ~0u

@@ -223,18 +223,37 @@ __global__ __launch_bounds__(256) void linear_operator_kernel(Teno t, const uint
     const uint32_t c = (uint32_t)(blk * 64 + cl);
     const int KP = t.K - 1, Mp = t.M - 1;
     const bool on = c < N && ((info[c] >> 20) & 1u);
+    // face-point basis values: the cell's table, or with compact tables the
+    // row of the face's reference edge (areas folded into A)
     const double* B = t.fbasis + ((blk * FQ + fq) * KP) * 64 + cl;
+    size_t bs = 64;
+    if (t.compact) {  // one face point per face: fq is the slot
+        const uint32_t e = c < N ? (info[c] >> (24 + 2 * fq)) & 3u : 0u;
+        B = t.fbc + (size_t)e * KP;
+        bs = 1;
+    }
     for (int i = 0; i < Mp; ++i) {
         double w = 0.0;
         if (on) {
             const size_t row = blk * 4 * Mp + i;  // central stencil, member i
             const double* Ai = t.A + row * (KP * 64) + cl;
             double acc = 0.0;
-            for (int k = 0; k < KP; ++k) acc = acc + B[k * 64] * Ai[k * 64];
-            w = acc * t.ta[row * 64 + cl];
+            for (int k = 0; k < KP; ++k) acc = acc + B[k * bs] * Ai[k * 64];
+            w = t.compact ? acc : acc * t.ta[row * 64 + cl];
         }
         lin[((blk * FQ + fq) * Mp + i) * 64 + cl] = w;
     }
+}
+
+// one thread per (table row, cell of the block)
+__global__ __launch_bounds__(256) void fold_areas_kernel(double* __restrict__ A, const double* __restrict__ ta,
+                                                         uint64_t total, int KP) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const uint64_t row = gid >> 6, cl = gid & 63u;
+    const double t = ta[gid];
+    double* a = A + row * (uint64_t)KP * 64 + cl;
+    for (int k = 0; k < KP; ++k) a[k * 64] = a[k * 64] * t;
 }
 
 }  // namespace
@@ -245,6 +264,15 @@ hipError_t teno_linear_operator(const Teno& t, const uint32_t* info, uint32_t N,
     const uint64_t total = (uint64_t)((N + 63) / 64) * 64 * FQ;
     hipLaunchKernelGGL(linear_operator_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, t, info,
                        N, FQ, total, lin);
+    return hipGetLastError();
+}
+
+hipError_t teno_fold_areas(double* A, const double* ta, uint64_t rows, int KP, hipStream_t stream) {
+    if (KP < 1) return hipErrorInvalidValue;
+    const uint64_t total = rows * 64;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(fold_areas_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, A, ta, total,
+                       KP);
     return hipGetLastError();
 }
 

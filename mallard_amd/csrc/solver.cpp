@@ -493,6 +493,8 @@ dev::Teno Solver::teno() const {
     t.gate_dec = gate_dec_;
     t.lin = lin_;
     t.blk_mixed = (gate_ && gate_split_) ? blk_mixed_ : nullptr;
+    t.compact = compact_ ? 1 : 0;
+    t.fbc = fbc_;
     t.gate = gate_ ? 1 : 0;
     if (gate_) t.net = dev::net_view(net_buf_, gate_thr_, nullptr);
     return t;
@@ -501,6 +503,62 @@ dev::Teno Solver::teno() const {
 void Solver::drop_graph() {
     if (step_exec_) { HIP_OK(hipGraphExecDestroy(step_exec_)); step_exec_ = nullptr; }
     if (step_graph_) { HIP_OK(hipGraphDestroy(step_graph_)); step_graph_ = nullptr; }
+}
+
+// The member areas folded into the pseudo-inverses (a = (A+ diag(ta)) (u_i -
+// u_c) instead of A+ (ta (u_i - u_c))) and the face-point basis values taken
+// from the reference triangle's edges instead of mapping each cell's face
+// points through its inverse Jacobian: the same quantities in exact
+// arithmetic, 472 fewer bytes per cell (the 11 x 4 member areas and the 3 x 5
+// face-basis values) for the TENO pass to read, different rounding.
+void Solver::compact_tables() {
+    if (compact_) return;
+    if (!L_.teno || L_.tspec.variant != TenoVariant::Corrected)
+        fail("compact TENO tables: TENO with the corrected weighting only");
+    if (!(L_.K == 6 && L_.M == 12 && L_.S == 3 && L_.Q == 1 && loc8_))
+        fail("compact TENO tables: p = 2 with the default stencil size only");
+    const int Kp = L_.K - 1, Q = (int)L_.Q;
+    const uint64_t rows = (uint64_t)((L_.N + 63) / 64) * L_.G * (L_.M - 1);
+    HIP_OK(dev::teno_fold_areas(A_, ta_, rows, Kp, stream_));
+    // basis values at the face point (the midpoint) of each edge of the
+    // reference triangle: e = 0 nodes 0-1, 1 nodes 1-2, 2 nodes 2-0
+    std::vector<double> tab((size_t)3 * Kp, 0.0);
+    const double R[3][2] = {{0.0, 0.0}, {1.0, 0.0}, {0.0, 1.0}};
+    for (int e = 0; e < 3; ++e) {
+        const int ia = e, ib = (e + 1) % 3;
+        double xq[2];
+        for (int i = 0; i < 2; i++) xq[i] = (L_.qface_pts[0] + 1.0) * 0.5 * (R[ib][i] - R[ia][i]) + R[ia][i];
+        for (int k = 1; k < L_.K; ++k)
+            tab[(size_t)e * Kp + (k - 1)] =
+                basis_2d(L_.tspec.basis, L_.poly[2 * k], L_.poly[2 * k + 1], xq[0], xq[1]) - L_.ipt[k];
+    }
+    void* p = nullptr;
+    HIP_OK(hipMalloc(&p, tab.size() * sizeof(double)));
+    allocs_.push_back(p);
+    dev_bytes_ += tab.size() * sizeof(double);
+    fbc_ = static_cast<double*>(p);
+    HIP_OK(hipMemcpy(fbc_, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    synchronize();
+    // the member areas and per-cell face basis are not read any more
+    auto release = [&](double*& buf, uint64_t bytes) {
+        for (void*& a : allocs_)
+            if (a == buf) {
+                HIP_OK(hipFree(a));
+                a = nullptr;
+            }
+        dev_bytes_ -= bytes;
+        buf = nullptr;
+    };
+    const size_t Np = L_.n_pad();
+    release(ta_, (uint64_t)L_.G * (L_.M - 1) * Np * sizeof(double));
+    release(fbasis_, (uint64_t)L_.S * Q * Kp * Np * sizeof(double));
+    compact_ = true;
+    drop_graph();
+    tuned_ = false;
+    if (lin_) {
+        HIP_OK(dev::teno_linear_operator(teno(), info_, L_.N, (int)(L_.S * L_.Q), lin_, stream_));
+        synchronize();
+    }
 }
 
 void Solver::set_indicator(const float* params, double threshold) {
@@ -764,6 +822,7 @@ void Solver::device_stencils(uint32_t cell, uint32_t* out) const {
 
 void Solver::device_matrices(uint32_t cell, double* aplus, double* ta) const {
     if (!L_.teno) fail("matrices exist only for TENO");
+    if (compact_) fail("the compact tables hold A+ times the member areas; the exact tables were released");
     if (cell >= L_.N) fail("cell out of range");
     const uint32_t j = L_.iperm[cell];
     const int K = L_.K, M = L_.M, G = L_.G;

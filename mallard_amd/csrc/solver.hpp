@@ -174,6 +174,12 @@ public:
     int tile_pos_bytes() const { return loc8_ ? 1 : 2; }
     double* d_fs() { return fs_; }
 
+    // compact TENO tables (corrected weighting, p = 2): the member areas
+    // folded into A+ and the face-point basis values from the reference
+    // triangle's edges; one way (the exact tables are released)
+    void compact_tables();
+    bool compact() const { return compact_; }
+
     // Launch shapes (results are identical for every choice). autotune()
     // times the candidates on the current state and keeps the fastest,
     // cached per mesh size and numerics for the process; it runs by itself
@@ -224,6 +230,8 @@ private:
     uint32_t* blk_off_ = nullptr;
     int32_t* blk_ids_ = nullptr;
     double *ta_ = nullptr, *A_ = nullptr, *fbasis_ = nullptr;
+    bool compact_ = false;
+    double* fbc_ = nullptr;  // compact tables: basis values per reference edge
     double *Un_ = nullptr, *Ua_ = nullptr, *Ub_ = nullptr, *W_ = nullptr, *prims_ = nullptr, *h_ = nullptr;
     double *fs_ = nullptr, *rhs_ = nullptr, *lam_cell_ = nullptr, *dt_dev_ = nullptr, *dt_hist_ = nullptr;
     unsigned long long* lam_blk_ = nullptr;  // per-block spectral maxima (room for 64-cell blocks)

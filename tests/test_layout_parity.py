@@ -93,3 +93,21 @@ def test_rejects_mesh_smaller_than_stencil(mlr, nx, ny, order):
     where the oracle's ring search runs out of memory."""
     with pytest.raises(mlr.MallardError, match="mesh too small"):
         _layout(mlr, "cartesian_tri", nx, ny, 0.0, 0, recon="TENO", order=order, teno_variant="corrected")
+
+
+def test_slot_edge_ids(mlr, oracle):
+    """Slot 0's info word holds, for each face slot l in bits 24 + 2 l, the
+    edge of the reference triangle the face maps to in the target frame:
+    0 for the cell's nodes 0-1, 1 for 1-2, 2 for 2-0 (the compact tables'
+    face basis rows)."""
+    L = _layout(mlr, "cartesian_tri", 9, 7, 0.2, 5, recon="TENO", order=2)
+    m = oracle.Mesh("cartesian_tri", 9, 7, 1.0, 1.0, 0.2, 5)
+    edge = {frozenset((0, 1)): 0, frozenset((1, 2)): 1, frozenset((0, 2)): 2}
+    for c in range(m.n_cells):
+        info0 = int(L.slots(c)["info"][0])
+        nodes = list(m.nodes_of_cell[3 * c:3 * c + 3])
+        for l in range(3):
+            f = m.faces_of_cell[3 * c + l]
+            a, b = m.nodes_of_face[2 * f], m.nodes_of_face[2 * f + 1]
+            want = edge[frozenset((nodes.index(a), nodes.index(b)))]
+            assert (info0 >> (24 + 2 * l)) & 3 == want, (c, l)
