@@ -246,10 +246,10 @@ def learned_run(args, mlr, dev, U_plain, value_plain, wall_plain):
     return out
 
 
-def ensemble_sizes(n_cases, seed):
-    """Quads per side of each case: log-uniform in [64, 1024]."""
+def ensemble_sizes(n_cases, seed, lo=64, hi=1024):
+    """Quads per side of each case: log-uniform in [lo, hi]."""
     rng = np.random.default_rng(seed)
-    return np.round(np.exp(rng.uniform(np.log(64), np.log(1024), n_cases))).astype(int)
+    return np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), n_cases))).astype(int)
 
 
 def ensemble_run(args, mlr):
@@ -258,7 +258,7 @@ def ensemble_run(args, mlr):
     launch per kernel phase over all cases, one hipGraph per step), plain
     and with the learned gate. Returns the JSON record."""
     from mallard_amd import indicator as ind
-    sizes = ensemble_sizes(args.ens_cases, args.ens_size_seed)
+    sizes = ensemble_sizes(args.ens_cases, args.ens_size_seed, args.ens_min_side, args.ens_max_side)
     rng = np.random.default_rng(args.seed)
     states = [ind.random_quadrant_states(rng) for _ in sizes]
     gens = [mlr.mesh_gen("cartesian_tri", int(n), int(n), 1.0, 1.0, args.jitter, args.seed + k)
@@ -293,6 +293,27 @@ def ensemble_run(args, mlr):
     U_plain = [ens.state(k)[0] for k in range(len(sizes))]
     plain = {"value": value_plain, "ms_per_step": wall / args.ens_steps * 1e3, "cases_physical": int(sum(ok)),
              "tune": list(tune)}
+
+    persistent = None
+    if not args.no_persistent:
+        # every step as one cooperative launch over all cases; states must equal the graph run's bit for bit
+        ens.set_persistent(True)
+        init()
+        if args.warmup > 0:
+            ens.run_steps(args.warmup)
+        init()
+        t = time.perf_counter()
+        ens.run_steps(args.ens_steps)
+        wall_p = time.perf_counter() - t
+        same = all(np.array_equal(ens.state(k)[0].view(np.int64), U_plain[k].view(np.int64))
+                   for k in range(len(sizes)))
+        value_p = N * 3 * args.ens_steps / wall_p
+        log(f"[bench] persistent: {wall_p:.3f} s, x{value_p / value_plain:.2f} vs graph, "
+            f"{ens.persistent_grid} blocks, bitwise equal {same}")
+        persistent = {"value": value_p, "ms_per_step": wall_p / args.ens_steps * 1e3,
+                      "speedup_vs_graph": value_p / value_plain, "grid": ens.persistent_grid,
+                      "bitwise_equal_graph": bool(same)}
+        ens.set_persistent(False)
 
     learned = None
     if not args.no_learned:
@@ -345,12 +366,14 @@ def ensemble_run(args, mlr):
                         f"(rho, p ~ U[0.1, 2], u, v ~ U[-1.5, 1.5], seed {args.seed}) on jittered triangle "
                         f"meshes of {sizes.min()}^2 .. {sizes.max()}^2 quads (log-uniform draw, seed "
                         f"{args.ens_size_seed}), TENO p={args.order} + HLLC + SSPRK3, CFL 0.4, one launch "
-                        f"per kernel phase, hipGraph per step",
+                        f"per kernel phase, hipGraph per step (the persistent record: one cooperative "
+                        f"launch per step)",
             "n_cells": N,
             "n_cases": len(sizes),
             "indicator": "learned MLP gate, threshold %g" % args.threshold if matched else "none",
         },
         "plain": plain,
+        "persistent": persistent,
         "learned": learned,
     }
 
@@ -440,6 +463,10 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=50)
     ap.add_argument("--ens-size-seed", type=int, default=2112,
                     help="seed of the case-size draw (2112: 70.3M cells in 256 cases)")
+    ap.add_argument("--ens-min-side", type=int, default=64, help="smallest case: quads per side")
+    ap.add_argument("--ens-max-side", type=int, default=1024, help="largest case: quads per side")
+    ap.add_argument("--no-persistent", action="store_true",
+                    help="ensemble: skip the timing of the one-launch-per-step (persistent) mode")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))

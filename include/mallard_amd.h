@@ -338,6 +338,25 @@ int mlr_ensemble_autotune(mlr_ensemble* ens, int32_t* tune);
 int mlr_ensemble_set_tune(mlr_ensemble* ens, const int32_t* tune);
 int mlr_ensemble_set_indicator(mlr_ensemble* ens, const float* params, double threshold);
 int mlr_ensemble_gated(mlr_ensemble* ens, int reset, int64_t* gated);
+/* 1: mlr_ensemble_run_steps issues each step as ONE persistent cooperative
+ * launch over every case (the spectral radii and dt, then per stage the TENO
+ * and flux + update phases, grid-wide barriers between phases) instead of
+ * the graph of per-phase launches; results are identical. For ensembles of
+ * many small cases, whose per-phase launches are too short to fill the GPU.
+ * Plain TENO at p = 2 with the default stencil size and tile width, or first
+ * order; fixes the flux and spectral blocks at 256 cells. 0: back to the
+ * graph. mlr_ensemble_run_steps_profiled then times each phase from device
+ * clock stamps at the barriers. Like mlr_ensemble_run_steps, it has no
+ * counterpart in the reference, which steps one case per process
+ * (solver/solver.cpp:352-373). */
+int mlr_ensemble_set_persistent(mlr_ensemble* ens, int on);
+/* blocks of the last persistent launch (0 before the first) */
+int mlr_ensemble_persistent_grid(const mlr_ensemble* ens, int32_t* blocks);
+/* Watchdog of the persistent step: the longest a block waits at a grid
+ * barrier (device wall clock) before it gives up; the launch then ends and
+ * mlr_ensemble_run_steps fails with the cases' states invalid. Default
+ * 2000 ms, within [0, 10000]. */
+int mlr_ensemble_persistent_wait_limit(mlr_ensemble* ens, double ms);
 
 /* Host-only input parsing (no GPU), for the input-compatibility tests.
  * Solver::init's TOML reading (solver.cpp:82-273, mesh.cpp:32-66,

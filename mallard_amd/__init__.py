@@ -120,6 +120,9 @@ _SIGS = {
     "mlr_ensemble_set_tune": (C.c_int, [_P, _P]),
     "mlr_ensemble_set_indicator": (C.c_int, [_P, _P, C.c_double]),
     "mlr_ensemble_gated": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
+    "mlr_ensemble_set_persistent": (C.c_int, [_P, C.c_int]),
+    "mlr_ensemble_persistent_grid": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "mlr_ensemble_persistent_wait_limit": (C.c_int, [_P, C.c_double]),
     "mlr_write_vtu": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_char_p), C.c_int]),
     "mlr_layout_write_vtu": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_char_p), C.c_int, _P, _P, _P]),
     "mlr_expr_eval": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, _P]),
@@ -634,16 +637,27 @@ class Ensemble:
         t = np.array([teno_occ, block, block, teno_occ_reuse], np.int32)
         self._check(lib().mlr_ensemble_set_tune(self.h, _p(t)))
 
-    def gate_decisions(self):
-        """The network's last decisions inside the TENO pass (1: gated), per cell."""
-        out = np.zeros(self.n_cells, np.uint8)
-        self._check(lib().mlr_indicator_decisions(self.h, _p(out)))
-        return out
-
     def gated(self, reset=False):
         n = C.c_int64()
         self._check(lib().mlr_ensemble_gated(self.h, int(reset), C.byref(n)))
         return n.value
+
+    def set_persistent(self, on=True):
+        """Each step as one persistent cooperative launch over every case
+        (plain TENO p = 2 or first order; 256-cell flux blocks)."""
+        self._check(lib().mlr_ensemble_set_persistent(self.h, int(bool(on))))
+
+    def persistent_wait_limit(self, ms):
+        """Longest wait of a block at a grid barrier of the persistent step
+        before the step is abandoned and run_steps raises (default 2000 ms)."""
+        self._check(lib().mlr_ensemble_persistent_wait_limit(self.h, float(ms)))
+
+    @property
+    def persistent_grid(self):
+        """Blocks of the last persistent launch (0 before the first)."""
+        b = C.c_int32()
+        self._check(lib().mlr_ensemble_persistent_grid(self.h, C.byref(b)))
+        return b.value
 
 
 N_FEATURES = 48

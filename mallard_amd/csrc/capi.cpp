@@ -963,6 +963,20 @@ int mlr_ensemble_indicator_split(mlr_ensemble* ens, int split) {
     return guarded(nullptr, [&] { ens->e->set_indicator_split(split != 0); });
 }
 
+int mlr_ensemble_set_persistent(mlr_ensemble* ens, int on) {
+    if (!ens) return 1;
+    return guarded(nullptr, [&] { ens->e->set_persistent(on != 0); });
+}
+int mlr_ensemble_persistent_wait_limit(mlr_ensemble* ens, double ms) {
+    if (!ens) return 1;
+    return guarded(nullptr, [&] { ens->e->set_persistent_wait_limit(ms); });
+}
+int mlr_ensemble_persistent_grid(const mlr_ensemble* ens, int32_t* blocks) {
+    if (!ens || !blocks) return 1;
+    *blocks = ens->e->persistent_grid();
+    return 0;
+}
+
 int mlr_indicator_decisions(mlr_ctx* ctx, uint8_t* out) {
     if (!ctx || !out) return 1;
     return guarded(ctx, [&] { ctx->solver.get_gate_decisions(out); });

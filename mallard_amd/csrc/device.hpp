@@ -194,6 +194,33 @@ hipError_t spectral_batch(const CaseBufs* cases, int n_cases, uint32_t total_blo
                           hipStream_t stream);
 hipError_t finalize_batch(const CaseBufs* cases, int n_cases, double cfl, double fixed_dt, int64_t hist_cap,
                           uint32_t bs, hipStream_t stream);
+// A whole ensemble step in one persistent cooperative launch (the spectral
+// radii and dt of every case, then per stage the TENO and flux + update
+// phases with grid barriers between them): plain TENO at p = 2 with the
+// default stencil size and 8-bit tile positions (teno) or first order, flux
+// and spectral blocks of 256 cells in the case table. Stage st runs op[st]
+// from BufRole in[st] to out[st]. grid_out receives the blocks launched.
+// Barrier state of the persistent step (zeroed once; each barrier leaves
+// count at 0). err: a block gave up waiting.
+struct StepBarrier {
+    uint32_t count, gen, err, pad;
+};
+struct PersistentStep {
+    int32_t n_stages;
+    int32_t op[4], in[4], out[4];
+    int32_t fo, corrected, use_cfl;
+    double cfl, fixed_dt;
+    int64_t hist_cap;
+    // NULL, or 3 + 2 n_stages device wall-clock stamps taken by block 0 after
+    // each barrier (start, radii, dt, then per stage TENO and flux): the
+    // flux phase of the last stage then ends with a barrier too
+    long long* clocks;
+    StepBarrier* bar;
+    long long spin_ticks;  // longest wait at a barrier, device wall-clock ticks
+};
+hipError_t ensemble_step(const CaseBufs* cases, int n_cases, uint32_t teno_total, uint32_t flux_total,
+                         const PersistentStep& ps, int riemann, bool teno, const Physics& ph, hipStream_t stream,
+                         int* grid_out);
 hipError_t probe_math(const double* x, const double* y, double* out, int n, int which, hipStream_t stream);
 
 // Point-wise operators (operators.hip).

@@ -58,6 +58,7 @@ Ensemble::~Ensemble() {
     if (graph_) (void)hipGraphDestroy(graph_);
     for (auto& e : prof_) (void)hipEventDestroy(e.second);
     if (d_cases_) (void)hipFree(d_cases_);
+    if (d_bar_) (void)hipFree(d_bar_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -95,6 +96,7 @@ void Ensemble::drop_graph() {
 }
 
 void Ensemble::set_indicator(const float* params, double threshold) {
+    if (persistent_ && params) fail("the persistent step runs plain TENO: switch it off before setting an indicator");
     for (auto& c : cases_) c->set_indicator(params, threshold);
     upload_table();
     drop_graph();
@@ -116,6 +118,7 @@ void Ensemble::set_indicator_split(bool split) {
 void Ensemble::set_tune(const dev::Tune& t) {
     auto bs_ok = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
     if (!bs_ok(t.flux_block)) fail("block sizes must be 64, 128 or 256 cells");
+    if (persistent_ && t.flux_block != 256) fail("the persistent step runs flux and spectral blocks of 256 cells");
     auto occ_ok = [](int o) { return o == 0 || o == 5 || o == 6; };
     if (!occ_ok(t.teno_occ) || !occ_ok(t.teno_occ_reuse)) fail("TENO occupancy target must be 0, 5 or 6");
     tune_ = t;
@@ -129,6 +132,7 @@ void Ensemble::set_tune(const dev::Tune& t) {
 // write only scratch outputs (face states, the first stage's U and
 // accumulator, the radii); gated-cell counters are restored afterwards.
 void Ensemble::autotune() {
+    if (persistent_) return;  // the persistent step's shapes are fixed
     drop_graph();
     const int n = size();
     hipEvent_t e0, e1;
@@ -254,6 +258,87 @@ void Ensemble::launch_step(bool prof) {
     }
 }
 
+void Ensemble::set_persistent(bool on) {
+    if (on == persistent_) return;
+    if (on) {
+        if (teno_ && !(K_ == 6 && M_ == 12 && FQ_ == 3 && small_))
+            fail("the persistent step runs TENO at p = 2 with the default stencil size and tile width");
+        if (gated_) fail("the persistent step runs plain TENO: switch the indicator off first");
+        int dev = 0, coop = 0;
+        hip_ok(hipGetDevice(&dev), "hipGetDevice");
+        hip_ok(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev), "hipDeviceGetAttribute");
+        if (!coop) fail("this device has no cooperative launches");
+        int khz = 0;
+        hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
+        if (khz <= 0) fail("the device reports no wall-clock rate");
+        spin_ticks_ = (long long)(wait_limit_ms_ * khz);
+        if (!d_bar_) {
+            hip_ok(hipMalloc((void**)&d_bar_, sizeof(dev::StepBarrier)), "hipMalloc");
+            hip_ok(hipMemset(d_bar_, 0, sizeof(dev::StepBarrier)), "hipMemset");
+        }
+        dev::Tune t = tune_;
+        t.flux_block = 256;
+        persistent_ = true;
+        set_tune(t);  // rebuilds the case table's block offsets for 256-cell blocks
+    } else {
+        persistent_ = false;
+        tuned_ = false;  // the per-phase shapes are timed again
+        drop_graph();
+    }
+}
+
+void Ensemble::set_persistent_wait_limit(double ms) {
+    if (!(ms >= 0.0) || ms > 10000.0) fail("the wait limit must be within [0, 10000] ms");
+    wait_limit_ms_ = ms;
+    if (persistent_) {
+        int dev = 0, khz = 0;
+        hip_ok(hipGetDevice(&dev), "hipGetDevice");
+        hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
+        spin_ticks_ = (long long)(wait_limit_ms_ * khz);
+    }
+}
+
+// After a persistent run has synchronised: a block that gave up at a barrier
+// leaves the barrier state inconsistent and the states invalid.
+void Ensemble::check_barrier() {
+    dev::StepBarrier b{};
+    hip_ok(hipMemcpy(&b, d_bar_, sizeof(b), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (b.err == 0) return;
+    hip_ok(hipMemset(d_bar_, 0, sizeof(dev::StepBarrier)), "hipMemset");
+    fail("persistent step: a block waited longer than " + std::to_string(wait_limit_ms_) +
+         " ms at a grid barrier; the step was abandoned and the cases' states are not valid");
+}
+
+// One step of every case: the spectral radii and dt, then per stage the TENO
+// and the flux + update phases, in one cooperative launch.
+void Ensemble::launch_persistent(long long* clocks) {
+    dev::PersistentStep ps{};
+    ps.clocks = clocks;
+    ps.bar = d_bar_;
+    ps.spin_ticks = spin_ticks_;
+    if (cfg_.integrator == Integrator::SSPRK3) {
+        ps.n_stages = 3;
+        const int op[3] = {dev::kSSP1, dev::kSSP2, dev::kSSP3};
+        const int in[3] = {dev::kBufUn, dev::kBufUa, dev::kBufUb};
+        const int out[3] = {dev::kBufUa, dev::kBufUb, dev::kBufUn};
+        for (int k = 0; k < 3; ++k) { ps.op[k] = op[k]; ps.in[k] = in[k]; ps.out[k] = out[k]; }
+    } else {
+        ps.n_stages = 4;
+        const int op[4] = {dev::kRK4_1, dev::kRK4_2, dev::kRK4_3, dev::kRK4_4};
+        const int in[4] = {dev::kBufUn, dev::kBufUa, dev::kBufUb, dev::kBufUa};
+        const int out[4] = {dev::kBufUa, dev::kBufUb, dev::kBufUa, dev::kBufUn};
+        for (int k = 0; k < 4; ++k) { ps.op[k] = op[k]; ps.in[k] = in[k]; ps.out[k] = out[k]; }
+    }
+    ps.fo = teno_ ? 0 : 1;
+    ps.corrected = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
+    ps.use_cfl = cfg_.use_cfl ? 1 : 0;
+    ps.cfl = cfg_.cfl;
+    ps.fixed_dt = cfg_.use_cfl ? 0.0 : cfg_.dt;
+    ps.hist_cap = cases_[0]->dt_hist_cap_;
+    hip_ok(dev::ensemble_step(d_cases_, size(), teno_blocks_, flux_blocks_, ps, (int)cfg_.riemann, teno_, ph_, stream_,
+                              &persistent_grid_), "ensemble_step");
+}
+
 void Ensemble::finish(int64_t s0, int64_t n) {
     for (auto& c : cases_) c->finish_steps(s0, n, nullptr);
 }
@@ -263,6 +348,19 @@ void Ensemble::finish(int64_t s0, int64_t n) {
 void Ensemble::run_steps(int64_t n) {
     if (n <= 0) return;
     check_dt_config(cfg_);
+    const int64_t cap = cases_[0]->dt_hist_cap_;
+    if (persistent_) {
+        for (int64_t done = 0; done < n;) {
+            const int64_t m = std::min<int64_t>(n - done, cap);
+            const int64_t s0 = cases_[0]->device_step();
+            for (int64_t i = 0; i < m; ++i) launch_persistent(nullptr);
+            hip_ok(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+            check_barrier();
+            finish(s0, m);
+            done += m;
+        }
+        return;
+    }
     if (!tuned_) autotune();
     if (!exec_) {
         hip_ok(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -277,7 +375,6 @@ void Ensemble::run_steps(int64_t n) {
         hip_ok(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
         hip_ok(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
     }
-    const int64_t cap = cases_[0]->dt_hist_cap_;
     for (int64_t done = 0; done < n;) {
         const int64_t m = std::min<int64_t>(n - done, cap);
         const int64_t s0 = cases_[0]->device_step();
@@ -288,10 +385,72 @@ void Ensemble::run_steps(int64_t n) {
     }
 }
 
+// Persistent mode: every step's phases from the device wall clock stamped
+// at the barriers (radii + dt from the start, TENO and flux per stage).
+void Ensemble::run_steps_profiled_persistent(int64_t n, double* stats) {
+    const int64_t cap = cases_[0]->dt_hist_cap_;
+    const int stages = cfg_.integrator == Integrator::SSPRK3 ? 3 : 4, marks = 3 + 2 * stages;
+    int dev = 0, khz = 0;
+    hip_ok(hipGetDevice(&dev), "hipGetDevice");
+    hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
+    if (khz <= 0) fail("the device reports no wall-clock rate");
+    const double ms_per_tick = 1.0 / khz;
+    const int64_t m_max = std::min<int64_t>(n, cap);
+    long long* d_clk = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_ok(hipMalloc((void**)&d_clk, (size_t)m_max * marks * sizeof(long long)), "hipMalloc");
+    std::vector<long long> clk((size_t)m_max * marks);
+    try {
+        hip_ok(hipEventCreate(&e0), "hipEventCreate");
+        hip_ok(hipEventCreate(&e1), "hipEventCreate");
+        for (int64_t done = 0; done < n;) {
+            const int64_t m = std::min<int64_t>(n - done, cap);
+            const int64_t s0 = cases_[0]->device_step();
+            hip_ok(hipEventRecord(e0, stream_), "hipEventRecord");
+            for (int64_t i = 0; i < m; ++i) launch_persistent(d_clk + i * marks);
+            hip_ok(hipEventRecord(e1, stream_), "hipEventRecord");
+            hip_ok(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+            check_barrier();
+            float tot = 0.0f;
+            hip_ok(hipEventElapsedTime(&tot, e0, e1), "hipEventElapsedTime");
+            stats[kStatTotalMs] += tot;
+            hip_ok(hipMemcpy(clk.data(), d_clk, (size_t)m * marks * sizeof(long long), hipMemcpyDeviceToHost),
+                   "hipMemcpy");
+            for (int64_t i = 0; i < m; ++i) {
+                const long long* c = clk.data() + i * marks;
+                stats[kStatDtMs] += (double)(c[2] - c[0]) * ms_per_tick;
+                stats[kStatDtN] += 1.0;
+                for (int st = 0; st < stages; ++st) {
+                    if (teno_) {
+                        stats[kStatTenoMs] += (double)(c[3 + 2 * st] - c[2 + 2 * st]) * ms_per_tick;
+                        stats[kStatTenoN] += 1.0;
+                    }
+                    stats[kStatFluxMs] += (double)(c[4 + 2 * st] - c[3 + 2 * st]) * ms_per_tick;
+                    stats[kStatFluxN] += 1.0;
+                }
+            }
+            finish(s0, m);
+            done += m;
+        }
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(d_clk);
+        throw;
+    }
+    hip_ok(hipEventDestroy(e0), "hipEventDestroy");
+    hip_ok(hipEventDestroy(e1), "hipEventDestroy");
+    hip_ok(hipFree(d_clk), "hipFree");
+}
+
 void Ensemble::run_steps_profiled(int64_t n, double* stats) {
     for (int k = 0; k < kStatCount; ++k) stats[k] = 0.0;
     if (n <= 0) return;
     check_dt_config(cfg_);
+    if (persistent_) {
+        run_steps_profiled_persistent(n, stats);
+        return;
+    }
     if (!tuned_) autotune();
     const int64_t cap = cases_[0]->dt_hist_cap_;
     for (int64_t done = 0; done < n;) {

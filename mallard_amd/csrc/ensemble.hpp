@@ -47,10 +47,24 @@ public:
     void run_steps(int64_t n);
     // the same launches one at a time with an event after each (Solver's StepStats layout)
     void run_steps_profiled(int64_t n, double* stats);
+    // run_steps as one persistent cooperative launch per step (every phase of
+    // every case, grid barriers between phases) instead of the graph of
+    // per-phase launches: plain TENO at p = 2 with the default stencil size
+    // and tile width, or first order. Pins flux and spectral blocks to 256
+    // cells. Profiled runs then take each phase's time from device clock
+    // stamps at the barriers (the last stage's flux phase gains a barrier).
+    void set_persistent(bool on);
+    bool persistent() const { return persistent_; }
+    int persistent_grid() const { return persistent_grid_; }
+    // longest wait of a block at a grid barrier of the persistent step before
+    // the step is abandoned and run_steps fails (default 2000 ms)
+    void set_persistent_wait_limit(double ms);
 
 private:
     void upload_table();
     void launch_step(bool prof);
+    void launch_persistent(long long* clocks);
+    void run_steps_profiled_persistent(int64_t n, double* stats);
     void mark(int tag);
     void drop_graph();
     void finish(int64_t s0, int64_t n);
@@ -67,6 +81,12 @@ private:
     hipStream_t stream_ = nullptr;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
+    bool persistent_ = false;
+    int persistent_grid_ = 0;
+    dev::StepBarrier* d_bar_ = nullptr;
+    double wait_limit_ms_ = 2000.0;
+    long long spin_ticks_ = 0;
+    void check_barrier();
     bool prof_on_ = false;
     std::vector<std::pair<int, hipEvent_t>> prof_;
     size_t prof_used_ = 0;

@@ -975,19 +975,134 @@ __global__ __launch_bounds__(1024) void finalize_dt_kernel(const unsigned long l
     if (step) *step += 1;
 }
 
-// per-case dt of an ensemble: block k finalises case k
-__global__ __launch_bounds__(256) void finalize_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases,
-                                                             double cfl, double fixed_dt, int64_t hist_cap,
-                                                             uint32_t bs) {
-    const int k = blockIdx.x;
-    if (k >= n_cases) return;
-    const CaseBufs& cs = cases[k];
+// one case's dt from its per-block maxima (blocks of bs cells), its history
+// entry and step counter; every thread of the block takes part
+__device__ inline void finalize_case(const CaseBufs& cs, double cfl, double fixed_dt, int64_t hist_cap, uint32_t bs) {
     const unsigned long long m = fixed_dt > 0.0 ? 0ull : join_blocks(cs.lam_blk, (cs.g.N + bs - 1) / bs);
     if (threadIdx.x != 0) return;
     const double d = fixed_dt > 0.0 ? fixed_dt : cfl / __longlong_as_double((long long)m);
     *cs.dt = d;
     cs.dt_hist[*cs.step % hist_cap] = d;
     *cs.step += 1;
+}
+
+// per-case dt of an ensemble: block k finalises case k
+__global__ __launch_bounds__(256) void finalize_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases,
+                                                             double cfl, double fixed_dt, int64_t hist_cap,
+                                                             uint32_t bs) {
+    const int k = blockIdx.x;
+    if (k >= n_cases) return;
+    finalize_case(cases[k], cfl, fixed_dt, hist_cap, bs);
+}
+
+// ------------------------------------------------------------ persistent ensemble step
+// The whole step of every case in one cooperative launch: the spectral radii
+// and each case's dt, then per stage the TENO face states and the flux +
+// stage update, with a grid-wide barrier between phases. The grid holds as
+// many blocks as fit on the GPU at once; each phase's work items (64-cell
+// TENO blocks, 256-cell flux / spectral blocks, over all cases through the
+// offset table) are dealt out so that each XCD walks one contiguous eighth,
+// as the per-phase kernels do. Every item runs the per-phase kernels' own
+// per-cell code, so results are bit-identical to them.
+// Grid-wide barrier of the persistent launch (every block resident: the
+// cooperative launch guarantees it). Each wave first waits for its own
+// stores to reach its XCD's L2; then one thread per block releases them past
+// that L2 (each XCD has its own), arrives on a counter and waits for the
+// generation the last arrival advances, and drops stale lines of its L2 once
+// through. The waits poll with relaxed loads and no invalidation, so blocks
+// that arrive early leave the L2 of those still working alone. A wait longer
+// than spin_ticks of the device wall clock gives up and raises bar->err (the
+// launch then still ends; the host reports the error and resets the state).
+__device__ inline void grid_barrier(StepBarrier* bar, uint32_t n_blocks, long long spin_ticks) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t g = __hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == n_blocks - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every arrival's release is ordered before
+            __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar->gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const long long t0 = wall_clock64();
+            while (__hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(8);
+                if (wall_clock64() - t0 > spin_ticks) {
+                    __hip_atomic_store(&bar->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+template <int RS, bool TENO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void ensemble_step_kernel(const CaseBufs* __restrict__ cases, int n_cases,
+                                                            uint32_t teno_total, uint32_t flux_total,
+                                                            PersistentStep ps, Physics ph) {
+    extern __shared__ __align__(16) unsigned char step_smem[];
+    const uint32_t G = gridDim.x, xcd = blockIdx.x & 7u, rank = blockIdx.x >> 3;
+    const uint32_t on_xcd = (G + 7u - xcd) >> 3;
+    auto stamp = [&](int i) {
+        if (ps.clocks && blockIdx.x == 0 && threadIdx.x == 0) ps.clocks[i] = wall_clock64();
+    };
+    stamp(0);
+    auto each = [&](uint32_t total, auto&& f) {
+        const uint32_t per = (total + 7u) >> 3, lo = xcd * per, hi = lo + per < total ? lo + per : total;
+        for (uint32_t w = lo + rank; w < hi; w += on_xcd) {
+            f(w);
+            __syncthreads();  // the next item reuses the block's LDS
+        }
+    };
+    if (ps.use_cfl) {
+        each(flux_total, [&](uint32_t gb) {
+            const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, true)];
+            spectral_block(cs.g, cs.Un, cs.prims, ph, cs.lam_cell, cs.lam_blk, gb - cs.blk_flux0);
+        });
+        grid_barrier(ps.bar, G, ps.spin_ticks);
+    }
+    stamp(1);
+    for (int k = (int)blockIdx.x; k < n_cases; k += (int)G) {
+        finalize_case(cases[k], ps.cfl, ps.fixed_dt, ps.hist_cap, 256u);
+        __syncthreads();
+    }
+    grid_barrier(ps.bar, G, ps.spin_ticks);
+    stamp(2);
+#pragma unroll 1
+    for (int st = 0; st < ps.n_stages; ++st) {
+        if constexpr (TENO) {
+            each(teno_total, [&](uint32_t gb) {
+                const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, false)];
+                teno_block<5, 11, 3, uint8_t, 256, 0>(cs.g, cs.t, case_buf(cs, ps.in[st]), cs.fs, cs.dec, ph,
+                                                      gb - cs.blk_teno0, reinterpret_cast<double2*>(step_smem),
+                                                      step_smem + 256 * 32);
+            });
+            grid_barrier(ps.bar, G, ps.spin_ticks);
+        }
+        stamp(3 + 2 * st);
+        each(flux_total, [&](uint32_t gb) {
+            const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, true)];
+            StageArgs a{};
+            a.op = ps.op[st];
+            a.riemann = RS;
+            a.fo = ps.fo;
+            a.corrected = ps.corrected;
+            a.Uin = case_buf(cs, ps.in[st]);
+            a.Un = cs.Un;
+            a.Uout = case_buf_w(cs, ps.out[st]);
+            a.W = cs.W;
+            a.prims = cs.prims;
+            a.h = cs.h;
+            a.rhs = cs.rhs;
+            a.fs = cs.fs;
+            a.dt = cs.dt;
+            flux_cell<RS>(cs.g, a, ph, (gb - cs.blk_flux0) * blockDim.x + threadIdx.x);
+        });
+        if (st + 1 < ps.n_stages || ps.clocks) grid_barrier(ps.bar, G, ps.spin_ticks);
+        stamp(4 + 2 * st);
+    }
 }
 
 __global__ void copy_kernel(const double* __restrict__ s, double* __restrict__ d, size_t n) {
@@ -1298,6 +1413,47 @@ hipError_t finalize_batch(const CaseBufs* cases, int n_cases, double cfl, double
     hipLaunchKernelGGL(finalize_batch_kernel, dim3(n_cases), dim3(256), 0, stream, cases, n_cases, cfl, fixed_dt,
                        hist_cap, bs);
     return hipGetLastError();
+}
+
+namespace {
+template <int RS, bool TENO>
+hipError_t launch_step_persistent(const CaseBufs* cases, int n_cases, uint32_t teno_total, uint32_t flux_total,
+                                  const PersistentStep& ps, const Physics& ph, hipStream_t stream, int* grid_out) {
+    const void* fn = reinterpret_cast<const void*>(&ensemble_step_kernel<RS, TENO>);
+    const size_t lds = TENO ? 256 * 32 : 0;
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds); e != hipSuccess) return e;
+    if (hipError_t e = hipGetDevice(&dev); e != hipSuccess) return e;
+    if (hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev); e != hipSuccess)
+        return e;
+    const int grid = per_cu * cus;  // every block resident at once: the grid barrier needs it
+    if (grid < 8) return hipErrorInvalidConfiguration;  // each XCD's range needs a block
+    if (grid_out) *grid_out = grid;
+    const CaseBufs* c = cases;
+    int n = n_cases;
+    uint32_t tt = teno_total, ft = flux_total;
+    PersistentStep p = ps;
+    Physics h = ph;
+    void* args[] = {&c, &n, &tt, &ft, &p, &h};
+    return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), args, (unsigned)lds, stream);
+}
+}  // namespace
+
+hipError_t ensemble_step(const CaseBufs* cases, int n_cases, uint32_t teno_total, uint32_t flux_total,
+                         const PersistentStep& ps, int riemann, bool teno, const Physics& ph, hipStream_t stream,
+                         int* grid_out) {
+    if (n_cases < 1 || ps.n_stages < 1 || ps.n_stages > 4) return hipErrorInvalidValue;
+#define MLR_STEP(RSV)                                                                                          \
+    return teno ? launch_step_persistent<RSV, true>(cases, n_cases, teno_total, flux_total, ps, ph, stream, grid_out) \
+                : launch_step_persistent<RSV, false>(cases, n_cases, teno_total, flux_total, ps, ph, stream, grid_out)
+    switch (riemann) {
+    case 0: MLR_STEP(0);
+    case 1: MLR_STEP(1);
+    case 2: MLR_STEP(2);
+    case 3: MLR_STEP(3);
+    default: return hipErrorInvalidValue;
+    }
+#undef MLR_STEP
 }
 
 hipError_t viscous_grad(const Geom& g, const double* U, const Physics& ph, double* vp, double* vg, uint32_t bs,
