@@ -75,6 +75,13 @@ typedef struct {
      * sutherland_S < 0) and heat conductivity mu cp / prandtl. */
     int32_t viscous;
     double mu_ref, T_mu_ref, sutherland_S, prandtl;
+    /* positivity != 0: the positivity-preserving flux limiter of Hu, Adams &
+     * Shu (J. Comput. Phys. 242 (2013) 169) on every face of every stage
+     * update (no reference counterpart). Each face's flux is blended toward
+     * the Rusanov flux of the two cell means just enough that both cells'
+     * one-face forward-Euler brackets keep rho, p > 0; faces it leaves alone
+     * give bit for bit the unlimited result. FE and SSP-RK3, Euler only. */
+    int32_t positivity;
 } mlr_numerics_desc;
 
 /* Structured generators of the reference (mesh.cpp:305-848). jitter moves

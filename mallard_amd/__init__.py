@@ -39,7 +39,7 @@ class MlrNumerics(C.Structure):
                 ("cfl", C.c_double), ("dt", C.c_double), ("n_bc", C.c_int32),
                 ("bc", MlrBC * 14), ("n_threads", C.c_int32),
                 ("viscous", C.c_int32), ("mu_ref", C.c_double), ("T_mu_ref", C.c_double),
-                ("sutherland_S", C.c_double), ("prandtl", C.c_double)]
+                ("sutherland_S", C.c_double), ("prandtl", C.c_double), ("positivity", C.c_int32)]
 
 
 class MlrMeshDesc(C.Structure):
@@ -189,13 +189,15 @@ def numerics(recon="TENO", basis="legendre", order=2, stencil_factor=2.0, quad_o
              quad_order_face=0, teno_variant="reference", riemann="HLLC", integrator="SSPRK3",
              gamma=1.4, p_ref=101325.0, T_ref=298.15, rho_ref=1.225, p_min=-1e20, p_max=1e20,
              cfl=0.4, dt=0.0, bcs=None, n_threads=0, physics="euler", mu_ref=0.0, T_mu_ref=0.0,
-             sutherland_S=-1.0, prandtl=0.72):
+             sutherland_S=-1.0, prandtl=0.72, positivity=False):
     """Numerics descriptor with the reference's defaults and key strings.
 
     bcs: list of (zone, type, params) with the [[boundaries]] keys.
     physics: "euler" (the reference's) or "navier-stokes" (Sutherland
     viscosity mu_ref (T/T_mu_ref)^1.5 (T_mu_ref+S)/(T+S), constant for S < 0;
     T_mu_ref <= 0 takes T_ref).
+    positivity: the positivity-preserving flux limiter (Hu, Adams & Shu
+    2013) on every stage update (FE and SSPRK3, Euler).
     """
     d = MlrNumerics()
     d.recon, d.basis, d.order = RECON[recon], BASIS[basis], order
@@ -220,6 +222,7 @@ def numerics(recon="TENO", basis="legendre", order=2, stencil_factor=2.0, quad_o
         raise ValueError("Unknown physics type: " + physics)
     d.viscous = 1 if physics == "navier-stokes" else 0
     d.mu_ref, d.T_mu_ref, d.sutherland_S, d.prandtl = mu_ref, T_mu_ref, sutherland_S, prandtl
+    d.positivity = 1 if positivity else 0
     return d
 
 

@@ -104,6 +104,7 @@ mlr::Config config_from_desc(const mlr_numerics_desc* d) {
         c.Pr = d->prandtl > 0.0 ? d->prandtl : 0.72;
         if (!(c.mu_ref >= 0.0)) fail("Invalid viscosity parameters.");
     }
+    c.positivity = d->positivity != 0;
     c.n_steps = 0;
     return c;
 }
@@ -156,6 +157,7 @@ void desc_from_config(const mlr::Config& c, mlr_mesh_gen* mesh, mlr_numerics_des
         d->T_mu_ref = c.T_mu;
         d->sutherland_S = c.S_mu;
         d->prandtl = c.Pr;
+        d->positivity = c.positivity ? 1 : 0;
     }
 }
 

@@ -115,6 +115,7 @@ struct StageArgs {
     // two-rarefaction estimate, and extrapolation boundaries whose ghost state
     // is the cell mean (a zero-gradient ghost cell) instead of the face state
     int32_t corrected;
+    int32_t pp;           // 1: positivity-preserving flux limiter (stage updates)
     const double* Uin;    // stage input state (gathered), [N][4]
     const double* Un;     // step start state
     double* Uout;
@@ -144,6 +145,7 @@ enum BufRole : int32_t { kBufUn = 0, kBufUa = 1, kBufUb = 2, kBufNone = 3 };
 struct BatchStage {
     int32_t op, fo, corrected;
     int32_t in, out;  // BufRole of the stage input and output
+    int32_t pp = 0;   // 1: positivity-preserving flux limiter
 };
 
 // Launch shapes an autotuner picks per mesh. Every choice gives identical

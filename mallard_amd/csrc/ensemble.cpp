@@ -151,7 +151,7 @@ void Ensemble::autotune() {
     dev::Tune best = tune_;
     const int corrected = (teno_ && cfg_.teno.variant == TenoVariant::Corrected) ? 1 : 0;
     const int op = cfg_.integrator == Integrator::RK4 ? dev::kRK4_1 : dev::kSSP1;
-    const dev::BatchStage st{op, teno_ ? 0 : 1, corrected, dev::kBufUn, dev::kBufUa};
+    const dev::BatchStage st{op, teno_ ? 0 : 1, corrected, dev::kBufUn, dev::kBufUa, cfg_.positivity ? 1 : 0};
     float bt = 1e30f;
     for (uint32_t bs : {64u, 128u, 256u}) {
         tune_.flux_block = tune_.spec_block = bs;
@@ -242,7 +242,7 @@ void Ensemble::launch_step(bool prof) {
                    "teno_batch");
             mark(kStatTenoMs);
         }
-        const dev::BatchStage st{op, teno_ ? 0 : 1, corrected, in, out};
+        const dev::BatchStage st{op, teno_ ? 0 : 1, corrected, in, out, cfg_.positivity ? 1 : 0};
         hip_ok(dev::flux_batch(d_cases_, n, flux_blocks_, st, (int)cfg_.riemann, ph_, bs, stream_), "flux_batch");
         mark(kStatFluxMs);
     };
@@ -264,6 +264,7 @@ void Ensemble::set_persistent(bool on) {
         if (teno_ && !(K_ == 6 && M_ == 12 && FQ_ == 3 && small_))
             fail("the persistent step runs TENO at p = 2 with the default stencil size and tile width");
         if (gated_) fail("the persistent step runs plain TENO: switch the indicator off first");
+        if (cfg_.positivity) fail("the persistent step has no positivity limiter");
         int dev = 0, coop = 0;
         hip_ok(hipGetDevice(&dev), "hipGetDevice");
         hip_ok(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev), "hipDeviceGetAttribute");

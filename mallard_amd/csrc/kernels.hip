@@ -676,7 +676,9 @@ __device__ inline void viscous_face(const Physics& ph, const Geom& g, const Stag
 // the same XCD.
 // PARTS: face sums taken, bit 0 interior faces, bit 1 boundary faces (3: the
 // whole residual; 1 or 2 only with kRhsOnly).
-template <int RS, int PARTS = 3>
+// PP: the positivity-preserving flux limiter (pp_limit) on every face, for
+// the stage updates (not the residual-only operator, which has no dt).
+template <int RS, int PARTS = 3, bool PP = false>
 __device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, const Physics& ph, uint32_t c) {
     const uint32_t N = g.N;
     if (c >= N) return;
@@ -684,6 +686,13 @@ __device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, con
     const uint32_t info0 = g.info[c];
     const uint32_t code = (info0 >> 8) & 0xFFu, nact = (info0 >> 16) & 0x7u;
     double r[4] = {0.0, 0.0, 0.0, 0.0};
+    D4 um{};
+    double vc = 0.0, pdt = 0.0;
+    if constexpr (PP) {
+        um = ld4(a.Uin, c);
+        vc = g.volume[c];
+        pdt = *a.dt;
+    }
     for (uint32_t rk = 0; rk < nact; ++rk) {
         const uint32_t l = (code >> (2 * rk)) & 3u;
         const size_t o = (size_t)l * N + c;
@@ -723,6 +732,19 @@ __device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, con
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) F[i] *= 0.5;
+        if constexpr (PP) {
+            // in side order (side 0 first), so both cells of a face agree bit for bit
+            const uint32_t nbc = bnd ? c : (uint32_t)g.nb[o];
+            const D4 mo = bnd ? um : ld4(a.Uin, nbc);
+            const double vo = bnd ? vc : g.volume[nbc];
+            const double* U0 = side ? mo.v : um.v;
+            const double* U1 = side ? um.v : mo.v;
+            const double V0 = side ? vo : vc, V1 = side ? vc : vo;
+            double Flo[4];
+            pp_low_flux(ph, g.bc, bnd, (inf >> 4) & 0xFu, n0, n1, U0, U1, Flo);
+            const double Sd = (double)g.S;
+            pp_limit(ph, F, Flo, U0, Sd * pdt * ar / V0, U1, Sd * pdt * ar / V1, bnd);
+        }
         if (ph.viscous) {
             double G[4];
             viscous_face(ph, g, a, o, c, inf, bnd, side, n0, n1, G);
@@ -807,10 +829,10 @@ __device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, con
     }
 }
 
-template <int RS>
+template <int RS, bool PP>
 __global__ __launch_bounds__(256) void flux_update_kernel(Geom g, StageArgs a, Physics ph, uint32_t blocks_per_xcd) {
     const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
-    flux_cell<RS>(g, a, ph, lb * blockDim.x + threadIdx.x);
+    flux_cell<RS, 3, PP>(g, a, ph, lb * blockDim.x + threadIdx.x);
 }
 
 // The residual restricted to the interior faces (PARTS = 1, the
@@ -824,7 +846,7 @@ __global__ __launch_bounds__(256) void rhs_parts_kernel(Geom g, StageArgs a, Phy
 
 // The flux + stage update of every case in one launch (blocks of blockDim.x
 // cells; the case table's block offsets count blocks of that size).
-template <int RS>
+template <int RS, bool PP>
 __global__ __launch_bounds__(256) void flux_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases,
                                                          BatchStage st, uint32_t total, Physics ph,
                                                          uint32_t blocks_per_xcd) {
@@ -845,7 +867,7 @@ __global__ __launch_bounds__(256) void flux_batch_kernel(const CaseBufs* __restr
     a.rhs = cs.rhs;
     a.fs = cs.fs;
     a.dt = cs.dt;
-    flux_cell<RS>(cs.g, a, ph, (gb - cs.blk_flux0) * blockDim.x + threadIdx.x);
+    flux_cell<RS, 3, PP>(cs.g, a, ph, (gb - cs.blk_flux0) * blockDim.x + threadIdx.x);
 }
 
 // ------------------------------------------------------------ dt
@@ -1277,13 +1299,20 @@ hipError_t flux_update(const Geom& g, const StageArgs& a, const Physics& ph, uin
     if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
     const uint32_t nblk = (g.N + bs - 1) / bs, per_xcd = (nblk + 7) / 8;
     const dim3 grid(8 * per_xcd), block(bs);
+    const bool pp = a.pp && a.op != kRhsOnly;
+#define MLR_FU(RSV)                                                                                           \
+    do {                                                                                                      \
+        if (pp) hipLaunchKernelGGL((flux_update_kernel<RSV, true>), grid, block, 0, stream, g, a, ph, per_xcd); \
+        else hipLaunchKernelGGL((flux_update_kernel<RSV, false>), grid, block, 0, stream, g, a, ph, per_xcd);   \
+    } while (0)
     switch (a.riemann) {
-    case 0: hipLaunchKernelGGL((flux_update_kernel<0>), grid, block, 0, stream, g, a, ph, per_xcd); break;
-    case 1: hipLaunchKernelGGL((flux_update_kernel<1>), grid, block, 0, stream, g, a, ph, per_xcd); break;
-    case 2: hipLaunchKernelGGL((flux_update_kernel<2>), grid, block, 0, stream, g, a, ph, per_xcd); break;
-    case 3: hipLaunchKernelGGL((flux_update_kernel<3>), grid, block, 0, stream, g, a, ph, per_xcd); break;
+    case 0: MLR_FU(0); break;
+    case 1: MLR_FU(1); break;
+    case 2: MLR_FU(2); break;
+    case 3: MLR_FU(3); break;
     default: return hipErrorInvalidValue;
     }
+#undef MLR_FU
     return hipGetLastError();
 }
 
@@ -1388,13 +1417,23 @@ hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks,
     if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
     const uint32_t per_xcd = (total_blocks + 7) / 8;
     const dim3 grid(8 * per_xcd), block(bs);
+#define MLR_FB(RSV)                                                                                   \
+    do {                                                                                              \
+        if (st.pp)                                                                                    \
+            hipLaunchKernelGGL((flux_batch_kernel<RSV, true>), grid, block, 0, stream, cases, n_cases, st, \
+                               total_blocks, ph, per_xcd);                                            \
+        else                                                                                          \
+            hipLaunchKernelGGL((flux_batch_kernel<RSV, false>), grid, block, 0, stream, cases, n_cases, st, \
+                               total_blocks, ph, per_xcd);                                            \
+    } while (0)
     switch (riemann) {
-    case 0: hipLaunchKernelGGL((flux_batch_kernel<0>), grid, block, 0, stream, cases, n_cases, st, total_blocks, ph, per_xcd); break;
-    case 1: hipLaunchKernelGGL((flux_batch_kernel<1>), grid, block, 0, stream, cases, n_cases, st, total_blocks, ph, per_xcd); break;
-    case 2: hipLaunchKernelGGL((flux_batch_kernel<2>), grid, block, 0, stream, cases, n_cases, st, total_blocks, ph, per_xcd); break;
-    case 3: hipLaunchKernelGGL((flux_batch_kernel<3>), grid, block, 0, stream, cases, n_cases, st, total_blocks, ph, per_xcd); break;
+    case 0: MLR_FB(0); break;
+    case 1: MLR_FB(1); break;
+    case 2: MLR_FB(2); break;
+    case 3: MLR_FB(3); break;
     default: return hipErrorInvalidValue;
     }
+#undef MLR_FB
     return hipGetLastError();
 }
 

@@ -222,5 +222,77 @@ __device__ inline void point_flux(const Physics& ph, const BCParam* __restrict__
     riemann_flux(riemann, guard, F, n0, n1, Cl[0], Wl, Wl[2], ph.gamma, Wl[4], rr, Wr, Wr[2], ph.gamma, Wr[4]);
 }
 
+// ---------------------------------------------------------------- positivity
+// Positivity-preserving flux limiter (Hu, Adams & Shu, J. Comput. Phys. 242
+// (2013) 169-180). No reference counterpart. A cell's forward-Euler update
+// splits into one bracket per face, U - S dt a / V F, whose mean over the
+// cell's S faces is the update. With the first-order Lax-Friedrichs flux of
+// the two cell means each bracket is admissible (rho, p > 0) while
+// S dt a alpha / V <= 1 (alpha the larger |u.n| + c); the face's flux is
+// blended theta F + (1 - theta) F_lo with the largest theta in [0, 1] that
+// keeps both neighbours' brackets admissible.
+
+__device__ inline double pp_pressure(const Physics& ph, const double* u) {
+    return (ph.gamma - 1.0) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0]);
+}
+
+// The low-order flux: Rusanov (the Lax-Friedrichs flux with the larger local
+// wave speed) of the cell means across the face; at a boundary, of the mean
+// and the boundary's ghost of it, walls taking the mirror state.
+__device__ inline void pp_low_flux(const Physics& ph, const BCParam* __restrict__ bct, bool bnd, int bc, double n0,
+                                   double n1, const double* U0, const double* U1, double* F) {
+    if (bnd && (bct[bc].type == 2 || bct[bc].type == 5)) {
+        double W[5], Wg[5];
+        prims(ph, U0, W);
+        for (int i = 0; i < 5; i++) Wg[i] = W[i];
+        const double un = W[0] * n0 + W[1] * n1;
+        Wg[0] = W[0] - 2.0 * un * n0;
+        Wg[1] = W[1] - 2.0 * un * n1;
+        riemann_flux(0, 0, F, n0, n1, U0[0], W, W[2], ph.gamma, W[4], U0[0], Wg, Wg[2], ph.gamma, Wg[4]);
+        return;
+    }
+    point_flux(ph, bct, 0, 0, bnd, bc, n0, n1, U0, bnd ? U0 : U1, F);
+}
+
+// The largest theta in [0, theta] for which U + s (F_lo + theta (F - F_lo))
+// stays admissible: rho and p at least min(1e-13, their theta = 0 values);
+// 0 when even theta = 0 is not admissible or the flux is not finite.
+__device__ inline double pp_theta(const Physics& ph, const double* U, double s, const double* F, const double* Flo,
+                                  double theta) {
+    double B0[4], B1[4], w[4];
+    for (int i = 0; i < 4; i++) {
+        B0[i] = U[i] + s * Flo[i];
+        B1[i] = U[i] + s * F[i];
+    }
+    const double p0 = pp_pressure(ph, B0);
+    if (!(B0[0] > 0.0) || !(p0 > 0.0) || !isfinite(B1[0] + B1[1] + B1[2] + B1[3])) return 0.0;
+    const double er = fmin(1.0e-13, B0[0]), ep = fmin(1.0e-13, p0);
+    if (B1[0] < er) theta = fmin(theta, (B0[0] - er) / (B0[0] - B1[0]));
+    for (int i = 0; i < 4; i++) w[i] = B0[i] + theta * (B1[i] - B0[i]);
+    if (pp_pressure(ph, w) < ep) {  // p is concave along the segment: bisect its crossing
+        double lo = 0.0, hi = theta;
+        for (int it = 0; it < 40; it++) {
+            const double mid = 0.5 * (lo + hi);
+            for (int i = 0; i < 4; i++) w[i] = B0[i] + mid * (B1[i] - B0[i]);
+            if (pp_pressure(ph, w) < ep) hi = mid;
+            else lo = mid;
+        }
+        theta = lo;
+    }
+    return theta;
+}
+
+// Limits the face flux F (side 0 to side 1, per unit area, unit normal n)
+// toward F_lo. k0 = S dt a / V0 (side 0 loses a F), k1 = S dt a / V1 (side 1
+// gains it); a boundary face has side 0 only. Side 0 first, then side 1, so
+// both cells of a face compute the same theta.
+__device__ inline void pp_limit(const Physics& ph, double* F, const double* Flo, const double* U0, double k0,
+                                const double* U1, double k1, bool bnd) {
+    double theta = pp_theta(ph, U0, -k0, F, Flo, 1.0);
+    if (!bnd) theta = pp_theta(ph, U1, k1, F, Flo, theta);
+    if (theta < 1.0)
+        for (int i = 0; i < 4; i++) F[i] = theta * F[i] + (1.0 - theta) * Flo[i];
+}
+
 }  // namespace
 }  // namespace mlr::dev

@@ -297,6 +297,9 @@ void Solver::setup(Mesh&& mesh) {
     if (cfg_.integrator != Integrator::FE && cfg_.integrator != Integrator::RK4 &&
         cfg_.integrator != Integrator::SSPRK3)
         fail("Unknown time integrator type.");
+    if (cfg_.positivity && cfg_.integrator == Integrator::RK4)
+        fail("the positivity limiter needs forward Euler or SSP-RK3 (RK4 is no convex combination of Euler steps)");
+    if (cfg_.positivity && cfg_.viscous) fail("the positivity limiter is for the Euler equations");
     L_ = build_layout(mesh_, cfg_.recon, cfg_.teno, cfg_.bcs, cfg_.n_threads, /*allow_device=*/true);
     HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     upload();
@@ -696,6 +699,7 @@ dev::StageArgs Solver::stage(int op, const double* Uin, double* Uout) const {
     a.riemann = (int32_t)cfg_.riemann;
     a.fo = L_.teno ? 0 : 1;
     a.corrected = (L_.teno && L_.tspec.variant == TenoVariant::Corrected) ? 1 : 0;
+    a.pp = cfg_.positivity ? 1 : 0;
     a.Uin = Uin;
     a.Un = Un_;
     a.Uout = Uout;

@@ -56,6 +56,9 @@ struct Config {
     TenoSpec teno;
     Riemann riemann = Riemann::HLLC;
     Integrator integrator = Integrator::SSPRK3;
+    // positivity-preserving flux limiter (Hu, Adams & Shu 2013; no reference
+    // counterpart): forward Euler and SSP-RK3, Euler equations
+    bool positivity = false;
     bool check_nan = false;
     std::vector<BoundarySpec> bcs;
     // [run]

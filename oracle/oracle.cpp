@@ -35,6 +35,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1397,6 +1398,88 @@ struct orc_solver {
     // boundary_wall_adiabatic.cpp:38-70, boundary_upt.cpp:83-100,
     // boundary_p_out.cpp:57-100). The boundary functors use the solver's
     // Euler instance.
+    // Positivity-preserving flux limiter (Hu, Adams & Shu, J. Comput. Phys. 242
+    // (2013) 169-180; no reference counterpart): each cell's forward-Euler
+    // update is the mean over its S faces of the brackets U - S dt a / V F.
+    // With the Rusanov flux of the two cell means (pp_low_flux) every bracket
+    // is admissible while S dt a alpha / V <= 1, so each face's flux is
+    // blended theta F + (1 - theta) F_lo with the largest theta in [0, 1]
+    // keeping both neighbours' brackets at rho, p >= min(1e-13, their
+    // theta = 0 values): the density bound in closed form, the pressure (concave
+    // along the blend) by 40 bisections. Side 0 first, then side 1.
+    void pp_low_flux(int bc_type, const orc_bc* bc, const double* nu, const double* U0, const double* U1,
+                     double* F) const {
+        double Wl[NP], Wr[NP];
+        phys.prims(Wl, U0);
+        double rr = U0[0];
+        if (bc_type < 0) {
+            phys.prims(Wr, U1);
+            rr = U1[0];
+        } else if (bc_type == ORC_BC_SYMMETRY || bc_type == ORC_BC_WALL_ADIABATIC) {
+            for (int i = 0; i < NP; i++) Wr[i] = Wl[i];
+            const double ul[2] = {Wl[0], Wl[1]};
+            const double un = dot2(ul, nu);
+            for (int i = 0; i < ND; i++) Wr[i] = ul[i] - 2.0 * un * nu[i];
+        } else if (bc_type == ORC_BC_EXTRAPOLATION) {
+            phys.prims(Wr, U0);
+        } else if (bc_type == ORC_BC_UPT) {
+            rr = bc->p / (phys.R * bc->T);
+            const double e_bc = phys.cv * bc->T;
+            Wr[1] = bc->u[1];
+            Wr[0] = bc->u[0];
+            Wr[2] = bc->p; Wr[3] = bc->T; Wr[4] = e_bc + bc->p / rr;
+        } else if (bc_type == ORC_BC_P_OUT) {
+            const double ul[2] = {Wl[0], Wl[1]};
+            const double um = norm2_2(ul);
+            const double a = phys.sos(Wl[2], U0[0]);
+            const double pout = (um < a) ? bc->p : Wl[2];
+            const double Tb = Wl[3];
+            rr = pout / (phys.R * Tb);
+            Wr[0] = ul[0]; Wr[1] = ul[1]; Wr[2] = pout; Wr[3] = Tb; Wr[4] = phys.cv * Tb + pout / rr;
+        } else {
+            throw std::runtime_error("Unknown boundary type.");
+        }
+        flux_rusanov(F, nu, U0[0], Wl, Wl[2], phys.gamma, Wl[4], rr, Wr, Wr[2], phys.gamma, Wr[4]);
+    }
+    double pp_theta(const double* U, double s, const double* F, const double* Flo, double theta) const {
+        double B0[NC], B1[NC], w[NC];
+        for (int v = 0; v < NC; v++) {
+            B0[v] = U[v] + s * Flo[v];
+            B1[v] = U[v] + s * F[v];
+        }
+        const double p0 = pressure_raw(B0);
+        if (!(B0[0] > 0.0) || !(p0 > 0.0) || !std::isfinite(B1[0] + B1[1] + B1[2] + B1[3])) return 0.0;
+        const double er = std::fmin(1.0e-13, B0[0]), ep = std::fmin(1.0e-13, p0);
+        if (B1[0] < er) theta = std::fmin(theta, (B0[0] - er) / (B0[0] - B1[0]));
+        for (int v = 0; v < NC; v++) w[v] = B0[v] + theta * (B1[v] - B0[v]);
+        if (pressure_raw(w) < ep) {
+            double lo = 0.0, hi = theta;
+            for (int it = 0; it < 40; it++) {
+                const double mid = 0.5 * (lo + hi);
+                for (int v = 0; v < NC; v++) w[v] = B0[v] + mid * (B1[v] - B0[v]);
+                if (pressure_raw(w) < ep) hi = mid; else lo = mid;
+            }
+            theta = lo;
+        }
+        return theta;
+    }
+    void pp_face(uint32_t f, int bc_type, const orc_bc* bc, const double* nu, double* F) const {
+        const int32_t c0 = m->cof(f, 0), c1 = m->cof(f, 1);
+        const bool bnd = bc_type >= 0;
+        const double* U0 = &Ucur[(size_t)c0 * NC];
+        const double* U1 = bnd ? U0 : &Ucur[(size_t)c1 * NC];
+        double Flo[NC];
+        pp_low_flux(bc_type, bc, nu, U0, U1, Flo);
+        const double a = m->face_area[f];
+        const double Sd = (double)m->n_faces_of_cell((uint32_t)c0);
+        const double V0 = m->cell_volume[c0], V1 = bnd ? V0 : m->cell_volume[c1];
+        const double k0 = Sd * pp_dt * a / V0, k1 = Sd * pp_dt * a / V1;
+        double theta = pp_theta(U0, -k0, F, Flo, 1.0);
+        if (!bnd) theta = pp_theta(U1, k1, F, Flo, theta);
+        if (theta < 1.0)
+            for (int v = 0; v < NC; v++) F[v] = theta * F[v] + (1.0 - theta) * Flo[v];
+        if (theta < 1.0) ++pp_limited;
+    }
     void face_flux(uint32_t f, int bc_type, const orc_bc* bc, double* r) {
         const FluxFn fn = flux_fn(cfg.riemann);
         const double* wq = qface.w.data();
@@ -1465,6 +1548,7 @@ struct orc_solver {
             for (int v = 0; v < NC; v++) F[v] += wq[q] * Ft[v];
         }
         for (int v = 0; v < NC; v++) F[v] *= 0.5;
+        if (pp_dt > 0.0) pp_face(f, bc_type, bc, nu, F);
         const double a = m->face_area[f];
         const int32_t c0 = m->cof(f, 0), c1 = m->cof(f, 1);
         if (canonical) {  // -a * F and a * F differ only in sign
@@ -1480,6 +1564,8 @@ struct orc_solver {
     // solver_rhs.cpp:44-115; parts: bit 0 calc_rhs_interior (:74-108),
     // bit 1 calc_rhs_boundaries (:110-115)
     const double* Ucur = nullptr;  // the state of the residual being evaluated
+    double pp_dt = 0.0;            // > 0: the positivity limiter's stage dt (inside take_step)
+    mutable std::atomic<int64_t> pp_limited{0};  // faces the limiter changed
     void calc_rhs(const double* Ux, double* r, int parts = 3) {
         Ucur = Ux;
         g_vacuum_guard = cfg.recon == ORC_RECON_TENO && cfg.teno_variant == ORC_TENO_CORRECTED;
@@ -1569,8 +1655,10 @@ struct orc_solver {
 
     // time_integrator.cpp:57-163 then solver.cpp:521-531
     void take_step(double dt) {
+        pp_dt = cfg.positivity ? dt : 0.0;  // every stage's forward-Euler substep takes the full dt
         integrate_step(cfg.integrator, dt, nthreads, U, rhs, Utmp,
                        [this](const double* u, double* r) { calc_rhs(u, r); });
+        pp_dt = 0.0;
         update_primitives();
         t += dt;
     }
@@ -1673,10 +1761,25 @@ orc_solver* orc_solver_create(orc_mesh* m, const orc_config* cfg, int n_threads)
     s->m = m;
     s->cfg = *cfg;
     s->nthreads = n_threads < 1 ? 1 : n_threads;
+    if (cfg->positivity && cfg->integrator == ORC_TI_RK4) {
+        g_err = "the positivity limiter needs forward Euler or SSP-RK3";
+        return nullptr;
+    }
     ORC_TRY(s->init(), return nullptr);
     return s.release();
 }
 void orc_solver_destroy(orc_solver* s) { delete s; }
+int64_t orc_pp_limited(const orc_solver* s) { return s->pp_limited.load(); }
+double orc_pp_theta(const orc_solver* s, const double* U, double sc, const double* F, const double* Flo,
+                    double theta) {
+    return s->pp_theta(U, sc, F, Flo, theta);
+}
+int orc_pp_low_flux(const orc_solver* s, int b, const double* nu, const double* U0, const double* U1, double* F) {
+    if (b >= s->cfg.n_bc) return 1;
+    ORC_TRY(s->pp_low_flux(b < 0 ? -1 : s->cfg.bc[b].type, b < 0 ? nullptr : &s->cfg.bc[b], nu, U0, U1, F),
+            return 1);
+    return 0;
+}
 void orc_set_canonical(orc_solver* s, int on) {
     s->canonical = on != 0;
     if (s->canonical) s->build_canonical_order();

@@ -55,6 +55,7 @@ typedef struct {
     int32_t n_bc;
     orc_bc bc[8];
     int32_t teno_variant; /* ORC_TENO_* */
+    int32_t positivity;   /* 1: positivity-preserving flux limiter (FE, SSP-RK3) */
 } orc_config;
 
 /* ---- mesh ---- */
@@ -82,6 +83,14 @@ int64_t orc_mesh_zone(const orc_mesh* m, int zone, uint32_t* faces_out, char* na
 
 /* ---- solver ---- */
 orc_solver* orc_solver_create(orc_mesh* m, const orc_config* cfg, int n_threads);
+/* faces whose flux the positivity limiter changed, summed over the steps so far */
+int64_t orc_pp_limited(const orc_solver* s);
+/* the limiter's pieces, for its tests: the largest theta in [0, theta] that
+ * keeps U + sc (Flo + theta (F - Flo)) admissible, and the low-order flux of
+ * two means across an interior face (b < 0) or of one mean at boundary b */
+double orc_pp_theta(const orc_solver* s, const double* U, double sc, const double* F, const double* Flo,
+                    double theta);
+int orc_pp_low_flux(const orc_solver* s, int b, const double* nu, const double* U0, const double* U1, double* F);
 void orc_solver_destroy(orc_solver* s);
 /* on != 0: the residual's face contributions are gathered per cell in the
    one-thread order instead of added with atomics, so every thread count gives

@@ -35,7 +35,7 @@ class OrcConfig(C.Structure):
                 ("integrator", C.c_int32), ("gamma", C.c_double), ("p_ref", C.c_double),
                 ("T_ref", C.c_double), ("rho_ref", C.c_double), ("p_min", C.c_double),
                 ("p_max", C.c_double), ("n_bc", C.c_int32), ("bc", OrcBC * 8),
-                ("teno_variant", C.c_int32)]
+                ("teno_variant", C.c_int32), ("positivity", C.c_int32)]
 
 
 _lib = None
@@ -64,6 +64,11 @@ def lib():
         L.orc_solver_create.restype = P
         L.orc_solver_create.argtypes = [P, C.POINTER(OrcConfig), C.c_int]
         L.orc_solver_destroy.argtypes = [P]
+        L.orc_pp_limited.restype = C.c_int64
+        L.orc_pp_limited.argtypes = [P]
+        L.orc_pp_theta.restype = C.c_double
+        L.orc_pp_theta.argtypes = [P, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_double]
+        L.orc_pp_low_flux.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_set_canonical.argtypes = [P, C.c_int]
         L.orc_last_error.restype = C.c_char_p
         L.orc_teno_sizes.argtypes = [P, C.c_void_p]
@@ -192,8 +197,9 @@ class Mesh:
 def make_config(recon="TENO", basis="legendre", order=2, factor=2.0, qcell=0, qface=0,
                 riemann="HLLC", integrator="SSPRK3", gamma=1.4, p_ref=101325.0, T_ref=298.15,
                 rho_ref=1.225, p_min=-1e20, p_max=1e20, bcs=None,
-                teno_variant="reference", zones=None):
-    """zones: the mesh's zone names in order (generated meshes: ZONES)."""
+                teno_variant="reference", zones=None, positivity=False):
+    """zones: the mesh's zone names in order (generated meshes: ZONES).
+    positivity: the positivity-preserving flux limiter on every stage."""
     cfg = OrcConfig()
     cfg.recon, cfg.basis, cfg.order = RECON[recon], BASIS[basis], order
     cfg.stencil_factor, cfg.quad_order_cell, cfg.quad_order_face = factor, qcell, qface
@@ -201,6 +207,7 @@ def make_config(recon="TENO", basis="legendre", order=2, factor=2.0, qcell=0, qf
     cfg.gamma, cfg.p_ref, cfg.T_ref, cfg.rho_ref = gamma, p_ref, T_ref, rho_ref
     cfg.p_min, cfg.p_max = p_min, p_max
     cfg.teno_variant = TENO_VARIANT[teno_variant]
+    cfg.positivity = 1 if positivity else 0
     bcs = bcs if bcs is not None else [(z, "symmetry", {}) for z in ("left", "right", "top", "bottom")]
     cfg.n_bc = len(bcs)
     for i, (zone, kind, par) in enumerate(bcs):
@@ -310,6 +317,23 @@ class Solver:
 
     def time(self):
         return lib().orc_time(self.h)
+
+    def pp_limited(self):
+        """Faces whose flux the positivity limiter changed so far."""
+        return int(lib().orc_pp_limited(self.h))
+
+    def pp_theta(self, U, sc, F, Flo, theta=1.0):
+        """The limiter's largest admissible blend factor for one bracket."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (U, F, Flo)]
+        return float(lib().orc_pp_theta(self.h, _p(a[0]), float(sc), _p(a[1]), _p(a[2]), float(theta)))
+
+    def pp_low_flux(self, b, nu, U0, U1=None):
+        """The limiter's low-order flux across an interior face (b < 0) or at boundary b."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (nu, U0, U0 if U1 is None else U1)]
+        F = np.zeros(4)
+        if lib().orc_pp_low_flux(self.h, int(b), _p(a[0]), _p(a[1]), _p(a[2]), _p(F)):
+            raise RuntimeError(lib().orc_last_error().decode())
+        return F
 
     def decisions(self):
         d = np.zeros((self.nc, 4), np.uint8)

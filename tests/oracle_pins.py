@@ -19,7 +19,11 @@ or a restatement that shares no code with the oracle:
     boundary_p_out.cpp:57-100) through residuals with closed forms;
   - the HLL flux (riemann_solver.h:377-439) against Toro's HLL formula with
     the adaptive pressure estimate (Toro, Riemann Solvers and Numerical
-    Methods for Fluid Dynamics, 3rd ed., sections 9.5 and 10.5).
+    Methods for Fluid Dynamics, 3rd ed., sections 9.5 and 10.5);
+  - the positivity-preserving flux limiter (no reference counterpart; Hu,
+    Adams & Shu 2013): its blend factor against the roots of the density
+    (linear) and pressure (quadratic) constraints, and its low-order flux
+    against the Euler flux and the mirror-state Lax-Friedrichs flux.
 """
 import math
 
@@ -526,10 +530,85 @@ def check_hll_toro(orc):
                                        err_msg=str((WL, WR, n)))
 
 
+def _euler_flux(U, n):
+    rho, m0, m1, E = U
+    u, v = m0 / rho, m1 / rho
+    p = (GAMMA - 1.0) * (E - 0.5 * rho * (u * u + v * v))
+    un = u * n[0] + v * n[1]
+    return np.array([rho * un, m0 * un + p * n[0], m1 * un + p * n[1], (E + p) * un]), u, v, p
+
+
+def check_positivity_limiter(orc):
+    """Blend factor theta of one bracket U + s (F_lo + theta (F - F_lo)):
+    1 when the high-order bracket is admissible; at a density violation the
+    root of the linear density, rho = 1e-13; at a pressure violation the
+    largest root of p = 1e-13, a quadratic in theta, approached from below by
+    the bisection; 0 when the low-order bracket is not admissible or the flux
+    is not finite. Low-order flux: the Euler flux for equal states and for an
+    extrapolation boundary, and at symmetry and slip walls the Lax-Friedrichs
+    flux against the mirror state, (0, (rho u_n^2 + p + alpha rho u_n) n, 0)
+    with alpha = |u_n| + c."""
+    bcs = [("left", "symmetry", {}), ("right", "wall_adiabatic", {}), ("top", "extrapolation", {}),
+           ("bottom", "extrapolation", {})]
+    m = orc.Mesh("cartesian_tri", 3, 3, 1.0, 1.0, 0.0, 0)
+    s = orc.Solver(m, orc.make_config(order=2, teno_variant="corrected", bcs=bcs, positivity=True), 1)
+    U = np.array([1.0, 0.0, 0.0, 2.5])  # rho 1, at rest, p 1
+    zero = np.zeros(4)
+    assert s.pp_theta(U, -1.0, np.array([0.1, 0.02, -0.03, 0.2]), zero) == 1.0
+    assert s.pp_theta(U, -1.0, np.array([0.1, 0.02, -0.03, 0.2]), zero, 0.3) == 0.3
+    # density: B(theta) = (1 - 2 theta, 0, 0, 2.5)
+    th = s.pp_theta(U, -1.0, np.array([2.0, 0.0, 0.0, 0.0]), zero)
+    assert abs(th - (1.0 - 1e-13) / 2.0) <= 1e-16, th
+    # pressure, linear: p(B(theta)) = 0.4 (2.5 - 5 theta)
+    th = s.pp_theta(U, -1.0, np.array([0.0, 0.0, 0.0, 5.0]), zero)
+    root = (2.5 - 1e-13 / 0.4) / 5.0
+    assert 0.0 <= root - th <= 2e-12, (th, root)
+    # pressure, quadratic: B(theta) = (1, -3 theta, 0, 2.5), p = 0.4 (2.5 - 4.5 theta^2)
+    th = s.pp_theta(U, -1.0, np.array([0.0, 3.0, 0.0, 0.0]), zero)
+    root = math.sqrt((2.5 - 1e-13 / 0.4) / 4.5)
+    assert 0.0 <= root - th <= 2e-12, (th, root)
+    # a mass flux that leaves rho > 0 moves the pressure root: p rho is quadratic in theta
+    F = np.array([0.9, 3.0, 0.0, 0.0])
+    th = s.pp_theta(U, -1.0, F, zero)
+    r_rho = (1.0 - 1e-13) / 0.9
+    # p(theta) rho(theta) / 0.4 = 2.5 (1 - 0.9 theta) - 4.5 theta^2 = 1e-13 (1 - 0.9 theta) / 0.4
+    a, b, c = -4.5, -2.25 + 0.9e-13 / 0.4, 2.5 - 1e-13 / 0.4
+    r_p = (-b - math.sqrt(b * b - 4 * a * c)) / (2 * a)
+    assert r_p < r_rho and 0.0 <= r_p - th <= 2e-12, (th, r_p)
+    assert s.pp_theta(U, -1.0, F, np.array([2.0, 0.0, 0.0, 0.0])) == 0.0  # low-order bracket rho < 0
+    assert s.pp_theta(U, -1.0, np.array([0.0, np.nan, 0.0, 0.0]), zero) == 0.0
+    assert s.pp_theta(U, -1.0, np.array([0.0, np.inf, 0.0, 0.0]), zero) == 0.0
+    # low-order fluxes
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        rho, u, v, p = rng.uniform(0.2, 2.0), rng.uniform(-1.5, 1.5), rng.uniform(-1.5, 1.5), rng.uniform(0.1, 2.0)
+        Um = np.array([rho, rho * u, rho * v, p / (GAMMA - 1.0) + 0.5 * rho * (u * u + v * v)])
+        ang = rng.uniform(0, 2 * math.pi)
+        n = np.array([math.cos(ang), math.sin(ang)])
+        Fe, _, _, _ = _euler_flux(Um, n)
+        np.testing.assert_allclose(s.pp_low_flux(-1, n, Um, Um), Fe, rtol=0, atol=1e-13 * np.abs(Fe).max())
+        r2, u2, v2, p2 = rng.uniform(0.2, 2.0), rng.uniform(-1.5, 1.5), rng.uniform(-1.5, 1.5), rng.uniform(0.1, 2.0)
+        U2 = np.array([r2, r2 * u2, r2 * v2, p2 / (GAMMA - 1.0) + 0.5 * r2 * (u2 * u2 + v2 * v2)])
+        F2, _, _, _ = _euler_flux(U2, n)
+        a2 = max(abs(u * n[0] + v * n[1]) + math.sqrt(GAMMA * p / rho),
+                 abs(u2 * n[0] + v2 * n[1]) + math.sqrt(GAMMA * p2 / r2))
+        lf = 0.5 * (Fe + F2) - 0.5 * a2 * (U2 - Um)
+        np.testing.assert_allclose(s.pp_low_flux(-1, n, Um, U2), lf, rtol=0, atol=1e-13 * np.abs(lf).max())
+        np.testing.assert_allclose(s.pp_low_flux(2, n, Um), Fe, rtol=0, atol=1e-13 * np.abs(Fe).max())
+        un = u * n[0] + v * n[1]
+        alpha = abs(un) + math.sqrt(GAMMA * p / rho)
+        mom = rho * un * un + p + alpha * rho * un
+        exp = np.array([0.0, mom * n[0], mom * n[1], 0.0])
+        for b in (0, 1):
+            got = s.pp_low_flux(b, n, Um)
+            np.testing.assert_allclose(got, exp, rtol=0, atol=1e-13 * max(1.0, abs(mom)), err_msg=str((b, got, exp)))
+
+
 CHECKS = {
     "dt": check_dt_closed_form,
     "eno": check_eno_branch,
     "directional": check_directional_stencils,
     "boundary": check_boundary_fluxes,
     "hll": check_hll_toro,
+    "positivity": check_positivity_limiter,
 }

@@ -60,6 +60,13 @@ MUTANTS = [
      "F[i] = (Sr * s.Fl[i] - Sl * s.Fr[i] + Sl * Sr * (s.Ul[i] - s.Ur[i])) / (Sr - Sl);"),
     ("hll", "two-shock estimate with the velocity jump reversed",
      "ps = (gl0 * pl + gr0 * pr - (ur - ul)) / (gl0 + gr0);", "ps = (gl0 * pl + gr0 * pr - (ul - ur)) / (gl0 + gr0);"),
+    ("positivity", "density bound without the 1e-13 margin",
+     "theta = std::fmin(theta, (B0[0] - er) / (B0[0] - B1[0]));", "theta = std::fmin(theta, B0[0] / (B0[0] - B1[0]));"),
+    ("positivity", "pressure bisection returns its failing end", "            theta = lo;\n", "            theta = hi;\n"),
+    ("positivity", "interior low-order flux from side 0 alone", "            phys.prims(Wr, U1);\n            rr = U1[0];",
+     "            phys.prims(Wr, U0);\n            rr = U0[0];"),
+    ("positivity", "low-order bracket not checked", "if (!(B0[0] > 0.0) || !(p0 > 0.0) || !std::isfinite",
+     "if (false || !std::isfinite"),
     ("hll", "linearised estimate with the velocity jump reversed",
      "ps = 0.5 * (pl + pr) + 0.5 * (ul - ur) * ravg * aavg;", "ps = 0.5 * (pl + pr) + 0.5 * (ur - ul) * ravg * aavg;"),
 ]
