@@ -18,7 +18,9 @@ reported as ms_per_step_graph. The plain and learned runs use the exact
 TENO tables (bit-identical to the oracle). Afterwards the same steps are
 timed on compact tables (plain_compact_tables: member areas folded into the
 pseudo-inverses, face basis per reference-triangle edge), with their largest
-relative difference from the exact tables' run (--exact-tables skips it).
+relative difference from the exact tables' run (--exact-tables skips it),
+and on a second solver with symmetry walls on the four sides and the
+positivity-preserving flux limiter (symmetry_walls; --no-walls skips it).
 
 cpu_baseline: the CPU oracle (the restatement of the reference's OpenMP path:
 AoS state, per-face atomics) timed on this host's cores on bounded samples of
@@ -66,7 +68,8 @@ def log(*a):
 def numerics_kw(args):
     bcs = [(z, args.bc, {}) for z in ("left", "right", "top", "bottom")]
     return dict(recon="TENO", basis="legendre", order=args.order, stencil_factor=2.0,
-                teno_variant=args.variant, riemann="HLLC", integrator=args.integrator, cfl=0.4, bcs=bcs)
+                teno_variant=args.variant, riemann="HLLC", integrator=args.integrator, cfl=0.4, bcs=bcs,
+                positivity=args.positivity)
 
 
 def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes, compact=False):
@@ -295,7 +298,7 @@ def ensemble_run(args, mlr):
              "tune": list(tune)}
 
     persistent = None
-    if not args.no_persistent:
+    if not args.no_persistent and not args.positivity:
         # every step as one cooperative launch over all cases; states must equal the graph run's bit for bit
         ens.set_persistent(True)
         init()
@@ -365,7 +368,8 @@ def ensemble_run(args, mlr):
             "workload": f"batched ensemble: {len(sizes)} four-quadrant Riemann problems, random states "
                         f"(rho, p ~ U[0.1, 2], u, v ~ U[-1.5, 1.5], seed {args.seed}) on jittered triangle "
                         f"meshes of {sizes.min()}^2 .. {sizes.max()}^2 quads (log-uniform draw, seed "
-                        f"{args.ens_size_seed}), TENO p={args.order} + HLLC + SSPRK3, CFL 0.4, one launch "
+                        f"{args.ens_size_seed}), TENO p={args.order} + HLLC + SSPRK3, CFL 0.4"
+                        f"{', positivity limiter' if args.positivity else ''}, one launch "
                         f"per kernel phase, hipGraph per step (the persistent record: one cooperative "
                         f"launch per step)",
             "n_cells": N,
@@ -447,6 +451,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-indicator run")
     ap.add_argument("--exact-tables", action="store_true",
                     help="skip the extra timing on compact TENO tables")
+    ap.add_argument("--positivity", action="store_true",
+                    help="the positivity-preserving flux limiter in the main (or ensemble) run")
+    ap.add_argument("--no-walls", action="store_true",
+                    help="skip the extra timing with symmetry walls and the positivity limiter")
     ap.add_argument("--train-cases", type=int, default=512)
     ap.add_argument("--threshold", type=float, default=0.9)
     ap.add_argument("--gate-split", type=int, default=1, choices=[0, 1],
@@ -570,6 +578,37 @@ def main():
         log(f"[bench] compact tables: {compact['ms_per_step']:.2f} ms per step (TENO {teno_c:.3f} ms), relative "
             f"L-inf against the exact tables after {args.steps} steps {dev_c:.3g}")
 
+    walls = None
+    if not args.no_walls and args.bc == "extrapolation" and args.integrator == "SSPRK3":
+        # the same mesh and data with symmetry walls on all four sides, which
+        # reach non-finite states by step 40 without the positivity limiter
+        kw = dict(numerics_kw(args), bcs=[(z, "symmetry", {}) for z in ("left", "right", "top", "bottom")],
+                  positivity=True)
+        ws = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", args.nx, args.nx, 1.0, 1.0, args.jitter, args.seed),
+                        num=mlr.numerics(**kw))
+        ws.init_quadrant(0.5, 0.5, QUADRANT)
+        ws.autotune()
+        if args.warmup > 0:
+            ws.run_steps(args.warmup)
+        ws.init_quadrant(0.5, 0.5, QUADRANT)
+        t = time.perf_counter()
+        stw = ws.run_steps_profiled(args.steps)
+        wall_w = time.perf_counter() - t
+        Uw, Ww = ws.state()
+        ok = bool(np.isfinite(Uw).all() and (Uw[:, 0] > 0).all() and (Ww[:, 2] > 0).all())
+        ws.init_quadrant(0.5, 0.5, QUADRANT)
+        t = time.perf_counter()
+        ws.run_steps(args.steps)
+        wall_wg = time.perf_counter() - t
+        walls = {"value": N * stages * args.steps / wall_w, "ms_per_step": wall_w / args.steps * 1e3,
+                 "ms_per_step_graph": wall_wg / args.steps * 1e3,
+                 "flux_ms": stw["flux_ms"] / max(stw["flux_n"], 1), "teno_ms": stw["teno_ms"] / max(stw["teno_n"], 1),
+                 "physical_after_steps": ok, "min_rho": float(np.nanmin(Uw[:, 0])), "min_p": float(np.nanmin(Ww[:, 2])),
+                 "boundaries": "symmetry on all four sides", "positivity_limiter": True}
+        ws.close()
+        log(f"[bench] symmetry walls + positivity limiter: {walls['ms_per_step']:.2f} ms per step (flux "
+            f"{walls['flux_ms']:.3f} ms against {flux_ms:.3f} ms unlimited), physical after {args.steps} steps: {ok}")
+
     matched = learned is not None and learned["matched_l1"]
     value = learned["value"] if matched else value_plain
     out = {
@@ -598,6 +637,7 @@ def main():
                   "tables": "exact (bit-identical to the oracle)",
                   "whole_step": step_roofline(alg, N, stages * args.steps, wall)},
         "plain_compact_tables": compact,
+        "symmetry_walls": walls,
         "learned": learned,
         "roofline": {
             "kernel": "teno_kernel (plain FP64 TENO run)",

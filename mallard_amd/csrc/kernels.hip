@@ -740,10 +740,8 @@ __device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, con
             const double* U0 = side ? mo.v : um.v;
             const double* U1 = side ? um.v : mo.v;
             const double V0 = side ? vo : vc, V1 = side ? vc : vo;
-            double Flo[4];
-            pp_low_flux(ph, g.bc, bnd, (inf >> 4) & 0xFu, n0, n1, U0, U1, Flo);
             const double Sd = (double)g.S;
-            pp_limit(ph, F, Flo, U0, Sd * pdt * ar / V0, U1, Sd * pdt * ar / V1, bnd);
+            pp_limit(ph, g.bc, bnd, (inf >> 4) & 0xFu, n0, n1, F, U0, Sd * pdt * ar / V0, U1, Sd * pdt * ar / V1);
         }
         if (ph.viscous) {
             double G[4];

@@ -282,12 +282,24 @@ __device__ inline double pp_theta(const Physics& ph, const double* U, double s, 
     return theta;
 }
 
+// Whether U + s F is finite with rho and p at least 1e-13.
+__device__ inline bool pp_admissible(const Physics& ph, const double* U, double s, const double* F) {
+    double B[4];
+    for (int i = 0; i < 4; i++) B[i] = U[i] + s * F[i];
+    return isfinite(B[0] + B[1] + B[2] + B[3]) && B[0] >= 1.0e-13 && pp_pressure(ph, B) >= 1.0e-13;
+}
+
 // Limits the face flux F (side 0 to side 1, per unit area, unit normal n)
-// toward F_lo. k0 = S dt a / V0 (side 0 loses a F), k1 = S dt a / V1 (side 1
-// gains it); a boundary face has side 0 only. Side 0 first, then side 1, so
-// both cells of a face compute the same theta.
-__device__ inline void pp_limit(const Physics& ph, double* F, const double* Flo, const double* U0, double k0,
-                                const double* U1, double k1, bool bnd) {
+// toward the low-order flux. k0 = S dt a / V0 (side 0 loses a F), k1 =
+// S dt a / V1 (side 1 gains it); a boundary face (boundary condition bc) has
+// side 0 only. F stands when both high-order brackets are admissible;
+// otherwise theta is found side 0 first, then side 1, so both cells of a
+// face compute the same value.
+__device__ inline void pp_limit(const Physics& ph, const BCParam* __restrict__ bct, bool bnd, int bc, double n0,
+                                double n1, double* F, const double* U0, double k0, const double* U1, double k1) {
+    if (pp_admissible(ph, U0, -k0, F) && (bnd || pp_admissible(ph, U1, k1, F))) return;
+    double Flo[4];
+    pp_low_flux(ph, bct, bnd, bc, n0, n1, U0, U1, Flo);
     double theta = pp_theta(ph, U0, -k0, F, Flo, 1.0);
     if (!bnd) theta = pp_theta(ph, U1, k1, F, Flo, theta);
     if (theta < 1.0)

@@ -1406,7 +1406,9 @@ struct orc_solver {
     // blended theta F + (1 - theta) F_lo with the largest theta in [0, 1]
     // keeping both neighbours' brackets at rho, p >= min(1e-13, their
     // theta = 0 values): the density bound in closed form, the pressure (concave
-    // along the blend) by 40 bisections. Side 0 first, then side 1.
+    // along the blend) by 40 bisections. Side 0 first, then side 1. A face
+    // whose high-order brackets are both admissible (finite, rho and p at
+    // least 1e-13) keeps its flux without the low-order one being formed.
     void pp_low_flux(int bc_type, const orc_bc* bc, const double* nu, const double* U0, const double* U1,
                      double* F) const {
         double Wl[NP], Wr[NP];
@@ -1463,17 +1465,25 @@ struct orc_solver {
         }
         return theta;
     }
+    // the high-order bracket U + s F finite with rho and p at least 1e-13
+    bool pp_admissible(const double* U, double s, const double* F) const {
+        double B[NC];
+        for (int v = 0; v < NC; v++) B[v] = U[v] + s * F[v];
+        return std::isfinite(B[0] + B[1] + B[2] + B[3]) && B[0] >= 1.0e-13 && pressure_raw(B) >= 1.0e-13;
+    }
     void pp_face(uint32_t f, int bc_type, const orc_bc* bc, const double* nu, double* F) const {
         const int32_t c0 = m->cof(f, 0), c1 = m->cof(f, 1);
         const bool bnd = bc_type >= 0;
         const double* U0 = &Ucur[(size_t)c0 * NC];
         const double* U1 = bnd ? U0 : &Ucur[(size_t)c1 * NC];
-        double Flo[NC];
-        pp_low_flux(bc_type, bc, nu, U0, U1, Flo);
         const double a = m->face_area[f];
         const double Sd = (double)m->n_faces_of_cell((uint32_t)c0);
         const double V0 = m->cell_volume[c0], V1 = bnd ? V0 : m->cell_volume[c1];
         const double k0 = Sd * pp_dt * a / V0, k1 = Sd * pp_dt * a / V1;
+        // F stands when both high-order brackets are admissible
+        if (pp_admissible(U0, -k0, F) && (bnd || pp_admissible(U1, k1, F))) return;
+        double Flo[NC];
+        pp_low_flux(bc_type, bc, nu, U0, U1, Flo);
         double theta = pp_theta(U0, -k0, F, Flo, 1.0);
         if (!bnd) theta = pp_theta(U1, k1, F, Flo, theta);
         if (theta < 1.0)
