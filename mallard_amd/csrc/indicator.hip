@@ -199,22 +199,30 @@ __global__ void adam_kernel(float* __restrict__ theta, float* __restrict__ m, fl
 }
 
 // master FP32 parameters -> MFMA operand layouts (bf16) and FP32 copies
-__global__ void pack_kernel(const float* __restrict__ theta, v4s* __restrict__ W1p, v4s* __restrict__ W2p,
+__global__ void pack_kernel(const float* __restrict__ theta, v8s* __restrict__ W1p, v8s* __restrict__ W2p,
                             float* __restrict__ b1, float* __restrict__ b2, float* __restrict__ w3,
                             float* __restrict__ b3, float* __restrict__ W2f) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < kW1pVec) {  // W1p[(nt*3+kc)*64 + l][j] = W1[kc*16 + 4g + j][nt*16 + l%16]
-        const int l = e & 63, blk = e >> 6, nt = blk / 3, kc = blk % 3, gg = l >> 4;
-        v4s x;
-        for (int j = 0; j < 4; ++j)
-            x[j] = (short)f32_to_bf16(theta[kOffW1 + (kc * 16 + 4 * gg + j) * kHid + nt * 16 + (l & 15)]);
+    // element j of lane l's operand for product block kk pairs with the
+    // input row 32 kk + 4g + j (j < 4) or 32 kk + 16 + 4g + j - 4 (j >= 4),
+    // g = l / 16, which is where mlp_wave puts those inputs; rows 48..63 of
+    // the first layer are zero padding
+    if (e < kW1pVec) {  // W1p[(nt*2+kk)*64 + l][j] = W1[row][nt*16 + l%16]
+        const int l = e & 63, blk = e >> 6, nt = blk >> 1, kk = blk & 1, gg = l >> 4;
+        v8s x;
+        for (int j = 0; j < 8; ++j) {
+            const int row = 32 * kk + (j < 4 ? 4 * gg + j : 16 + 4 * gg + j - 4);
+            x[j] = row < kFeat ? (short)f32_to_bf16(theta[kOffW1 + row * kHid + nt * 16 + (l & 15)]) : (short)0;
+        }
         W1p[e] = x;
     }
     if (e < kW2pVec) {
-        const int l = e & 63, blk = e >> 6, nt = blk / 4, kc = blk % 4, gg = l >> 4;
-        v4s x;
-        for (int j = 0; j < 4; ++j)
-            x[j] = (short)f32_to_bf16(theta[kOffW2 + (kc * 16 + 4 * gg + j) * kHid + nt * 16 + (l & 15)]);
+        const int l = e & 63, blk = e >> 6, nt = blk >> 1, kk = blk & 1, gg = l >> 4;
+        v8s x;
+        for (int j = 0; j < 8; ++j) {
+            const int row = 32 * kk + (j < 4 ? 4 * gg + j : 16 + 4 * gg + j - 4);
+            x[j] = (short)f32_to_bf16(theta[kOffW2 + row * kHid + nt * 16 + (l & 15)]);
+        }
         W2p[e] = x;
     }
     if (e < kHid * kHid) W2f[e] = bf16_to_f32(f32_to_bf16(theta[kOffW2 + e]));
@@ -244,7 +252,7 @@ __global__ void batch_kernel(const uint32_t* __restrict__ pool, uint64_t n_pool,
 }  // namespace
 
 size_t net_bytes() {
-    return (size_t)(kW1pVec + kW2pVec) * sizeof(v4s) + (3 * kHid + 1) * sizeof(float) +
+    return (size_t)(kW1pVec + kW2pVec) * sizeof(v8s) + (3 * kHid + 1) * sizeof(float) +
            (size_t)kHid * kHid * sizeof(float);
 }
 
@@ -252,9 +260,9 @@ ind::Net net_view(void* buf, double threshold, const float** W2f) {
     char* p = static_cast<char*>(buf);
     ind::Net n{};
     n.W1p = p;
-    p += kW1pVec * sizeof(v4s);
+    p += kW1pVec * sizeof(v8s);
     n.W2p = p;
-    p += kW2pVec * sizeof(v4s);
+    p += kW2pVec * sizeof(v8s);
     n.b1 = reinterpret_cast<const float*>(p);
     n.b2 = n.b1 + kHid;
     n.w3 = n.b2 + kHid;
@@ -268,7 +276,7 @@ hipError_t net_pack(const float* theta, void* buf, hipStream_t stream) {
     const float* W2f = nullptr;
     const ind::Net n = net_view(buf, 0.5, &W2f);
     hipLaunchKernelGGL(pack_kernel, dim3((kHid * kHid + 255) / 256), dim3(256), 0, stream, theta,
-                       static_cast<v4s*>(const_cast<void*>(n.W1p)), static_cast<v4s*>(const_cast<void*>(n.W2p)),
+                       static_cast<v8s*>(const_cast<void*>(n.W1p)), static_cast<v8s*>(const_cast<void*>(n.W2p)),
                        const_cast<float*>(n.b1),
                        const_cast<float*>(n.b2), const_cast<float*>(n.w3), const_cast<float*>(n.b3),
                        const_cast<float*>(W2f));

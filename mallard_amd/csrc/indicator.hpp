@@ -1,6 +1,7 @@
 // Learned troubled-cell indicator: per-cell stencil features and a
 // 48-64-64-1 MLP (GELU, GELU, sigmoid) with BF16 weights and activations and
-// FP32 accumulation on the CDNA matrix cores (v_mfma_f32_16x16x16_bf16).
+// FP32 accumulation on the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16,
+// 32 products per lane group and output).
 //
 // Features of cell c, from the 12 members of its central stencil (target
 // first). For members m = 1..11 the raw differences, in FP64,
@@ -16,8 +17,13 @@
 //
 // MFMA mapping (one wave = 16 cells, MFMA column n = lane % 16, row group
 // g = lane / 16): every layer is computed transposed, D = W^T X^T, so the
-// accumulator a lane receives (rows 4g..4g+3 of its column) is already the
-// B operand layout of the next layer.
+// accumulator a lane receives (rows 4g..4g+3 of its column) is already half
+// of a B operand of the next layer. A 16x16x32 operand takes 8 products per
+// lane: the two 4-element groups a lane already holds (features of members
+// g and 4 + g, then of 8 + g and zeros; hidden units 32 kk + 4g + r and
+// 32 kk + 16 + 4g + r) side by side, and the weights are packed in that
+// order. Both operands pair element e of lane group g with the same product
+// index, so the order within the 32 only permutes the sum.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,13 +43,16 @@ constexpr int kOffB2 = kOffW2 + kHid * kHid;
 constexpr int kOffW3 = kOffB2 + kHid;
 constexpr int kOffB3 = kOffW3 + kHid;
 static_assert(kOffB3 + 1 == kNParams, "parameter layout");
-// packed MFMA A operands: W1p[nt 4][kc 3][lane 64], W2p[nt 4][kc 4][lane 64], 4 bf16 each
-constexpr int kW1pVec = 4 * 3 * 64, kW2pVec = 4 * 4 * 64;
+// packed MFMA A operands: W1p[nt 4][kk 2][lane 64], W2p[nt 4][kk 2][lane 64], 8 bf16 each
+constexpr int kW1pVec = 4 * 2 * 64, kW2pVec = 4 * 2 * 64;
+constexpr int kOperandBytes = 16;
 // the inference part of the packed network (W1p, W2p, b1, b2, w3, b3 from
 // the buffer's start), padded to 16 bytes: what a block stages in LDS
-constexpr int kNetInferBytes = ((kW1pVec + kW2pVec) * 8 + (3 * kHid + 1) * 4 + 15) / 16 * 16;
+constexpr int kNetInferBytes = ((kW1pVec + kW2pVec) * kOperandBytes + (3 * kHid + 1) * 4 + 15) / 16 * 16;
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
@@ -175,12 +184,19 @@ __device__ inline void stage_net(const Net& net, unsigned char* lds) {
 __device__ inline Net lds_net(const Net& net, const unsigned char* lds) {
     Net n = net;
     n.W1p = lds;
-    n.W2p = lds + kW1pVec * 8;
-    n.b1 = reinterpret_cast<const float*>(lds + (kW1pVec + kW2pVec) * 8);
+    n.W2p = lds + kW1pVec * kOperandBytes;
+    n.b1 = reinterpret_cast<const float*>(lds + (kW1pVec + kW2pVec) * kOperandBytes);
     n.b2 = n.b1 + kHid;
     n.w3 = n.b2 + kHid;
     n.b3 = n.w3 + kHid;
     return n;
+}
+
+// the 16x16x32 BF16 product with its two 4-element halves given apart
+__device__ inline v4f mfma32(v8s a, v4s b_lo, v4s b_hi, v4f acc) {
+    const v8s b = {b_lo[0], b_lo[1], b_lo[2], b_lo[3], b_hi[0], b_hi[1], b_hi[2], b_hi[3]};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), acc,
+                                                  0, 0, 0);
 }
 
 // Forward pass of one wave. xb[kc] holds, for cell lane % 16, the features
@@ -200,10 +216,9 @@ __device__ inline float mlp_wave(const Net& net, const v4s* xb, int lane, float*
     for (int nt = 0; nt < 4; ++nt) {
         const float4 bb = B1[nt * 4 + g];
         v4f acc = {bb.x, bb.y, bb.z, bb.w};
-#pragma unroll
-        for (int kc = 0; kc < 3; ++kc)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(static_cast<const v4s*>(net.W1p)[(nt * 3 + kc) * 64 + lane],
-                                                           xb[kc], acc, 0, 0, 0);
+        const v8s* W1 = static_cast<const v8s*>(net.W1p);
+        acc = mfma32(W1[(nt * 2 + 0) * 64 + lane], xb[0], xb[1], acc);
+        acc = mfma32(W1[(nt * 2 + 1) * 64 + lane], xb[2], v4s{0, 0, 0, 0}, acc);
         const v2u w = {bf16x2(gelu(acc[0]), gelu(acc[1])), bf16x2(gelu(acc[2]), gelu(acc[3]))};
         h1[nt] = __builtin_bit_cast(v4s, w);
         if (pre1) {
@@ -220,10 +235,9 @@ __device__ inline float mlp_wave(const Net& net, const v4s* xb, int lane, float*
     for (int nt = 0; nt < 4; ++nt) {
         const float4 bb = B2[nt * 4 + g];
         v4f acc = {bb.x, bb.y, bb.z, bb.w};
-#pragma unroll
-        for (int kc = 0; kc < 4; ++kc)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(static_cast<const v4s*>(net.W2p)[(nt * 4 + kc) * 64 + lane],
-                                                           h1[kc], acc, 0, 0, 0);
+        const v8s* W2 = static_cast<const v8s*>(net.W2p);
+        acc = mfma32(W2[(nt * 2 + 0) * 64 + lane], h1[0], h1[1], acc);
+        acc = mfma32(W2[(nt * 2 + 1) * 64 + lane], h1[2], h1[3], acc);
         const float4 ww = W3[nt * 4 + g];
         if (pre2)
 #pragma unroll
