@@ -716,8 +716,12 @@ __device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, con
                 const uint32_t nbc = (uint32_t)g.nb[o];
                 const uint32_t nbs = inf & 3u;
                 oth = a.fo ? ld4(a.Uin, nbc) : ld4(a.fs, (size_t)(nbs * Q + q) * N + nbc);
-                const double* L = side ? oth.v : own.v;
-                const double* R = side ? own.v : oth.v;
+                double L[4], R[4];  // selected by value (a selected pointer can put them in scratch)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    L[i] = side ? oth.v[i] : own.v[i];
+                    R[i] = side ? own.v[i] : oth.v[i];
+                }
                 point_flux(ph, g.bc, RS, a.corrected, false, 0, n0, n1, L, R, Ft);
             } else {
                 // the ghost state an extrapolation boundary copies: the face
@@ -737,8 +741,12 @@ __device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, con
             const uint32_t nbc = bnd ? c : (uint32_t)g.nb[o];
             const D4 mo = bnd ? um : ld4(a.Uin, nbc);
             const double vo = bnd ? vc : g.volume[nbc];
-            const double* U0 = side ? mo.v : um.v;
-            const double* U1 = side ? um.v : mo.v;
+            double U0[4], U1[4];  // selected by value: a selected pointer would put the means in scratch
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                U0[i] = side ? mo.v[i] : um.v[i];
+                U1[i] = side ? um.v[i] : mo.v[i];
+            }
             const double V0 = side ? vo : vc, V1 = side ? vc : vo;
             const double Sd = (double)g.S;
             pp_limit(ph, g.bc, bnd, (inf >> 4) & 0xFu, n0, n1, F, U0, Sd * pdt * ar / V0, U1, Sd * pdt * ar / V1);
@@ -828,7 +836,8 @@ __device__ __forceinline__ void flux_cell(const Geom& g, const StageArgs& a, con
 }
 
 template <int RS, bool PP>
-__global__ __launch_bounds__(256) void flux_update_kernel(Geom g, StageArgs a, Physics ph, uint32_t blocks_per_xcd) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PP ? 4 : 1, 8)))
+void flux_update_kernel(Geom g, StageArgs a, Physics ph, uint32_t blocks_per_xcd) {
     const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
     flux_cell<RS, 3, PP>(g, a, ph, lb * blockDim.x + threadIdx.x);
 }

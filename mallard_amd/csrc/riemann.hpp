@@ -251,7 +251,8 @@ __device__ inline void pp_low_flux(const Physics& ph, const BCParam* __restrict_
         riemann_flux(0, 0, F, n0, n1, U0[0], W, W[2], ph.gamma, W[4], U0[0], Wg, Wg[2], ph.gamma, Wg[4]);
         return;
     }
-    point_flux(ph, bct, 0, 0, bnd, bc, n0, n1, U0, bnd ? U0 : U1, F);
+    if (bnd) point_flux(ph, bct, 0, 0, true, bc, n0, n1, U0, U0, F);  // two calls: a selected pointer
+    else point_flux(ph, bct, 0, 0, false, 0, n0, n1, U0, U1, F);        // would put the means in scratch
 }
 
 // The largest theta in [0, theta] for which U + s (F_lo + theta (F - F_lo))
