@@ -546,6 +546,13 @@ def main():
     nb = 2 << 30
     stream_copy = 2 * nb / (mlr.probe_stream(0, nb, 10) * 1e-3) / 1e9
     stream_read = nb / (mlr.probe_stream(1, nb, 10) * 1e-3) / 1e9
+    # the same ceiling by other methods: the runtime's device-to-device copy,
+    # 64 contiguous bytes per lane, non-temporal loads
+    memcpy_d2d = 2 * nb / (mlr.probe_stream(2, nb, 10) * 1e-3) / 1e9
+    read_wide = nb / (mlr.probe_stream(3, nb, 10) * 1e-3) / 1e9
+    read_nt = nb / (mlr.probe_stream(4, nb, 10) * 1e-3) / 1e9
+    log(f"[bench] HBM probes (GB/s): copy {stream_copy:.0f}, read {stream_read:.0f}, hipMemcpy D2D "
+        f"{memcpy_d2d:.0f}, read 64 B/lane {read_wide:.0f}, non-temporal read {read_nt:.0f}")
 
     value_plain = N * stages * args.steps / wall
     learned = None
@@ -653,6 +660,9 @@ def main():
             "avg_launch_ms": teno_ms,
             "stream_copy_GBs": stream_copy,
             "stream_read_GBs": stream_read,
+            "memcpy_d2d_GBs": memcpy_d2d,
+            "stream_read_wide_GBs": read_wide,
+            "stream_read_nontemporal_GBs": read_nt,
         },
         "roofline_learned": learned_roofline(learned, N, dev),
         "kernels_ms": {"teno": teno_ms, "flux_update": flux_ms,

@@ -442,7 +442,9 @@ int mlr_get_volumes(const mlr_ctx* ctx, double* vol);
 int mlr_probe_math(int which, const double* x, const double* y, double* out, int n);
 /* Streaming bandwidth probe (no reference counterpart; context for the
  * roofline figures): kind 0 copies `bytes`, kind 1 reads them, with 16-byte
- * loads per lane over a grid-stride loop; ms_avg = mean duration of `reps`
+ * loads per lane over a grid-stride loop; kind 2 is hipMemcpyAsync device to
+ * device; kind 3 reads 64 contiguous bytes per lane per iteration; kind 4
+ * reads with 16-byte non-temporal loads. ms_avg = mean duration of `reps`
  * launches after one warm-up, by HIP events. */
 int mlr_probe_stream(int kind, int64_t bytes, int reps, double* ms_avg);
 /* Host builds of the same double-double pow and correctly rounded x^6. */

@@ -787,7 +787,9 @@ def integrator_take_step(integrator, dt, n, solution_ptrs, rhs_ptrs, rhs_fn, sol
 
 
 def probe_stream(kind, nbytes, reps=10):
-    """Mean ms of a streaming copy (kind 0) or read (kind 1) of nbytes on the GPU."""
+    """Mean ms of a streaming copy (kind 0) or read (kind 1) of nbytes on the
+    GPU; kind 2 hipMemcpyAsync device to device, kind 3 a read of 64
+    contiguous bytes per lane, kind 4 a read with non-temporal loads."""
     ms = C.c_double()
     if lib().mlr_probe_stream(kind, int(nbytes), reps, C.byref(ms)):
         raise MallardError(lib().mlr_last_error(None).decode())

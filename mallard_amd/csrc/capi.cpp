@@ -718,7 +718,7 @@ int mlr_probe_math(int which, const double* x, const double* y, double* out, int
 }
 
 int mlr_probe_stream(int kind, int64_t bytes, int reps, double* ms_avg) {
-    if (!ms_avg || bytes < 4096 || reps < 1 || (kind != 0 && kind != 1)) return 1;
+    if (!ms_avg || bytes < 4096 || reps < 1 || kind < 0 || kind > 4) return 1;
     return guarded(nullptr, [&] {
         struct Buf {
             void* p = nullptr;
@@ -730,7 +730,7 @@ int mlr_probe_stream(int kind, int64_t bytes, int reps, double* ms_avg) {
         } e0, e1;
         auto chk = [](hipError_t e) { if (e != hipSuccess) mlr::fail(hipGetErrorString(e)); };
         const size_t n = (size_t)bytes / 16 * 16;
-        const size_t dbytes = kind == 0 ? n : (size_t)mlr::dev::stream_probe_threads(n) * 8;
+        const size_t dbytes = (kind == 0 || kind == 2) ? n : (size_t)mlr::dev::stream_probe_threads(n) * 8;
         chk(hipMalloc(&src.p, n));
         chk(hipMalloc(&dst.p, dbytes));
         chk(hipMemset(src.p, 0, n));

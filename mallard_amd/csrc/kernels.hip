@@ -1170,6 +1170,41 @@ __global__ __launch_bounds__(256) void stream_read_kernel(const double2* __restr
     out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = acc0 + acc1;
 }
 
+// Two more read patterns for the same ceiling: 64 contiguous bytes per lane
+// per iteration (four 16-byte loads from one lane's own run), and 16-byte
+// non-temporal loads (no reuse expected in the caches).
+__global__ __launch_bounds__(256) void stream_read_wide_kernel(const double2* __restrict__ s,
+                                                               double* __restrict__ out, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc0 = 0.0, acc1 = 0.0;
+    for (; 4 * i + 3 < n; i += stride) {
+        const double2 a = s[4 * i], b = s[4 * i + 1], c = s[4 * i + 2], e = s[4 * i + 3];
+        acc0 += (a.x + b.x) + (c.x + e.x);
+        acc1 += (a.y + b.y) + (c.y + e.y);
+    }
+    for (size_t j = 4 * i; j < n && j < 4 * i + 4; ++j) acc0 += s[j].x + s[j].y;
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = acc0 + acc1;
+}
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void stream_read_nt_kernel(const double2* __restrict__ s2, double* __restrict__ out,
+                                                             size_t n) {
+    const v2d* s = reinterpret_cast<const v2d*>(s2);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc0 = 0.0, acc1 = 0.0;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const v2d a = __builtin_nontemporal_load(s + i), b = __builtin_nontemporal_load(s + i + stride);
+        const v2d c = __builtin_nontemporal_load(s + i + 2 * stride);
+        const v2d e = __builtin_nontemporal_load(s + i + 3 * stride);
+        acc0 += (a.x + b.x) + (c.x + e.x);
+        acc1 += (a.y + b.y) + (c.y + e.y);
+    }
+    for (; i < n; i += stride) acc0 += s[i].x + s[i].y;
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = acc0 + acc1;
+}
+
 // Device math probes for the parity tests (sqrt, division, pow, pow6).
 __global__ void probe_kernel(const double* x, const double* y, double* out, int n, int which) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1527,12 +1562,26 @@ hipError_t stream_probe(int kind, const void* src, void* dst, size_t bytes, hipS
     const uint32_t threads = stream_probe_threads(bytes);
     if (threads == 0) return hipErrorInvalidValue;
     const dim3 grid(threads / 256), block(256);
-    if (kind == 0)
+    switch (kind) {
+    case 0:
         hipLaunchKernelGGL(stream_copy_kernel, grid, block, 0, stream, static_cast<const double2*>(src),
                            static_cast<double2*>(dst), n);
-    else
+        break;
+    case 1:
         hipLaunchKernelGGL(stream_read_kernel, grid, block, 0, stream, static_cast<const double2*>(src),
                            static_cast<double*>(dst), n);
+        break;
+    case 2: return hipMemcpyAsync(dst, src, bytes / 16 * 16, hipMemcpyDeviceToDevice, stream);
+    case 3:
+        hipLaunchKernelGGL(stream_read_wide_kernel, grid, block, 0, stream, static_cast<const double2*>(src),
+                           static_cast<double*>(dst), n);
+        break;
+    case 4:
+        hipLaunchKernelGGL(stream_read_nt_kernel, grid, block, 0, stream, static_cast<const double2*>(src),
+                           static_cast<double*>(dst), n);
+        break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
