@@ -106,6 +106,8 @@ Config config_from_toml(const toml::Value& in) {
                                             {"SSPRK3", Integrator::SSPRK3}},
                                            "time integrator type");
     c.check_nan = find_or_bool(in, {"numerics", "check_nan"}, false);
+    // extension key (no reference counterpart): the positivity-preserving flux limiter
+    c.positivity = find_or_bool(in, {"numerics", "positivity_limiter"}, false);
     if (c.recon == Recon::TENO) {
         c.teno.basis = lookup_enum<BasisType>(find_or_str(fr, {"basis_type"}, "monomial"),
                                               {{"monomial", BasisType::Monomial}, {"legendre", BasisType::Legendre}},

@@ -158,3 +158,13 @@ def test_run_section_fixed_dt(mlr, tmp_path):
     run = mlr.parse_config(p)[2]
     assert run["cfl"] == 0.0 and run["dt"] == 2.5e-4
     assert run["t_stop"] == 0.01 and run["n_steps"] == UINT32_MAX
+
+
+def test_positivity_limiter_key(mlr, tmp_path):
+    """[numerics] positivity_limiter (extension key, default off)."""
+    base = open(os.path.join(HERE, "sod.toml")).read()
+    assert "[numerics]" in base
+    p = tmp_path / "pp.toml"
+    p.write_text(base.replace("[numerics]", "[numerics]\npositivity_limiter = true", 1))
+    assert mlr.parse_config(p)[1].positivity == 1
+    assert mlr.parse_config(os.path.join(HERE, "sod.toml"))[1].positivity == 0
