@@ -175,8 +175,8 @@ int mlr_run_steps_profiled(mlr_ctx* ctx, int64_t n, double* stats);
 int mlr_compact_tables(mlr_ctx* ctx);
 /* Launch shapes; no reference counterpart (the reference leaves them to
  * Kokkos' defaults). tune[4] = { TENO occupancy target in waves per SIMD
- * (0: the compiler's choice, 5, 6; used at p = 2 with the default stencil
- * size), cells per flux/stage-update block (64, 128, 256), cells per
+ * (0: the compiler's choice; 5 or 6 at p = 2 and 4 or 5 for plain TENO at
+ * p = 3, each with the default stencil size), cells per flux/stage-update block (64, 128, 256), cells per
  * spectral-radius block (64, 128, 256), the TENO occupancy target of gated
  * passes that reuse recorded decisions }. Every choice gives identical
  * results. mlr_autotune times the candidates on the current state and

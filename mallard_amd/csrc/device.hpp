@@ -151,8 +151,8 @@ struct BatchStage {
 // Launch shapes an autotuner picks per mesh. Every choice gives identical
 // results; only the speed differs.
 struct Tune {
-    int32_t teno_occ = 0;       // TENO waves per SIMD targeted at p = 2, stencil factor 2: 0 (the
-                                // compiler's choice), 5 or 6
+    int32_t teno_occ = 0;       // TENO waves per SIMD targeted with stencil factor 2: 0 (the
+                                // compiler's choice); 5 or 6 at p = 2, 4 or 5 for plain p = 3
     uint32_t flux_block = 256;  // cells per flux / stage-update block: 64, 128 or 256
     uint32_t spec_block = 256;  // cells per spectral-radius block: 64, 128 or 256
     int32_t teno_occ_reuse = 0; // the same target for gated passes that reuse recorded decisions

@@ -1226,7 +1226,7 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
     const int Kp = t.K - 1, Mp = t.M - 1;
     const uint32_t FQ = g.S * g.Q;
     if (g.S != 3 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
-    if (occ != 0 && occ != 5 && occ != 6) return hipErrorInvalidValue;
+    if (occ != 0 && occ != 4 && occ != 5 && occ != 6) return hipErrorInvalidValue;
     const uint32_t nblk = (g.N + 63) / 64, per_xcd = (nblk + 7) / 8;
     const dim3 grid(8 * per_xcd), block(256);
     const bool small = t.loc8 != nullptr;
@@ -1326,7 +1326,22 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
         switch (Kp) {
         case 2: MLR_TENO_G(2, 0, 6, 0); break;
         case 5: MLR_TENO(5, 0, 6); break;
-        case 9: if (Mp == 19) MLR_TENO(9, 19, 6); else MLR_TENO(9, 0, 6); break;
+        case 9:
+            if (Mp == 19 && !t.gate && (occ == 4 || occ == 5)) {  // plain p = 3 at an occupancy target
+                if (small && occ == 4) MLR_TENO_S(9, 19, 6, 0, 4);
+                else if (small) MLR_TENO_S(9, 19, 6, 0, 5);
+                else if (occ == 4)
+                    hipLaunchKernelGGL((teno_kernel<9, 19, 6, uint16_t, 1024, 0, 4>), grid, block, 1024 * 32, stream,
+                                       g, t, U, fs, dec, ph, per_xcd);
+                else
+                    hipLaunchKernelGGL((teno_kernel<9, 19, 6, uint16_t, 1024, 0, 5>), grid, block, 1024 * 32, stream,
+                                       g, t, U, fs, dec, ph, per_xcd);
+            } else if (Mp == 19) {
+                MLR_TENO(9, 19, 6);
+            } else {
+                MLR_TENO(9, 0, 6);
+            }
+            break;
         default: return hipErrorInvalidValue;
         }
     }

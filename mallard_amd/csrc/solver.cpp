@@ -850,8 +850,8 @@ void Solver::device_matrices(uint32_t cell, double* aplus, double* ta) const {
 void Solver::set_tune(const dev::Tune& t) {
     auto bs_ok = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
     if (!bs_ok(t.flux_block) || !bs_ok(t.spec_block)) fail("block sizes must be 64, 128 or 256 cells");
-    auto occ_ok = [](int o) { return o == 0 || o == 5 || o == 6; };
-    if (!occ_ok(t.teno_occ) || !occ_ok(t.teno_occ_reuse)) fail("TENO occupancy target must be 0, 5 or 6");
+    auto occ_ok = [](int o) { return o == 0 || o == 4 || o == 5 || o == 6; };
+    if (!occ_ok(t.teno_occ) || !occ_ok(t.teno_occ_reuse)) fail("TENO occupancy target must be 0, 4, 5 or 6");
     tune_ = t;
     tuned_ = true;
     drop_graph();
@@ -912,12 +912,14 @@ void Solver::autotune() {
             HIP_OK(hipMalloc((void**)&saved, gated_slots_ * sizeof(uint32_t)));
             HIP_OK(hipMemcpyAsync(saved, gated_, gated_slots_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
         }
-        // the occupancy targets exist for the p = 2 kernels, and not for
-        // split gated passes
+        // occupancy targets: 5 and 6 waves per SIMD for the p = 2 kernels
+        // (not for split gated passes), 4 and 5 for plain p = 3 with the
+        // default stencil size; the compiler's own choice (0) everywhere
         const bool occ_tunable = L_.K == 6 && L_.M == 12 && fq == 3 && loc8_ != nullptr && !(gate_ && gate_split_);
+        const bool occ_p3 = L_.K == 10 && L_.M == 20 && fq == 6 && !gate_;
         bt = 1e30f;
-        for (int occ : {0, 5, 6}) {
-            if (occ != 0 && !occ_tunable) continue;
+        for (int occ : {0, 4, 5, 6}) {
+            if (occ != 0 && !(occ_tunable && occ >= 5) && !(occ_p3 && occ <= 5)) continue;
             const float t = timed([&] {
                 HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, occ, 0, stream_));
             });

@@ -20,7 +20,8 @@ SOD = np.array([[0.125, 0.0, 0.0, 0.1], [1.0, 0.0, 0.0, 1.0], [1.0, 0.0, 0.0, 1.
                 [0.125, 0.0, 0.0, 0.1]])
 
 
-def pair(mlr, oracle, kind, nx, ny, Lx=1.0, Ly=1.0, jitter=0.0, seed=0, bcs=None, **kw):
+def pair(mlr, oracle, kind, nx, ny, Lx=1.0, Ly=1.0, jitter=0.0, seed=0, bcs=None, threads=1, canonical=False,
+         **kw):
     num = mlr.numerics(bcs=bcs, **kw)
     dev = mlr.Solver(gen=mlr.mesh_gen(kind, nx, ny, Lx, Ly, jitter, seed), num=num)
     m = oracle.Mesh(kind, nx, ny, Lx, Ly, jitter, seed)
@@ -29,7 +30,7 @@ def pair(mlr, oracle, kind, nx, ny, Lx=1.0, Ly=1.0, jitter=0.0, seed=0, bcs=None
     if "stencil_factor" in kw:
         okw["factor"] = kw["stencil_factor"]
     cfg = oracle.make_config(bcs=bcs, **okw)
-    orc = oracle.Solver(m, cfg, 1)
+    orc = oracle.Solver(m, cfg, threads, canonical=canonical)
     return dev, orc, m
 
 
@@ -183,6 +184,27 @@ def test_teno_corrected_100_steps(mlr, oracle):
     assert np.isfinite(U2).all()
     err = rel_linf(U1, U2)
     print("corrected TENO, 100 steps: rel L-inf %.3e, identical cells %.4f" % (err, np.mean(U1 == U2)))
+    assert err <= 1e-10
+    assert np.allclose(dts, dto, rtol=1e-12)
+
+
+def test_teno_p3_corrected_50_steps(mlr, oracle):
+    """The p = 3 secondary case (the reference example's basis order: K = 10,
+    20-member stencils, two Gauss points per face, the 16-bit tile form) for
+    50 steps of the four-quadrant problem on a 96^2 jittered mesh, against
+    the oracle in its canonical order."""
+    bcs = [(z, "extrapolation", {}) for z in ("left", "right", "top", "bottom")]
+    dev, orc, m = pair(mlr, oracle, "cartesian_tri", 96, 96, jitter=0.2, seed=3033, recon="TENO",
+                       order=3, teno_variant="corrected", cfl=0.4, bcs=bcs, threads=8, canonical=True)
+    assert (dev.K, dev.M, dev.Q) == (10, 20, 2)
+    dev.init_quadrant(0.5, 0.5, RIEMANN2D)
+    orc.init_quadrant(0.5, 0.5, RIEMANN2D)
+    dts = dev.run_steps(50)
+    dto = orc.run(50, 0.4)
+    U1, U2 = dev.state()[0], orc.state()[0]
+    assert np.isfinite(U2).all()
+    err = rel_linf(U1, U2)
+    print("corrected TENO p=3, 50 steps: rel L-inf %.3e, identical cells %.4f" % (err, np.mean(U1 == U2)))
     assert err <= 1e-10
     assert np.allclose(dts, dto, rtol=1e-12)
 

@@ -49,6 +49,24 @@ def test_every_shape_gives_identical_results(mlr, gate):
         assert 0 < counts[0] < 48 * 48 * 2 * 3 * 8
 
 
+def test_p3_occupancy_targets_give_identical_results(mlr):
+    """Plain TENO at p = 3 in its three occupancy targets (the compiler's
+    choice, 4 and 5 waves per SIMD) gives the same states bit for bit."""
+    gen = mlr.mesh_gen("cartesian_tri", 40, 40, 1.0, 1.0, 0.2, 12)
+    num = mlr.numerics(recon="TENO", order=3, teno_variant="corrected", cfl=0.4, bcs=EXTRAP)
+    out = []
+    for occ in (0, 4, 5):
+        s = mlr.Solver(gen=gen, num=num)
+        s.init_quadrant(0.5, 0.5, RIEMANN2D)
+        s.set_tune(occ, 128, 128, 0)
+        s.run_steps(5)
+        out.append(s.state()[0])
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+    s = mlr.Solver(gen=gen, num=num)
+    s.init_quadrant(0.5, 0.5, RIEMANN2D)
+    assert s.autotune()[0] in (0, 4, 5)
+
+
 def test_autotune_leaves_state_and_counts(mlr):
     s = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", 40, 40, 1.0, 1.0, 0.2, 9), num=_num(mlr))
     s.init_quadrant(0.5, 0.5, RIEMANN2D)
