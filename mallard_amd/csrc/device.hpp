@@ -2,11 +2,24 @@
 // passes plain device pointers; no HIP types appear here except the stream.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 #include <hip/hip_runtime_api.h>
 
 namespace mlr::dev {
+
+// Blocked TENO member tables (stencil ids, tile positions, member areas; A+
+// with one more level for its dof k): for 64-cell block b and member i
+// after the target, the block's 64 cells x 4 stencils, cell-major. A wave's
+// 64 lanes (16 cells x 4 stencils, lane = 4 cell + stencil) then read 64
+// consecutive entries for each (i, k).
+constexpr size_t teno_mem_off(size_t b, int Mp, int i, uint32_t cl, int s) {
+    return ((b * (size_t)Mp + (size_t)i) * 64 + cl) * 4 + (size_t)s;
+}
+constexpr size_t teno_a_off(size_t b, int Mp, int KP, int i, int k, uint32_t cl, int s) {
+    return (((b * (size_t)Mp + (size_t)i) * (size_t)KP + (size_t)k) * 64 + cl) * 4 + (size_t)s;
+}
 
 struct Physics {
     double gamma, R, cp, cv, p_min, p_max;

@@ -20,7 +20,7 @@ namespace {
 // cell id, read through the block tile tables
 __device__ inline int32_t central_member(const Teno& t, uint32_t c, int i) {
     const int Mp = t.M - 1;
-    const size_t o = (((size_t)(c >> 6) * 4) * Mp + i) * 64 + (c & 63u);
+    const size_t o = teno_mem_off(c >> 6, Mp, i, c & 63u, 0);
     const uint32_t pos = t.loc8 ? (uint32_t)t.loc8[o] : (uint32_t)t.loc16[o];
     return t.blk_ids[t.blk_off[c >> 6] + pos];
 }

@@ -165,7 +165,6 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         const bool vm = cm < N;
         const D4 Uo = ld4(U, vm ? cm : 0);
         const ind::CellRef r = ind::cell_ref_col(t.gamma, Uo.v, gq, lane & 15);
-        const size_t crow = (size_t)lb * 4 * Mp;  // stencil 0 rows of this block
         // lane (column cell, group gq) holds members 4 kc + gq, kc = 0..2
         double d[3][4], smax[4] = {0.0, 0.0, 0.0, 0.0};
         bool ok = r.ok;
@@ -173,7 +172,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         for (int kc = 0; kc < 3; ++kc) {
             const int m = 4 * kc + gq;
             if (m == 0) continue;
-            const uint32_t pos = loc[(crow + (m - 1)) * 64 + (cm & 63u)];
+            const uint32_t pos = loc[teno_mem_off(lb, Mp, m - 1, cm & 63u, 0)];  // stencil 0
             const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
             const double Um[4] = {u01.x, u01.y, u23.x, u23.y};
             ok = ind::member_diff(t.gamma, r, Um, d[kc]) && ok;
@@ -220,9 +219,10 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
     const uint32_t info0 = g.info[cc];
     const D4 uc = ld4(U, cc);
     const uint32_t cl = c & 63u;
-    // the member update a += A+_i (ta_i (u_i - u_c)) of stencil row0's member i
-    auto member = [&](size_t row0, int i, double (&a)[KP][4]) {
-        const size_t o = (row0 + i) * 64 + cl;
+    // the member update a += A+_i (ta_i (u_i - u_c)) of member i of this
+    // lane's stencil q; a wave's 64 lanes read 64 consecutive entries
+    auto member = [&](int i, double (&a)[KP][4]) {
+        const size_t o = teno_mem_off(lb, Mp, i, cl, q);
         const uint32_t pos = loc[o];
         const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
         const D4 un = {{u01.x, u01.y, u23.x, u23.y}};
@@ -235,10 +235,10 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
 #pragma unroll
             for (int v = 0; v < 4; ++v) b[v] = tt * (un.v[v] - uc.v[v]);
         }
-        const double* Ai = t.A + (row0 + i) * (KP * 64) + cl;
+        const double* Ai = t.A + teno_a_off(lb, Mp, KP, i, 0, cl, q);
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
-            const double Ak = Ai[k * 64];
+            const double Ak = Ai[(size_t)k * 256];
 #pragma unroll
             for (int v = 0; v < 4; ++v) a[k][v] = a[k][v] + Ak * b[v];
         }
@@ -255,10 +255,9 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
         const bool has0 = q < FQ, has1 = q + 4 < FQ;
         if (valid && has0) {
-            const size_t row0 = (size_t)lb * 4 * Mp;  // central stencil rows of this block
             const double* Lw = t.lin + ((size_t)lb * FQ) * Mp * 64 + cl;
             auto add = [&](int i) {
-                const uint32_t pos = loc[(row0 + i) * 64 + cl];
+                const uint32_t pos = loc[teno_mem_off(lb, Mp, i, cl, 0)];  // central stencil
                 const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
                 const double d[4] = {u01.x - uc.v[0], u01.y - uc.v[1], u23.x - uc.v[2], u23.y - uc.v[3]};
                 const double w0 = Lw[((size_t)q * Mp + i) * 64];
@@ -301,13 +300,11 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
             for (int v = 0; v < 4; ++v) a[k][v] = 0.0;
         double w[4] = {0.0, 0.0, 0.0, 0.0};
         if (active) {
-            // blocked tables: this block's rows start at (lb * 4 + q) * Mp
-            const size_t row0 = ((size_t)lb * 4 + q) * Mp;
             if constexpr (MP > 0) {
 #pragma unroll
-                for (int i = 0; i < MP; ++i) member(row0, i, a);
+                for (int i = 0; i < MP; ++i) member(i, a);
             } else {
-                for (int i = 0; i < Mp; ++i) member(row0, i, a);
+                for (int i = 0; i < Mp; ++i) member(i, a);
             }
 #pragma unroll
             for (int v = 0; v < 4; ++v) {

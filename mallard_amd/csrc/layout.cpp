@@ -637,6 +637,7 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
     L.K = ctx.K;
     L.M = ctx.M;
     L.G = (int)S + 1;
+    if (L.G != 4) fail("TENO tables are laid out for triangles (4 stencils per cell)");
     L.Q = (uint32_t)ctx.qf.n;
     L.qface_w = ctx.qf.w;
     L.qface_pts = ctx.qf.pts;

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "device.hpp"
 #include "mesh.hpp"
 #include "numerics.hpp"
 
@@ -92,18 +93,18 @@ struct Layout {
     uint64_t device_bytes() const;
 
     // Blocked (AoSoA) indices of the per-stencil tables: cells in blocks of
-    // 64 (one tile), stencil- then member-major inside a block, so the data
-    // one thread block reads is one contiguous range. i counts the members
-    // after the target (0 .. M-2), fq = slot * Q + point.
-    //   ids / ta / positions: [N/64][G][M-1][64]
-    //   A+:                   [N/64][G][M-1][K-1][64]
+    // 64 (one tile), member-major inside a block, so the data one thread
+    // block reads is one contiguous range, and for each member the block's
+    // cells x stencils cell-major (G = 4), so a wave's 64 lanes (16 cells x 4
+    // stencils) read 64 consecutive entries (dev::teno_mem_off, teno_a_off).
+    // i counts the members after the target (0 .. M-2), fq = slot * Q + point.
+    //   ids / ta / positions: [N/64][M-1][64][G]
+    //   A+:                   [N/64][M-1][K-1][64][G]
     //   face basis:           [N/64][S*Q][K-1][64]
     size_t n_pad() const { return ((size_t)N + 63) / 64 * 64; }
-    size_t mem_idx(int s, int i, uint32_t c) const {
-        return (((size_t)(c >> 6) * G + s) * (M - 1) + i) * 64 + (c & 63u);
-    }
+    size_t mem_idx(int s, int i, uint32_t c) const { return dev::teno_mem_off(c >> 6, M - 1, i, c & 63u, s); }
     size_t a_idx(int s, int i, int k, uint32_t c) const {
-        return ((((size_t)(c >> 6) * G + s) * (M - 1) + i) * (K - 1) + k) * 64 + (c & 63u);
+        return dev::teno_a_off(c >> 6, M - 1, K - 1, i, k, c & 63u, s);
     }
     size_t fb_idx(uint32_t fq, int k, uint32_t c) const {
         return (((size_t)(c >> 6) * (S * Q) + fq) * (K - 1) + k) * 64 + (c & 63u);

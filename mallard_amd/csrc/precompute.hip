@@ -108,17 +108,16 @@ __device__ inline void member_row(const PreInput& in, uint32_t nb, const double*
     }
 }
 
-// thread = (block of 64 cells, stencil s, cell in block): the same order as
-// the blocked tables, so consecutive lanes write consecutive doubles
+// thread = (block of 64 cells, cell in block, stencil s): the order of the
+// blocked tables, so consecutive lanes write consecutive doubles
 template <int K, int MP>
 __global__ __launch_bounds__(256) void aplus_kernel(PreInput in, double* __restrict__ A, double* __restrict__ ta,
                                                     int* __restrict__ err) {
     constexpr int KP = K - 1;
-    const uint32_t blk = blockIdx.x, s = threadIdx.x >> 6, cl = threadIdx.x & 63u;
+    const uint32_t blk = blockIdx.x, s = threadIdx.x & 3u, cl = threadIdx.x >> 2;
     const uint32_t c = blk * 64u + cl;
     if (c >= in.N) return;
     if (!((in.info[c] >> (20 + s)) & 1u)) return;
-    const size_t row0 = ((size_t)blk * 4 + s) * MP;  // blocked member rows of this stencil
 
     // target frame
     double w0[2], w1[2], w2[2], J[4], Ji[4];
@@ -145,11 +144,11 @@ __global__ __launch_bounds__(256) void aplus_kernel(PreInput in, double* __restr
     member_row<K>(in, c, w0, Ji, row, at);
     if (row[0] - at * in.ipt[0] > 1.0e-12) col0 = false;
     for (int i = 0; i < MP; ++i) {
-        const uint32_t nb = (uint32_t)in.ids[(row0 + i) * 64 + cl];
+        const uint32_t nb = (uint32_t)in.ids[teno_mem_off(blk, MP, i, cl, (int)s)];
         member_row<K>(in, nb, w0, Ji, row, at);
         if (row[0] - at * in.ipt[0] > 1.0e-12) col0 = false;
         for (int k = 1; k < K; ++k) B[i][k - 1] = row[k] - at * in.ipt[k];
-        ta[(row0 + i) * 64 + cl] = at;
+        ta[teno_mem_off(blk, MP, i, cl, (int)s)] = at;
     }
     if (!col0) {
         atomicOr(err, 2);
@@ -207,7 +206,7 @@ __global__ __launch_bounds__(256) void aplus_kernel(PreInput in, double* __restr
             X[i][jj] = (Y[i][jj] - sum) / R[i][i];
         }
     for (int i = 0; i < MP; ++i)
-        for (int k = 0; k < KP; ++k) A[((row0 + i) * KP + k) * 64 + cl] = X[k][i];
+        for (int k = 0; k < KP; ++k) A[teno_a_off(blk, MP, KP, i, k, cl, (int)s)] = X[k][i];
 }
 
 // thread = (block of 64 cells, face point fq, cell in block); the member loop
@@ -234,26 +233,25 @@ __global__ __launch_bounds__(256) void linear_operator_kernel(Teno t, const uint
     }
     for (int i = 0; i < Mp; ++i) {
         double w = 0.0;
-        if (on) {
-            const size_t row = blk * 4 * Mp + i;  // central stencil, member i
-            const double* Ai = t.A + row * (KP * 64) + cl;
+        if (on) {  // central stencil (s = 0), member i
+            const double* Ai = t.A + teno_a_off(blk, Mp, KP, i, 0, cl, 0);
             double acc = 0.0;
-            for (int k = 0; k < KP; ++k) acc = acc + B[k * bs] * Ai[k * 64];
-            w = t.compact ? acc : acc * t.ta[row * 64 + cl];
+            for (int k = 0; k < KP; ++k) acc = acc + B[k * bs] * Ai[(size_t)k * 256];
+            w = t.compact ? acc : acc * t.ta[teno_mem_off(blk, Mp, i, cl, 0)];
         }
         lin[((blk * FQ + fq) * Mp + i) * 64 + cl] = w;
     }
 }
 
-// one thread per (table row, cell of the block)
+// one thread per member-table entry e = (block, member) * 256 + cell * 4 + stencil;
+// its A+ entries are those of the same (block, member) and slot, 256 apart over k
 __global__ __launch_bounds__(256) void fold_areas_kernel(double* __restrict__ A, const double* __restrict__ ta,
                                                          uint64_t total, int KP) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= total) return;
-    const uint64_t row = gid >> 6, cl = gid & 63u;
     const double t = ta[gid];
-    double* a = A + row * (uint64_t)KP * 64 + cl;
-    for (int k = 0; k < KP; ++k) a[k * 64] = a[k * 64] * t;
+    double* a = A + (gid >> 8) * (uint64_t)KP * 256 + (gid & 255u);
+    for (int k = 0; k < KP; ++k) a[(uint64_t)k * 256] = a[(uint64_t)k * 256] * t;
 }
 
 }  // namespace
