@@ -20,6 +20,11 @@ constexpr size_t teno_mem_off(size_t b, int Mp, int i, uint32_t cl, int s) {
 constexpr size_t teno_a_off(size_t b, int Mp, int KP, int i, int k, uint32_t cl, int s) {
     return (((b * (size_t)Mp + (size_t)i) * (size_t)KP + (size_t)k) * 64 + cl) * 4 + (size_t)s;
 }
+// the central stencil's linear operator of gated runs: for block b and
+// member i, the block's 64 cells x FQ face points, cell-major
+constexpr size_t teno_lin_off(size_t b, int Mp, int i, uint32_t cl, int FQ, int fq) {
+    return ((b * (size_t)Mp + (size_t)i) * 64 + cl) * (size_t)FQ + (size_t)fq;
+}
 
 struct Physics {
     double gamma, R, cp, cv, p_min, p_max;
@@ -89,8 +94,8 @@ struct Teno {
     ind::Net net;
     uint32_t* gated;
     uint8_t* gate_dec;  // per cell: 1 when the last network evaluation judged it smooth
-    // gated runs: the central stencil's linear reconstruction as one operator,
-    // lin[N/64][S*Q][M-1][64]: face point fq of cell c is
+    // gated runs: the central stencil's linear reconstruction as one operator
+    // (teno_lin_off): face point fq of cell c is
     // u_c + sum_i lin(fq, i) (u_i - u_c) over the stencil's members i
     const double* lin;
     double gamma;

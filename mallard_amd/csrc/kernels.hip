@@ -255,16 +255,16 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
         const bool has0 = q < FQ, has1 = q + 4 < FQ;
         if (valid && has0) {
-            const double* Lw = t.lin + ((size_t)lb * FQ) * Mp * 64 + cl;
             auto add = [&](int i) {
+                const double* Lw = t.lin + teno_lin_off(lb, Mp, i, cl, FQ, q);
                 const uint32_t pos = loc[teno_mem_off(lb, Mp, i, cl, 0)];  // central stencil
                 const double2 u01 = sh_u[2 * pos], u23 = sh_u[2 * pos + 1];
                 const double d[4] = {u01.x - uc.v[0], u01.y - uc.v[1], u23.x - uc.v[2], u23.y - uc.v[3]};
-                const double w0 = Lw[((size_t)q * Mp + i) * 64];
+                const double w0 = Lw[0];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[0][v] = acc[0][v] + w0 * d[v];
                 if (has1) {
-                    const double w1 = Lw[((size_t)(q + 4) * Mp + i) * 64];
+                    const double w1 = Lw[4];
 #pragma unroll
                     for (int v = 0; v < 4; ++v) acc[1][v] = acc[1][v] + w1 * d[v];
                 }

@@ -209,16 +209,17 @@ __global__ __launch_bounds__(256) void aplus_kernel(PreInput in, double* __restr
         for (int k = 0; k < KP; ++k) A[teno_a_off(blk, MP, KP, i, k, cl, (int)s)] = X[k][i];
 }
 
-// thread = (block of 64 cells, face point fq, cell in block); the member loop
-// writes lin(fq, i) for every member of the cell's central stencil
+// thread = (block of 64 cells, cell in block, face point fq), the order of
+// the operator's table; the member loop writes lin(fq, i) for every member
+// of the cell's central stencil
 __global__ __launch_bounds__(256) void linear_operator_kernel(Teno t, const uint32_t* __restrict__ info, uint32_t N,
                                                               int FQ, uint64_t total, double* __restrict__ lin) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= total) return;
-    const uint32_t cl = (uint32_t)(gid & 63u);
-    const uint64_t rest = gid >> 6;
-    const int fq = (int)(rest % (uint64_t)FQ);
-    const uint64_t blk = rest / (uint64_t)FQ;
+    const int fq = (int)(gid % (uint64_t)FQ);
+    const uint64_t rest = gid / (uint64_t)FQ;
+    const uint32_t cl = (uint32_t)(rest & 63u);
+    const uint64_t blk = rest >> 6;
     const uint32_t c = (uint32_t)(blk * 64 + cl);
     const int KP = t.K - 1, Mp = t.M - 1;
     const bool on = c < N && ((info[c] >> 20) & 1u);
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(256) void linear_operator_kernel(Teno t, const uint
             for (int k = 0; k < KP; ++k) acc = acc + B[k * bs] * Ai[(size_t)k * 256];
             w = t.compact ? acc : acc * t.ta[teno_mem_off(blk, Mp, i, cl, 0)];
         }
-        lin[((blk * FQ + fq) * Mp + i) * 64 + cl] = w;
+        lin[teno_lin_off(blk, Mp, i, cl, FQ, fq)] = w;
     }
 }
 
