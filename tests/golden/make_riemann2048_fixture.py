@@ -13,11 +13,16 @@ The fixture holds the 100 dt values, the final conserved state of a seeded
 sample of cells (10,000 drawn uniformly plus each variable's extreme cells),
 and whole-field norms per variable (max |U|, sum |U| V, sum U V, min U).
 
+With --symmetry the four sides are symmetry walls (the boundary set the
+survey's configuration names) and the positivity-preserving flux limiter is
+on, which that set needs to stay physical for 100 steps; the default output
+is then tests/golden/riemann2048_symmetry_pp_100.npz.
+
 The oracle runs with OpenMP in its canonical mode: each cell gathers its
 face contributions in the order of a one-thread run, so the fixture does not
 depend on the thread count (about 30 GB of host memory; minutes on 16
 cores). Usage:
-    python tests/golden/make_riemann2048_fixture.py [out.npz] [--threads N] [--nx N]
+    python tests/golden/make_riemann2048_fixture.py [out.npz] [--threads N] [--nx N] [--symmetry]
 """
 import argparse
 import os
@@ -37,17 +42,23 @@ NX, JITTER, SEED, STEPS, CFL, N_SAMPLE = 2048, 0.2, 2024, 100, 0.4, 10000
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("out", nargs="?", default=os.path.join(HERE, "riemann2048_corrected_100.npz"))
+    ap.add_argument("out", nargs="?", default=None)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--nx", type=int, default=NX)
+    ap.add_argument("--symmetry", action="store_true",
+                    help="symmetry walls on the four sides, with the positivity limiter")
     args = ap.parse_args()
+    if args.out is None:
+        args.out = os.path.join(HERE, "riemann2048_symmetry_pp_100.npz" if args.symmetry
+                                else "riemann2048_corrected_100.npz")
     import oracle_py as orc
     threads = args.threads or int(os.environ.get("OMP_NUM_THREADS", 0)) or os.cpu_count() or 1
     t0 = time.perf_counter()
     m = orc.Mesh("cartesian_tri", args.nx, args.nx, 1.0, 1.0, JITTER, SEED)
-    bcs = [(z, "extrapolation", {}) for z in ("left", "right", "top", "bottom")]
-    s = orc.Solver(m, orc.make_config(order=2, teno_variant="corrected", bcs=bcs), n_threads=threads,
-                   canonical=True)
+    kind = "symmetry" if args.symmetry else "extrapolation"
+    bcs = [(z, kind, {}) for z in ("left", "right", "top", "bottom")]
+    s = orc.Solver(m, orc.make_config(order=2, teno_variant="corrected", bcs=bcs, positivity=args.symmetry),
+                   n_threads=threads, canonical=True)
     s.init_quadrant(0.5, 0.5, QUADRANT)
     print(f"setup {time.perf_counter() - t0:.1f} s, {m.n_cells} cells, {threads} threads", flush=True)
     dts = np.zeros(STEPS)
@@ -71,7 +82,7 @@ def main():
         total=(U * V[:, None]).sum(axis=0), umin=U.min(axis=0),
         meta=np.array([args.nx, JITTER, SEED, STEPS, CFL, threads], dtype=np.float64))
     print(f"wrote {args.out}: {len(ids)} sampled cells, t = {dts.sum():.6g}, "
-          f"{time.perf_counter() - t0:.1f} s in all", flush=True)
+          f"{s.pp_limited()} limited faces, {time.perf_counter() - t0:.1f} s in all", flush=True)
 
 
 if __name__ == "__main__":

@@ -1,8 +1,10 @@
 """Parity at the benchmark's full size: the 8,388,608-cell four-quadrant
 Riemann workload (2048^2 quads split into triangles, jitter 0.2, seed 2024,
-TENO p = 2 corrected, HLLC, SSP-RK3, CFL 0.4, extrapolation boundaries) run
-100 steps on the GPU against the CPU oracle's run of the same workload,
-committed as tests/golden/riemann2048_corrected_100.npz by
+TENO p = 2 corrected, HLLC, SSP-RK3, CFL 0.4) run 100 steps on the GPU
+against the CPU oracle's run of the same workload, in two boundary sets:
+extrapolation on the four sides (tests/golden/riemann2048_corrected_100.npz),
+and symmetry walls on the four sides with the positivity-preserving flux
+limiter on (tests/golden/riemann2048_symmetry_pp_100.npz), both written by
 tests/golden/make_riemann2048_fixture.py.
 
 Bar (the acceptance tolerance of the path): conserved variables within 1e-10
@@ -20,18 +22,22 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "riemann2048_corrected_100.npz")
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 QUADRANT = np.array([[1.5, 0.0, 0.0, 1.5], [0.5323, 1.206, 0.0, 0.3],
                      [0.138, 1.206, 1.206, 0.029], [0.5323, 0.0, 1.206, 0.3]])
 
 
-def test_riemann2048_100_steps_against_oracle(mlr):
-    ref = np.load(FIXTURE)
+@pytest.mark.parametrize("fixture,kind,positivity", [
+    ("riemann2048_corrected_100.npz", "extrapolation", False),
+    ("riemann2048_symmetry_pp_100.npz", "symmetry", True),
+], ids=["extrapolation", "symmetry_positivity"])
+def test_riemann2048_100_steps_against_oracle(mlr, fixture, kind, positivity):
+    ref = np.load(os.path.join(GOLDEN, fixture))
     nx, jitter, seed, steps, cfl = ref["meta"][:5]
     nx, seed, steps = int(nx), int(seed), int(steps)
-    bcs = [(z, "extrapolation", {}) for z in ("left", "right", "top", "bottom")]
+    bcs = [(z, kind, {}) for z in ("left", "right", "top", "bottom")]
     num = mlr.numerics(recon="TENO", basis="legendre", order=2, stencil_factor=2.0, teno_variant="corrected",
-                       riemann="HLLC", integrator="SSPRK3", cfl=cfl, bcs=bcs)
+                       riemann="HLLC", integrator="SSPRK3", cfl=cfl, bcs=bcs, positivity=positivity)
     dev = mlr.Solver(gen=mlr.mesh_gen("cartesian_tri", nx, nx, 1.0, 1.0, jitter, seed), num=num)
     assert dev.n_cells == 2 * nx * nx
     dev.init_quadrant(0.5, 0.5, QUADRANT)
