@@ -72,7 +72,7 @@ def numerics_kw(args):
                 positivity=args.positivity)
 
 
-def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes, compact=False):
+def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes, compact=False, derived=False):
     """Algorithmic HBM bytes of one TENO face-state launch.
 
     alg: each compulsory element counted once, per non-empty stencil the
@@ -83,12 +83,15 @@ def teno_bytes(n_cells, n_stencils, K, M, S, Q, tile_entries, pos_bytes, compact
     positions, the tile lists (4 B per entry), the cell state, its info word,
     the face-point basis values, the face states written (S*Q points x 4
     variables) and the decision byte. Compact tables read neither the areas
-    (folded into the pseudo-inverse) nor the per-cell face basis.
+    (folded into the pseudo-inverse) nor the per-cell face basis; the
+    derived face basis reads the cell's node coordinates (48 B) instead of
+    the face basis.
     """
     alg = n_stencils * (8 * K * M + 12 * M) + 32 * n_cells
-    area_b, basis_b = (0, 0) if compact else (8, 8 * S * Q * (K - 1))
+    area_b = 0 if compact else 8
+    basis_b = 0 if (compact or derived) else 8 * S * Q * (K - 1)
     layout = (n_stencils * (8 * (K - 1) * (M - 1) + (area_b + pos_bytes) * (M - 1)) + 4 * tile_entries +
-              n_cells * (32 + 4 + basis_b + 8 * S * Q * 4 + 1))
+              n_cells * (32 + 4 + basis_b + 8 * S * Q * 4 + 1 + (48 if derived else 0)))
     return alg, layout
 
 
@@ -503,7 +506,8 @@ def main():
 
     # launch shapes picked on this mesh before anything is timed
     tune = dev.autotune()
-    log(f"[bench] autotuned launch shapes (TENO occupancy, flux block, spectral block): {tune}")
+    log(f"[bench] autotuned launch shapes (TENO occupancy, flux block, spectral block, reuse occupancy, "
+        f"TENO face basis stored 0 / derived 1): {tune}")
     if args.warmup > 0:
         dev.run_steps(args.warmup)
     log(f"[bench] warmup {args.warmup} steps done; timing {args.steps} steps")
@@ -528,7 +532,8 @@ def main():
     stages = {"SSPRK3": 3, "FE": 1}[args.integrator]
     teno_ms = st["teno_ms"] / max(st["teno_n"], 1)
     flux_ms = st["flux_ms"] / max(st["flux_n"], 1)
-    alg, lay = teno_bytes(N, dev.n_stencils, dev.K, dev.M, dev.S, dev.Q, dev.tile_entries, dev.tile_pos_bytes)
+    alg, lay = teno_bytes(N, dev.n_stencils, dev.K, dev.M, dev.S, dev.Q, dev.tile_entries, dev.tile_pos_bytes,
+                          derived=tune[4] == 1)
     achieved = alg / (teno_ms * 1e-3) / 1e9
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # summary, used only when it was measured on this workload and on the
@@ -641,7 +646,8 @@ def main():
         },
         "plain": {"value": value_plain, "ms_per_step": wall / args.steps * 1e3,
                   "ms_per_step_graph": wall_graph / args.steps * 1e3,
-                  "tables": "exact (bit-identical to the oracle)",
+                  "tables": ("exact, face basis derived in the kernel from node coordinates"
+                             if tune[4] == 1 else "exact, stored") + " (bit-identical to the oracle)",
                   "whole_step": step_roofline(alg, N, stages * args.steps, wall)},
         "plain_compact_tables": compact,
         "symmetry_walls": walls,
@@ -669,7 +675,7 @@ def main():
                        "spectral_dt": st["dt_ms"] / max(st["dt_n"], 1),
                        "copy": st["copy_ms"] / max(st["copy_n"], 1) if st["copy_n"] else None},
         "tune": {"teno_occupancy": tune[0], "flux_block": tune[1], "spectral_block": tune[2],
-                 "teno_occupancy_reuse": tune[3]},
+                 "teno_occupancy_reuse": tune[3], "teno_face_basis": "derived" if tune[4] == 1 else "stored"},
     }
     if not args.no_cpu:
         log("[bench] CPU baseline ...")

@@ -174,12 +174,16 @@ int mlr_run_steps_profiled(mlr_ctx* ctx, int64_t n, double* stats);
  * the exact tables are released. */
 int mlr_compact_tables(mlr_ctx* ctx);
 /* Launch shapes; no reference counterpart (the reference leaves them to
- * Kokkos' defaults). tune[4] = { TENO occupancy target in waves per SIMD
+ * Kokkos' defaults). tune[5] = { TENO occupancy target in waves per SIMD
  * (0: the compiler's choice; 5 or 6 at p = 2 and 4 or 5 for plain TENO at
  * p = 3, each with the default stencil size), cells per flux/stage-update block (64, 128, 256), cells per
  * spectral-radius block (64, 128, 256), the TENO occupancy target of gated
- * passes that reuse recorded decisions }. Every choice gives identical
- * results. mlr_autotune times the candidates on the current state and
+ * passes that reuse recorded decisions, the plain TENO pass's face-basis
+ * source: 0 reads the stored face-point basis values, 1 derives them in
+ * the kernel from the cell's node coordinates with the precompute's
+ * operations (p = 2, Legendre basis, default stencil size; occupancy target
+ * 0, 4 or 5): 72 fewer bytes per cell to read }. Every choice gives
+ * bit-identical results. mlr_autotune times the candidates on the current state and
  * returns the fastest (also run by itself before the first steps, cached per
  * mesh size and numerics); mlr_set_tune pins a choice. */
 int mlr_autotune(mlr_ctx* ctx, int32_t* tune);
@@ -340,7 +344,8 @@ int mlr_ensemble_run_steps(mlr_ensemble* ens, int64_t n);
 /* kernel-by-kernel with events (the stats layout of mlr_run_steps_profiled) */
 int mlr_ensemble_run_steps_profiled(mlr_ensemble* ens, int64_t n, double* stats);
 /* launch shapes of the batched kernels, as mlr_autotune / mlr_set_tune (the
- * flux and spectral block sizes are one value here: tune[2] = tune[1]), tune[4] */
+ * flux and spectral block sizes are one value here: tune[2] = tune[1]; the
+ * table source tune[4] is 0), tune[5] */
 int mlr_ensemble_autotune(mlr_ensemble* ens, int32_t* tune);
 int mlr_ensemble_set_tune(mlr_ensemble* ens, const int32_t* tune);
 int mlr_ensemble_set_indicator(mlr_ensemble* ens, const float* params, double threshold);

@@ -412,17 +412,18 @@ class Solver:
     def autotune(self):
         """Time the launch-shape candidates on the current state and keep the
         fastest: (TENO occupancy target, flux block, spectral block, TENO
-        occupancy target of gated passes reusing recorded decisions)."""
-        t = np.zeros(4, np.int32)
+        occupancy target of gated passes reusing recorded decisions, plain
+        TENO face-basis source: 0 stored, 1 derived from node coordinates)."""
+        t = np.zeros(5, np.int32)
         self._check(lib().mlr_autotune(self.h, _p(t)))
         return tuple(int(x) for x in t)
 
-    def set_tune(self, teno_occ=0, flux_block=256, spec_block=256, teno_occ_reuse=0):
-        t = np.array([teno_occ, flux_block, spec_block, teno_occ_reuse], np.int32)
+    def set_tune(self, teno_occ=0, flux_block=256, spec_block=256, teno_occ_reuse=0, teno_tables=0):
+        t = np.array([teno_occ, flux_block, spec_block, teno_occ_reuse, teno_tables], np.int32)
         self._check(lib().mlr_set_tune(self.h, _p(t)))
 
     def tune(self):
-        t = np.zeros(4, np.int32)
+        t = np.zeros(5, np.int32)
         self._check(lib().mlr_get_tune(self.h, _p(t)))
         return tuple(int(x) for x in t)
 
@@ -632,12 +633,12 @@ class Ensemble:
             self._check(lib().mlr_ensemble_indicator_split(self.h, int(split)))
 
     def autotune(self):
-        t = np.zeros(4, np.int32)
+        t = np.zeros(5, np.int32)
         self._check(lib().mlr_ensemble_autotune(self.h, _p(t)))
-        return tuple(int(x) for x in t)
+        return tuple(int(x) for x in t[:4])
 
     def set_tune(self, teno_occ=0, block=256, teno_occ_reuse=0):
-        t = np.array([teno_occ, block, block, teno_occ_reuse], np.int32)
+        t = np.array([teno_occ, block, block, teno_occ_reuse, 0], np.int32)
         self._check(lib().mlr_ensemble_set_tune(self.h, _p(t)))
 
     def gated(self, reset=False):

@@ -25,6 +25,10 @@ constexpr size_t teno_a_off(size_t b, int Mp, int KP, int i, int k, uint32_t cl,
 constexpr size_t teno_lin_off(size_t b, int Mp, int i, uint32_t cl, int FQ, int fq) {
     return ((b * (size_t)Mp + (size_t)i) * 64 + cl) * (size_t)FQ + (size_t)fq;
 }
+// Bit of the slot-0 info word that is set when face slot l (0..2) of a
+// triangle runs from its edge's higher cell node to its lower one (the
+// face's node 0 is the cell node hi of the edge (lo, hi) in bits 24 + 2 l)
+constexpr int face_flip_shift(int l) { return l == 0 ? 30 : (l == 1 ? 31 : 19); }
 
 struct Physics {
     double gamma, R, cp, cv, p_min, p_max;
@@ -110,6 +114,18 @@ struct Teno {
     // slot-0 info word
     int32_t compact;
     const double* fbc;
+    // derived face basis (p = 2, Legendre basis, the default stencil size):
+    // the plain pass recomputes the face-point basis values from the cell's
+    // node coordinates with the precompute's own operations (bit-identical
+    // to fbasis, which it then does not read). cvert: [N][6] the cell's
+    // nodes (x0, y0, x1, y1, x2, y2) in its node order; ipt, poly: the
+    // basis' cell means and (px, py) per dof; qf: the face quadrature point
+    // on [-1, 1] (one per face)
+    int32_t derived;
+    const double* cvert;
+    double ipt[10];
+    uint8_t poly[20];
+    double qf;
 };
 
 // Stage programs of the fused residual + SSP-RK update.
@@ -174,6 +190,8 @@ struct Tune {
     uint32_t flux_block = 256;  // cells per flux / stage-update block: 64, 128 or 256
     uint32_t spec_block = 256;  // cells per spectral-radius block: 64, 128 or 256
     int32_t teno_occ_reuse = 0; // the same target for gated passes that reuse recorded decisions
+    int32_t teno_tables = 0;    // plain TENO pass: 0 reads the face-basis table, 1 derives
+                                // it from the cell's node coordinates (Teno::derived)
 };
 
 // Launchers (asynchronous on `stream`). Each returns a hipError_t.

@@ -94,6 +94,47 @@ __device__ inline void quad_gather(double f, double* out, int q) {
     for (int j = 0; j < 4; ++j) out[j] = sel4(r, (q - j) & 3);
 }
 
+// ------------------------------------------------------------ derived tables
+// The target cell's frame: node 0 and the inverse of the Jacobian of its
+// map from the reference triangle, as the precompute forms them.
+struct TriFrame {
+    double w0x, w0y, j0, j1, j2, j3;
+};
+__device__ inline TriFrame tri_frame(double x0, double y0, double x1, double y1, double x2, double y2) {
+    TriFrame F;
+    F.w0x = x0;
+    F.w0y = y0;
+    const double J0 = x1 - x0, J1 = x2 - x0, J2 = y1 - y0, J3 = y2 - y0;
+    const double det = J0 * J3 - J1 * J2;
+    const double id = 1.0 / det;
+    F.j0 = J3 * id;
+    F.j1 = -J1 * id;
+    F.j2 = -J2 * id;
+    F.j3 = J0 * id;
+    return F;
+}
+// a point in the frame: Ji (x - w0)
+__device__ inline void to_frame(const TriFrame& F, double x, double y, double& a, double& b) {
+    const double tx = x - F.w0x, ty = y - F.w0y;
+    a = F.j0 * tx + F.j1 * ty;
+    b = F.j2 * tx + F.j3 * ty;
+}
+// area in the frame of the triangle with nodes (x0, y0), (x1, y1), (x2, y2)
+__device__ inline double frame_area(const TriFrame& F, double x0, double y0, double x1, double y1, double x2,
+                                    double y2) {
+    double a0, b0, a1, b1, a2, b2;
+    to_frame(F, x0, y0, a0, b0);
+    to_frame(F, x1, y1, a1, b1);
+    to_frame(F, x2, y2, a2, b2);
+    return 0.5 * fabs(a0 * (b1 - b2) + a1 * (b2 - b0) + a2 * (b0 - b1));
+}
+// Legendre P_p(x) for p <= 2, terms combined as the basis evaluator does:
+// P_2 = 0.5 ((3 x) x - 1)
+__device__ inline double legendre_le2(int p, double x) {
+    const double p2 = 0.5 * ((3.0 * x) * x - 1.0);
+    return p == 0 ? 1.0 : (p == 1 ? x : p2);
+}
+
 // ------------------------------------------------------------ TENO kernel
 // 64 cells per 256-thread block, 4 lanes (a quad) per cell.
 // Phase A: quad lane s owns stencil s: stencil-major loads of ids, areas and
@@ -118,7 +159,11 @@ __device__ inline void quad_gather(double f, double* out, int q) {
 // CT: compact tables (corrected weighting): A carries the member areas and
 // the face-point basis values come from the small per-edge table t.fbc, so
 // neither ta nor the per-cell face basis is read.
-template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, bool CT = false>
+// DV: derived face basis (p = 2, Legendre basis, one face point per face):
+// the face-point basis values are recomputed from the cell's node
+// coordinates with the precompute's operations in its order, so they equal
+// the table bit for bit, and the per-cell face basis is not read.
+template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, bool CT = false, bool DV = false>
 __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const double* __restrict__ U,
                                            double* __restrict__ fs, uint8_t* __restrict__ dec, const Physics& ph,
                                            uint32_t lb, double2* sh_u, unsigned char* sh_net) {
@@ -378,6 +423,34 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
                 const double* B = t.fbc + ((info0 >> (24 + 2 * fq)) & 3u) * KP;
 #pragma unroll
                 for (int k = 0; k < KP; ++k) bk[k] = B[k];
+            } else if constexpr (DV) {
+                static_assert(FQ == 3 && KP == 5, "derived tables: p = 2, one face point per face");
+                // face slot fq runs from cell node ia to node ib: the edge
+                // (lo, hi) from bits 24 + 2 fq, its direction from the flip bit
+                const double2* cv = reinterpret_cast<const double2*>(t.cvert) + 3 * (size_t)cc;
+                const double2 n0 = cv[0], n1 = cv[1], n2 = cv[2];
+                const TriFrame F = tri_frame(n0.x, n0.y, n1.x, n1.y, n2.x, n2.y);
+                const uint32_t e = (info0 >> (24 + 2 * fq)) & 3u;
+                const bool flip = ((info0 >> face_flip_shift(fq)) & 1u) != 0u;
+                const uint32_t lo = e == 1u ? 1u : 0u, hi = e == 0u ? 1u : 2u;
+                const uint32_t ia = flip ? hi : lo, ib = flip ? lo : hi;
+                const double xa = ia == 0u ? n0.x : (ia == 1u ? n1.x : n2.x);
+                const double ya = ia == 0u ? n0.y : (ia == 1u ? n1.y : n2.y);
+                const double xb = ib == 0u ? n0.x : (ib == 1u ? n1.x : n2.x);
+                const double yb = ib == 0u ? n0.y : (ib == 1u ? n1.y : n2.y);
+                double ax, ay, bx, by;
+                to_frame(F, xa, ya, ax, ay);
+                to_frame(F, xb, yb, bx, by);
+                const double s = (t.qf + 1.0) * 0.5;
+                const double xq = s * (bx - ax) + ax, yq = s * (by - ay) + ay;
+                // the reference weighting adds ipt_k over the cell's own area in its frame
+                const double ta0 = t.variant != 0 ? 1.0 : frame_area(F, n0.x, n0.y, n1.x, n1.y, n2.x, n2.y);
+#pragma unroll
+                for (int k = 0; k < KP; ++k) {
+                    const double psi =
+                        legendre_le2(t.poly[2 * k + 2], xq) * legendre_le2(t.poly[2 * k + 3], yq);
+                    bk[k] = t.variant != 0 ? psi - t.ipt[k + 1] : psi + (t.ipt[k + 1] / ta0);
+                }
             } else {
                 const double* B = t.fbasis + (((size_t)lb * FQ + fq) * KP) * 64 + (c & 63u);
 #pragma unroll
@@ -462,7 +535,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
 
 // OCC: waves per SIMD the register allocation targets (0: the compiler's
 // own choice); an autotuner picks it per mesh, results are identical
-template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, int OCC, bool CT = false>
+template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, int OCC, bool CT = false, bool DV = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC == 0 ? 1 : OCC, OCC == 0 ? 8 : OCC)))
 void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
                                                    double* __restrict__ fs, uint8_t* __restrict__ dec,
@@ -470,8 +543,8 @@ void teno_kernel(Geom g, Teno t, const double* __restrict__ U,
     // dynamic LDS: the tile's states (TILE cells), then the network when gated
     extern __shared__ __align__(16) unsigned char teno_smem[];
     const uint32_t lb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
-    teno_block<KP, MP, FQ, IdxT, TILE, GM, CT>(g, t, U, fs, dec, ph, lb, reinterpret_cast<double2*>(teno_smem),
-                                               teno_smem + TILE * 32);
+    teno_block<KP, MP, FQ, IdxT, TILE, GM, CT, DV>(g, t, U, fs, dec, ph, lb, reinterpret_cast<double2*>(teno_smem),
+                                                   teno_smem + TILE * 32);
 }
 
 // case of global block gb: the last case whose first block is <= gb
@@ -1249,6 +1322,25 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
         else MLR_TENO_G(KP, MP, FQV, 1);                   \
     } while (0)
     if (t.gate && Mp < ind::kMembers - 1) return hipErrorInvalidValue;
+    if (t.derived) {
+        // derived face basis: plain TENO at p = 2 with the default stencil
+        // size and 8-bit tile positions
+        if (!(FQ == 3 && Kp == 5 && Mp == 11 && small) || t.cvert == nullptr || t.compact || t.gate)
+            return hipErrorInvalidValue;
+        const size_t lds = 256 * 32;
+        if (occ == 4)
+            hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 0, 4, false, true>), grid, block, lds, stream, g,
+                               t, U, fs, dec, ph, per_xcd);
+        else if (occ == 5)
+            hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 0, 5, false, true>), grid, block, lds, stream, g,
+                               t, U, fs, dec, ph, per_xcd);
+        else if (occ == 0)
+            hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 0, 0, false, true>), grid, block, lds, stream, g,
+                               t, U, fs, dec, ph, per_xcd);
+        else
+            return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     if (t.compact) {
         // compact tables: p = 2 with the default stencil size, 8-bit tile
         // positions, the compiler's occupancy

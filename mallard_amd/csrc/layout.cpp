@@ -732,6 +732,7 @@ Layout build_layout(const Mesh& m, Recon recon, const TenoSpec& ts, const std::v
                         const int lo = ia < ib ? ia : ib, hi = ia < ib ? ib : ia;
                         const uint32_t e = (lo == 0 && hi == 1) ? 0u : (lo == 1 && hi == 2) ? 1u : 2u;
                         L.info[j] |= e << (24 + 2 * l);
+                        if (ia > ib) L.info[j] |= 1u << dev::face_flip_shift((int)l);
                     }
                 }
                 double x0[2], x1[2];

@@ -40,6 +40,10 @@ struct TenoSpec {
 //   bits 8-15 (slot 0 only) accumulation order: rank r -> slot (2 bits each)
 //   bits 16-18 (slot 0 only) number of slots that contribute to the residual
 //   bits 20-23 (slot 0 only) bitmask of non-empty TENO stencils
+//   bits 24-29 (slot 0 only, TENO) reference-triangle edge of each slot l
+//              (bits 24 + 2 l: 0 nodes 0-1, 1 nodes 1-2, 2 nodes 0-2)
+//   bits 30, 31, 19 (slot 0 only, TENO) set when slot 0, 1, 2 runs from the
+//              edge's higher node to its lower one (dev::face_flip_shift)
 constexpr uint32_t kSlotBoundary = 1u << 3;
 constexpr uint32_t kNoBC = 15u;
 

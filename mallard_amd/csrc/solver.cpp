@@ -418,6 +418,18 @@ void Solver::upload() {
             up(A_, L_.st_A);
         }
         up(fbasis_, L_.fbasis);
+        if (L_.S == 3 && L_.Q == 1 && L_.K == 6 && L_.M == 12 && !L_.st_loc8.empty() &&
+            L_.tspec.basis == BasisType::Legendre && L_.qface_pts.size() == 1) {
+            // the cells' node coordinates, for the plain TENO pass's derived face basis
+            std::vector<double> cv(6 * (size_t)L_.N);
+            for (uint32_t j = 0; j < L_.N; ++j)
+                for (int r = 0; r < 3; ++r) {
+                    const uint32_t nd = mesh_.node_of_cell(L_.perm[j], r);
+                    cv[6 * (size_t)j + 2 * r] = mesh_.nc(nd, 0);
+                    cv[6 * (size_t)j + 2 * r + 1] = mesh_.nc(nd, 1);
+                }
+            up(cvert_, cv);
+        }
         // host copies of the big tables are not needed any more
         std::vector<int32_t>().swap(L_.st_ids);
         std::vector<uint8_t>().swap(L_.st_loc8);
@@ -501,6 +513,15 @@ dev::Teno Solver::teno() const {
     t.compact = compact_ ? 1 : 0;
     t.fbc = fbc_;
     t.gate = gate_ ? 1 : 0;
+    // the derived face basis serves the plain pass (no gate, exact tables kept)
+    t.derived = (tune_.teno_tables == 1 && cvert_ && !compact_ && !gate_) ? 1 : 0;
+    t.cvert = cvert_;
+    for (int k = 0; k < L_.K && k < 10; ++k) {
+        t.ipt[k] = L_.ipt[k];
+        t.poly[2 * k] = L_.poly[2 * k];
+        t.poly[2 * k + 1] = L_.poly[2 * k + 1];
+    }
+    t.qf = L_.qface_pts.empty() ? 0.0 : L_.qface_pts[0];
     if (gate_) t.net = dev::net_view(net_buf_, gate_thr_, nullptr);
     return t;
 }
@@ -852,6 +873,11 @@ void Solver::set_tune(const dev::Tune& t) {
     if (!bs_ok(t.flux_block) || !bs_ok(t.spec_block)) fail("block sizes must be 64, 128 or 256 cells");
     auto occ_ok = [](int o) { return o == 0 || o == 4 || o == 5 || o == 6; };
     if (!occ_ok(t.teno_occ) || !occ_ok(t.teno_occ_reuse)) fail("TENO occupancy target must be 0, 4, 5 or 6");
+    if (t.teno_tables != 0 && t.teno_tables != 1) fail("TENO face-basis source must be 0 (stored) or 1 (derived)");
+    if (t.teno_tables == 1) {
+        if (!cvert_) fail("derived TENO face basis: p = 2, Legendre basis, default stencil size, tiles of <= 256 cells");
+        if (t.teno_occ == 6) fail("derived TENO face basis: occupancy target 0, 4 or 5");
+    }
     tune_ = t;
     tuned_ = true;
     drop_graph();
@@ -867,7 +893,7 @@ void Solver::autotune() {
     const int fq = (int)(L_.S * L_.Q);
     const std::vector<int64_t> key = {(int64_t)L_.N, L_.K, L_.M, fq, loc8_ ? 1 : 0, gate_ ? 1 : 0,
                                       (int64_t)cfg_.riemann, L_.teno ? 1 : 0, gate_split_ ? 1 : 0,
-                                      gate_per_stage_ ? 1 : 0, cfg_.viscous ? 1 : 0};
+                                      gate_per_stage_ ? 1 : 0, cfg_.viscous ? 1 : 0, compact_ ? 1 : 0};
     {
         std::lock_guard<std::mutex> lk(mu);
         auto it = cache.find(key);
@@ -914,18 +940,26 @@ void Solver::autotune() {
         }
         // occupancy targets: 5 and 6 waves per SIMD for the p = 2 kernels
         // (not for split gated passes), 4 and 5 for plain p = 3 with the
-        // default stencil size; the compiler's own choice (0) everywhere
+        // default stencil size; the compiler's own choice (0) everywhere.
+        // The plain p = 2 pass also tries its derived face basis at 0, 4 and 5.
         const bool occ_tunable = L_.K == 6 && L_.M == 12 && fq == 3 && loc8_ != nullptr && !(gate_ && gate_split_);
         const bool occ_p3 = L_.K == 10 && L_.M == 20 && fq == 6 && !gate_;
+        const bool derivable = cvert_ != nullptr && !gate_ && !compact_;
         bt = 1e30f;
-        for (int occ : {0, 4, 5, 6}) {
-            if (occ != 0 && !(occ_tunable && occ >= 5) && !(occ_p3 && occ <= 5)) continue;
-            const float t = timed([&] {
-                HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, occ, 0, stream_));
-            });
-            if (t < bt) { bt = t; best.teno_occ = occ; }
+        for (int tables : {0, 1}) {
+            if (tables == 1 && !derivable) continue;
+            for (int occ : {0, 4, 5, 6}) {
+                if (tables == 0 && occ != 0 && !(occ_tunable && occ >= 5) && !(occ_p3 && occ <= 5)) continue;
+                if (tables == 1 && occ == 6) continue;
+                tune_.teno_tables = tables;
+                const float t = timed([&] {
+                    HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, occ, 0, stream_));
+                });
+                if (t < bt) { bt = t; best.teno_occ = occ; best.teno_tables = tables; }
+            }
         }
         tune_.teno_occ = best.teno_occ;
+        tune_.teno_tables = best.teno_tables;
         HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, tune_.teno_occ, 0, stream_));
         if (gate_ && !gate_per_stage_) {  // passes that reuse the decisions just recorded
             bt = 1e30f;

@@ -235,6 +235,9 @@ private:
     double *ta_ = nullptr, *A_ = nullptr, *fbasis_ = nullptr;
     bool compact_ = false;
     double* fbc_ = nullptr;  // compact tables: basis values per reference edge
+    // derived face basis: [N][6] node coordinates of each cell (null where
+    // the plain TENO pass cannot derive it)
+    double* cvert_ = nullptr;
     double *Un_ = nullptr, *Ua_ = nullptr, *Ub_ = nullptr, *W_ = nullptr, *prims_ = nullptr, *h_ = nullptr;
     double *fs_ = nullptr, *rhs_ = nullptr, *lam_cell_ = nullptr, *dt_dev_ = nullptr, *dt_hist_ = nullptr;
     unsigned long long* lam_blk_ = nullptr;  // per-block spectral maxima (room for 64-cell blocks)

@@ -1,7 +1,8 @@
 """Launch-shape autotuner on the GPU: every shape it can pick (TENO occupancy
-target, flux and spectral block sizes) gives bit-identical results, with and
-without the learned gate, for single solvers and ensembles; autotuning leaves
-the state and the gated-cell count untouched."""
+target, flux and spectral block sizes, stored or derived TENO face basis) gives
+bit-identical results, with and without the learned gate, for single
+solvers and ensembles; autotuning leaves the state and the gated-cell count
+untouched."""
 import numpy as np
 import pytest
 
@@ -12,7 +13,8 @@ pytestmark = pytest.mark.gpu
 EXTRAP = [(z, "extrapolation", {}) for z in ("left", "right", "top", "bottom")]
 RIEMANN2D = np.array([[1.5, 0.0, 0.0, 1.5], [0.5323, 1.206, 0.0, 0.3],
                       [0.138, 1.206, 1.206, 0.029], [0.5323, 0.0, 1.206, 0.3]])
-SHAPES = [(0, 256, 256, 0), (5, 64, 128, 6), (6, 128, 64, 5)]
+SHAPES = [(0, 256, 256, 0, 0), (5, 64, 128, 6, 0), (6, 128, 64, 5, 0), (0, 256, 256, 0, 1), (4, 128, 64, 5, 1),
+          (5, 64, 256, 6, 1)]
 
 
 def _num(mlr):
@@ -58,7 +60,7 @@ def test_p3_occupancy_targets_give_identical_results(mlr):
     for occ in (0, 4, 5):
         s = mlr.Solver(gen=gen, num=num)
         s.init_quadrant(0.5, 0.5, RIEMANN2D)
-        s.set_tune(occ, 128, 128, 0)
+        s.set_tune(occ, 128, 128, 0, 0)
         s.run_steps(5)
         out.append(s.state()[0])
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
@@ -74,9 +76,10 @@ def test_autotune_leaves_state_and_counts(mlr):
     s.set_tune(0, 256, 256, 0)
     s.run_steps(3)
     U0, g0 = s.state()[0], s.gated()
-    occ, fb, sb, occ2 = s.autotune()
-    assert occ in (0, 5, 6) and occ2 in (0, 5, 6) and fb in (64, 128, 256) and sb in (64, 128, 256)
-    assert s.tune() == (occ, fb, sb, occ2)
+    occ, fb, sb, occ2, tables = s.autotune()
+    assert occ in (0, 4, 5, 6) and occ2 in (0, 5, 6) and fb in (64, 128, 256) and sb in (64, 128, 256)
+    assert tables in (0, 1)
+    assert s.tune() == (occ, fb, sb, occ2, tables)
     assert np.array_equal(s.state()[0], U0)
     assert s.gated() == g0
     # and the steps after it equal the steps of an untuned twin

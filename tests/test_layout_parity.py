@@ -99,7 +99,8 @@ def test_slot_edge_ids(mlr, oracle):
     """Slot 0's info word holds, for each face slot l in bits 24 + 2 l, the
     edge of the reference triangle the face maps to in the target frame:
     0 for the cell's nodes 0-1, 1 for 1-2, 2 for 2-0 (the compact tables'
-    face basis rows)."""
+    face basis rows), and in bit 30, 31 or 19 whether the face's node 0 is
+    the edge's higher cell node (the derived tables' face direction)."""
     L = _layout(mlr, "cartesian_tri", 9, 7, 0.2, 5, recon="TENO", order=2)
     m = oracle.Mesh("cartesian_tri", 9, 7, 1.0, 1.0, 0.2, 5)
     edge = {frozenset((0, 1)): 0, frozenset((1, 2)): 1, frozenset((0, 2)): 2}
@@ -111,3 +112,5 @@ def test_slot_edge_ids(mlr, oracle):
             a, b = m.nodes_of_face[2 * f], m.nodes_of_face[2 * f + 1]
             want = edge[frozenset((nodes.index(a), nodes.index(b)))]
             assert (info0 >> (24 + 2 * l)) & 3 == want, (c, l)
+            flip = 1 if nodes.index(a) > nodes.index(b) else 0
+            assert (info0 >> (30, 31, 19)[l]) & 1 == flip, (c, l)
