@@ -418,8 +418,8 @@ class Solver:
         self._check(lib().mlr_autotune(self.h, _p(t)))
         return tuple(int(x) for x in t)
 
-    def set_tune(self, teno_occ=0, flux_block=256, spec_block=256, teno_occ_reuse=0, teno_tables=0):
-        t = np.array([teno_occ, flux_block, spec_block, teno_occ_reuse, teno_tables], np.int32)
+    def set_tune(self, teno_occ=0, flux_block=256, spec_block=256, teno_occ_reuse=0, face_basis=0):
+        t = np.array([teno_occ, flux_block, spec_block, teno_occ_reuse, face_basis], np.int32)
         self._check(lib().mlr_set_tune(self.h, _p(t)))
 
     def tune(self):

@@ -336,7 +336,7 @@ void tune_out(const mlr::dev::Tune& t, int32_t* out) {
     out[1] = (int32_t)t.flux_block;
     out[2] = (int32_t)t.spec_block;
     out[3] = t.teno_occ_reuse;
-    out[4] = t.teno_tables;
+    out[4] = t.teno_face_basis;
 }
 mlr::dev::Tune tune_in(const int32_t* in) {
     mlr::dev::Tune t;
@@ -344,7 +344,7 @@ mlr::dev::Tune tune_in(const int32_t* in) {
     t.flux_block = (uint32_t)in[1];
     t.spec_block = (uint32_t)in[2];
     t.teno_occ_reuse = in[3];
-    t.teno_tables = in[4];
+    t.teno_face_basis = in[4];
     return t;
 }
 }  // namespace

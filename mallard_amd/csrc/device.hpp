@@ -190,7 +190,7 @@ struct Tune {
     uint32_t flux_block = 256;  // cells per flux / stage-update block: 64, 128 or 256
     uint32_t spec_block = 256;  // cells per spectral-radius block: 64, 128 or 256
     int32_t teno_occ_reuse = 0; // the same target for gated passes that reuse recorded decisions
-    int32_t teno_tables = 0;    // plain TENO pass: 0 reads the face-basis table, 1 derives
+    int32_t teno_face_basis = 0; // plain TENO pass: 0 reads the face-basis table, 1 derives
                                 // it from the cell's node coordinates (Teno::derived)
 };
 

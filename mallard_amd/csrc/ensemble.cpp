@@ -121,7 +121,7 @@ void Ensemble::set_tune(const dev::Tune& t) {
     if (persistent_ && t.flux_block != 256) fail("the persistent step runs flux and spectral blocks of 256 cells");
     auto occ_ok = [](int o) { return o == 0 || o == 5 || o == 6; };
     if (!occ_ok(t.teno_occ) || !occ_ok(t.teno_occ_reuse)) fail("TENO occupancy target must be 0, 5 or 6");
-    if (t.teno_tables != 0) fail("ensembles read the stored TENO tables (table source 0)");
+    if (t.teno_face_basis != 0) fail("ensembles read the stored TENO face basis (face-basis source 0)");
     tune_ = t;
     tune_.spec_block = t.flux_block;  // flux and spectral blocks share the case table's offsets
     tuned_ = true;

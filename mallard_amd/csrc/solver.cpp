@@ -514,7 +514,7 @@ dev::Teno Solver::teno() const {
     t.fbc = fbc_;
     t.gate = gate_ ? 1 : 0;
     // the derived face basis serves the plain pass (no gate, exact tables kept)
-    t.derived = (tune_.teno_tables == 1 && cvert_ && !compact_ && !gate_) ? 1 : 0;
+    t.derived = (tune_.teno_face_basis == 1 && cvert_ && !compact_ && !gate_) ? 1 : 0;
     t.cvert = cvert_;
     for (int k = 0; k < L_.K && k < 10; ++k) {
         t.ipt[k] = L_.ipt[k];
@@ -873,8 +873,9 @@ void Solver::set_tune(const dev::Tune& t) {
     if (!bs_ok(t.flux_block) || !bs_ok(t.spec_block)) fail("block sizes must be 64, 128 or 256 cells");
     auto occ_ok = [](int o) { return o == 0 || o == 4 || o == 5 || o == 6; };
     if (!occ_ok(t.teno_occ) || !occ_ok(t.teno_occ_reuse)) fail("TENO occupancy target must be 0, 4, 5 or 6");
-    if (t.teno_tables != 0 && t.teno_tables != 1) fail("TENO face-basis source must be 0 (stored) or 1 (derived)");
-    if (t.teno_tables == 1) {
+    if (t.teno_face_basis != 0 && t.teno_face_basis != 1)
+        fail("TENO face-basis source must be 0 (stored) or 1 (derived)");
+    if (t.teno_face_basis == 1) {
         if (!cvert_) fail("derived TENO face basis: p = 2, Legendre basis, default stencil size, tiles of <= 256 cells");
         if (t.teno_occ == 6) fail("derived TENO face basis: occupancy target 0, 4 or 5");
     }
@@ -946,20 +947,20 @@ void Solver::autotune() {
         const bool occ_p3 = L_.K == 10 && L_.M == 20 && fq == 6 && !gate_;
         const bool derivable = cvert_ != nullptr && !gate_ && !compact_;
         bt = 1e30f;
-        for (int tables : {0, 1}) {
-            if (tables == 1 && !derivable) continue;
+        for (int fb : {0, 1}) {
+            if (fb == 1 && !derivable) continue;
             for (int occ : {0, 4, 5, 6}) {
-                if (tables == 0 && occ != 0 && !(occ_tunable && occ >= 5) && !(occ_p3 && occ <= 5)) continue;
-                if (tables == 1 && occ == 6) continue;
-                tune_.teno_tables = tables;
+                if (fb == 0 && occ != 0 && !(occ_tunable && occ >= 5) && !(occ_p3 && occ <= 5)) continue;
+                if (fb == 1 && occ == 6) continue;
+                tune_.teno_face_basis = fb;
                 const float t = timed([&] {
                     HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, occ, 0, stream_));
                 });
-                if (t < bt) { bt = t; best.teno_occ = occ; best.teno_tables = tables; }
+                if (t < bt) { bt = t; best.teno_occ = occ; best.teno_face_basis = fb; }
             }
         }
         tune_.teno_occ = best.teno_occ;
-        tune_.teno_tables = best.teno_tables;
+        tune_.teno_face_basis = best.teno_face_basis;
         HIP_OK(dev::teno_faces(geom(), teno(), Un_, fs_, dec_, ph_, tune_.teno_occ, 0, stream_));
         if (gate_ && !gate_per_stage_) {  // passes that reuse the decisions just recorded
             bt = 1e30f;
