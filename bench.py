@@ -285,7 +285,7 @@ def ensemble_run(args, mlr):
 
     init()
     tune = ens.autotune()
-    log(f"[bench] autotuned (TENO occupancy, block, block): {tune}")
+    log(f"[bench] autotuned (TENO occupancy, block, block, reuse occupancy, face basis): {tune}")
     if args.warmup > 0:
         ens.run_steps(args.warmup)
     init()

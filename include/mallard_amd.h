@@ -343,9 +343,9 @@ int mlr_ensemble_get_volumes(mlr_ensemble* ens, int k, double* vol);
 int mlr_ensemble_run_steps(mlr_ensemble* ens, int64_t n);
 /* kernel-by-kernel with events (the stats layout of mlr_run_steps_profiled) */
 int mlr_ensemble_run_steps_profiled(mlr_ensemble* ens, int64_t n, double* stats);
-/* launch shapes of the batched kernels, as mlr_autotune / mlr_set_tune (the
- * flux and spectral block sizes are one value here: tune[2] = tune[1]; the
- * table source tune[4] is 0), tune[5] */
+/* launch shapes of the batched kernels and the plain pass's face-basis
+ * source, as mlr_autotune / mlr_set_tune (the flux and spectral block sizes
+ * are one value here: tune[2] = tune[1]), tune[5] */
 int mlr_ensemble_autotune(mlr_ensemble* ens, int32_t* tune);
 int mlr_ensemble_set_tune(mlr_ensemble* ens, const int32_t* tune);
 int mlr_ensemble_set_indicator(mlr_ensemble* ens, const float* params, double threshold);

@@ -633,12 +633,14 @@ class Ensemble:
             self._check(lib().mlr_ensemble_indicator_split(self.h, int(split)))
 
     def autotune(self):
+        """(TENO occupancy target, block, block, reuse occupancy target,
+        face-basis source)."""
         t = np.zeros(5, np.int32)
         self._check(lib().mlr_ensemble_autotune(self.h, _p(t)))
-        return tuple(int(x) for x in t[:4])
+        return tuple(int(x) for x in t)
 
-    def set_tune(self, teno_occ=0, block=256, teno_occ_reuse=0):
-        t = np.array([teno_occ, block, block, teno_occ_reuse, 0], np.int32)
+    def set_tune(self, teno_occ=0, block=256, teno_occ_reuse=0, face_basis=0):
+        t = np.array([teno_occ, block, block, teno_occ_reuse, face_basis], np.int32)
         self._check(lib().mlr_ensemble_set_tune(self.h, _p(t)))
 
     def gated(self, reset=False):

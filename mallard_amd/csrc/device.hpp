@@ -223,8 +223,9 @@ hipError_t viscous_grad(const Geom& g, const double* U, const Physics& ph, doubl
 // width (small = 8-bit) and Riemann solver.
 // gated: some case runs the learned gate (the block stages the network in LDS);
 // split: every case's gate runs split (Teno::blk_mixed set), as teno_faces
+// derived: plain pass with every case's face basis derived (Teno::derived)
 hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
-                      bool small, bool gated, int occ, int gate_reuse, bool split, const Physics& ph,
+                      bool small, bool gated, int occ, int gate_reuse, bool split, bool derived, const Physics& ph,
                       hipStream_t stream);
 hipError_t flux_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, const BatchStage& st, int riemann,
                       const Physics& ph, uint32_t bs, hipStream_t stream);

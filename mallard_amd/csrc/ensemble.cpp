@@ -48,6 +48,8 @@ Ensemble::Ensemble(const Config& base, const std::vector<MeshSpec>& meshes) : cf
                                               (int)(c->L_.S * c->L_.Q) != FQ_ || (c->loc8_ != nullptr) != small_)))
             fail("ensemble cases must share the reconstruction layout (order, stencil size, tile width)");
     }
+    derivable_ = teno_;
+    for (const auto& c : cases_) derivable_ = derivable_ && c->cvert_ != nullptr;
     hip_ok(hipStreamCreate(&stream_), "hipStreamCreate");
     hip_ok(hipMalloc(&d_cases_, cases_.size() * sizeof(dev::CaseBufs)), "hipMalloc");
     upload_table();
@@ -120,8 +122,16 @@ void Ensemble::set_tune(const dev::Tune& t) {
     if (!bs_ok(t.flux_block)) fail("block sizes must be 64, 128 or 256 cells");
     if (persistent_ && t.flux_block != 256) fail("the persistent step runs flux and spectral blocks of 256 cells");
     auto occ_ok = [](int o) { return o == 0 || o == 5 || o == 6; };
-    if (!occ_ok(t.teno_occ) || !occ_ok(t.teno_occ_reuse)) fail("TENO occupancy target must be 0, 5 or 6");
-    if (t.teno_face_basis != 0) fail("ensembles read the stored TENO face basis (face-basis source 0)");
+    if (t.teno_face_basis != 0 && t.teno_face_basis != 1)
+        fail("TENO face-basis source must be 0 (stored) or 1 (derived)");
+    if (t.teno_face_basis == 1) {
+        if (!derivable_)
+            fail("derived TENO face basis: p = 2, Legendre basis, default stencil size, tiles of <= 256 cells");
+        if (!(t.teno_occ == 0 || t.teno_occ == 4 || t.teno_occ == 5) || !occ_ok(t.teno_occ_reuse))
+            fail("derived TENO face basis: occupancy target 0, 4 or 5");
+    } else if (!occ_ok(t.teno_occ) || !occ_ok(t.teno_occ_reuse)) {
+        fail("TENO occupancy target must be 0, 5 or 6");
+    }
     tune_ = t;
     tune_.spec_block = t.flux_block;  // flux and spectral blocks share the case table's offsets
     tuned_ = true;
@@ -176,14 +186,19 @@ void Ensemble::autotune() {
             }
         }
         const bool occ_tunable = K_ == 6 && M_ == 12 && FQ_ == 3 && small_ && !(gated_ && split_);
+        // the plain pass also tries the derived face basis at 0, 4 and 5
         bt = 1e30f;
-        for (int occ : {0, 5, 6}) {
-            if (occ != 0 && !occ_tunable) continue;
-            const float t = timed([&] {
-                hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, dev::kBufUn, K_, M_, FQ_, small_, gated_, occ, 0,
-                                       split_, ph_, stream_), "teno_batch");
-            });
-            if (t < bt) { bt = t; best.teno_occ = occ; }
+        for (int fb : {0, 1}) {
+            if (fb == 1 && !(derivable_ && !gated_)) continue;
+            for (int occ : {0, 4, 5, 6}) {
+                if (fb == 0 && (occ == 4 || (occ != 0 && !occ_tunable))) continue;
+                if (fb == 1 && occ == 6) continue;
+                const float t = timed([&] {
+                    hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, dev::kBufUn, K_, M_, FQ_, small_, gated_, occ,
+                                           0, split_, fb == 1, ph_, stream_), "teno_batch");
+                });
+                if (t < bt) { bt = t; best.teno_occ = occ; best.teno_face_basis = fb; }
+            }
         }
         if (gated_ && !gate_per_stage_) {  // passes that reuse the decisions just recorded
             bt = 1e30f;
@@ -191,7 +206,7 @@ void Ensemble::autotune() {
                 if (occ != 0 && !occ_tunable) continue;
                 const float t = timed([&] {
                     hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, dev::kBufUn, K_, M_, FQ_, small_, gated_, occ,
-                                           1, split_, ph_, stream_), "teno_batch");
+                                           1, split_, false, ph_, stream_), "teno_batch");
                 });
                 if (t < bt) { bt = t; best.teno_occ_reuse = occ; }
             }
@@ -239,7 +254,8 @@ void Ensemble::launch_step(bool prof) {
             const bool first = op == dev::kSSP1 || op == dev::kRK4_1;
             const int reuse = (gated_ && !gate_per_stage_ && !first) ? 1 : 0;
             hip_ok(dev::teno_batch(d_cases_, n, teno_blocks_, in, K_, M_, FQ_, small_, gated_,
-                                   reuse ? tune_.teno_occ_reuse : tune_.teno_occ, reuse, split_, ph_, stream_),
+                                   reuse ? tune_.teno_occ_reuse : tune_.teno_occ, reuse, split_,
+                                   tune_.teno_face_basis == 1 && !gated_, ph_, stream_),
                    "teno_batch");
             mark(kStatTenoMs);
         }

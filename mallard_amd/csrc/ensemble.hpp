@@ -75,6 +75,7 @@ private:
     uint32_t teno_blocks_ = 0, flux_blocks_ = 0;
     int K_ = 0, M_ = 0, FQ_ = 0;
     bool small_ = true, teno_ = false, gated_ = false, gate_per_stage_ = false, split_ = false;
+    bool derivable_ = false;  // every case holds its cells' node coordinates (derived face basis)
     dev::Physics ph_{};
     dev::Tune tune_{};
     bool tuned_ = false;

@@ -568,7 +568,7 @@ __device__ inline double* case_buf_w(const CaseBufs& c, int role) {
 // The TENO pass of every case of an ensemble in one launch: the grid covers
 // all cases' 64-cell blocks (XCD-aware order over the whole grid), each
 // block finds its case in the offset table and runs that case's block.
-template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, int OCC>
+template <int KP, int MP, int FQ, typename IdxT, int TILE, int GM, int OCC, bool DV = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC == 0 ? 1 : OCC, OCC == 0 ? 8 : OCC)))
 void teno_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, int in_role,
                                                          uint32_t total, Physics ph, uint32_t blocks_per_xcd) {
@@ -576,9 +576,9 @@ void teno_batch_kernel(const CaseBufs* __restrict__ cases, int n_cases, int in_r
     const uint32_t gb = (blockIdx.x & 7u) * blocks_per_xcd + (blockIdx.x >> 3);
     if (gb >= total) return;
     const CaseBufs& cs = cases[case_of_block(cases, n_cases, gb, false)];
-    teno_block<KP, MP, FQ, IdxT, TILE, GM>(cs.g, cs.t, case_buf(cs, in_role), cs.fs, cs.dec, ph,
-                                           gb - cs.blk_teno0, reinterpret_cast<double2*>(teno_smem),
-                                           teno_smem + TILE * 32);
+    teno_block<KP, MP, FQ, IdxT, TILE, GM, false, DV>(cs.g, cs.t, case_buf(cs, in_role), cs.fs, cs.dec, ph,
+                                                      gb - cs.blk_teno0, reinterpret_cast<double2*>(teno_smem),
+                                                      teno_smem + TILE * 32);
 }
 
 // ------------------------------------------------------------ viscous terms
@@ -1502,14 +1502,31 @@ hipError_t finalize_dt(const unsigned long long* lam_blk, uint32_t n_blk, double
 }
 
 hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks, int in_role, int K, int M, int FQ,
-                      bool small, bool gated, int occ, int gate_reuse, bool split, const Physics& ph,
+                      bool small, bool gated, int occ, int gate_reuse, bool split, bool derived, const Physics& ph,
                       hipStream_t stream) {
     const int Kp = K - 1, Mp = M - 1;
     const size_t net_lds = (gated && !gate_reuse) ? ind::kNetInferBytes : 0;
     if (n_cases < 1 || (FQ != 3 && FQ != 6) || Mp < 1) return hipErrorInvalidValue;
-    if (occ != 0 && occ != 5 && occ != 6) return hipErrorInvalidValue;
     const uint32_t per_xcd = (total_blocks + 7) / 8;
     const dim3 grid(8 * per_xcd), block(256);
+    if (derived) {
+        // derived face basis: plain p = 2 with the default stencil size and
+        // 8-bit tile positions in every case (each case's Teno::cvert set)
+        if (!(FQ == 3 && Kp == 5 && Mp == 11 && small) || gated) return hipErrorInvalidValue;
+        if (occ == 4)
+            hipLaunchKernelGGL((teno_batch_kernel<5, 11, 3, uint8_t, 256, 0, 4, true>), grid, block, 256 * 32,
+                               stream, cases, n_cases, in_role, total_blocks, ph, per_xcd);
+        else if (occ == 5)
+            hipLaunchKernelGGL((teno_batch_kernel<5, 11, 3, uint8_t, 256, 0, 5, true>), grid, block, 256 * 32,
+                               stream, cases, n_cases, in_role, total_blocks, ph, per_xcd);
+        else if (occ == 0)
+            hipLaunchKernelGGL((teno_batch_kernel<5, 11, 3, uint8_t, 256, 0, 0, true>), grid, block, 256 * 32,
+                               stream, cases, n_cases, in_role, total_blocks, ph, per_xcd);
+        else
+            return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
+    if (occ != 0 && occ != 5 && occ != 6) return hipErrorInvalidValue;
 #define MLR_TB_S(KP, MP, FQV, G, O)                                                                             \
     hipLaunchKernelGGL((teno_batch_kernel<KP, MP, FQV, uint8_t, 256, G, O>), grid, block, 256 * 32 + net_lds,   \
                        stream, cases, n_cases, in_role, total_blocks, ph, per_xcd)

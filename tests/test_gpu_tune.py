@@ -117,5 +117,32 @@ def test_ensemble_shapes_give_identical_results(mlr):
     ens = mlr.Ensemble(gens, _num(mlr))
     for k in range(len(gens)):
         ens.init_quadrant(k, 0.5, 0.5, states[k])
-    occ, bs, bs2, occ2 = ens.autotune()
-    assert occ in (0, 5, 6) and occ2 in (0, 5, 6) and bs in (64, 128, 256) and bs2 == bs
+    occ, bs, bs2, occ2, fb = ens.autotune()
+    assert occ in (0, 4, 5, 6) and occ2 in (0, 5, 6) and bs in (64, 128, 256) and bs2 == bs and fb in (0, 1)
+
+
+def test_ensemble_derived_face_basis_gives_identical_results(mlr):
+    """The plain batched TENO pass with each case's face basis derived from
+    its node coordinates, at three occupancy targets, equals the stored-basis
+    pass bit for bit, and each case equals its standalone solver."""
+    sizes = [(20, 20), (33, 27), (48, 48)]
+    gens = [mlr.mesh_gen("cartesian_tri", nx, ny, 1.0, 1.0, 0.2, 70 + nx) for nx, ny in sizes]
+    res = []
+    for occ, fb in [(0, 0), (0, 1), (4, 1), (5, 1)]:
+        ens = mlr.Ensemble(gens, _num(mlr))
+        for k in range(len(gens)):
+            ens.init_quadrant(k, 0.5, 0.5, RIEMANN2D)
+        ens.set_tune(occ, 128, 0, fb)
+        ens.run_steps(8)
+        res.append([ens.state(k)[0] for k in range(len(gens))])
+        ens.close()
+    for r in res[1:]:
+        for k in range(len(gens)):
+            assert np.array_equal(res[0][k], r[k]), k
+    for k, gen in enumerate(gens):
+        s = mlr.Solver(gen=gen, num=_num(mlr))
+        s.init_quadrant(0.5, 0.5, RIEMANN2D)
+        s.set_tune(0, 128, 128, 0, 1)
+        s.run_steps(8)
+        assert np.array_equal(s.state()[0], res[0][k]), k
+        s.close()
