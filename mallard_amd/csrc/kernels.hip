@@ -264,6 +264,52 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
     const uint32_t info0 = g.info[cc];
     const D4 uc = ld4(U, cc);
     const uint32_t cl = c & 63u;
+    // Quad exchanges of the plain pass and of the gated passes that keep no
+    // network in LDS (reuse, ungated cells), with 8-bit tiles, go through LDS: 4
+    // planes of 258 doubles per block (the 2-double pad puts the 4 lanes of
+    // a quad on distinct banks when they read one plane), 8 LDS instructions
+    // per 4 x 4 transpose instead of the DPP form's selects and moves. A
+    // wave reads only its own lanes' entries, and a wave's LDS accesses run
+    // in order, so wave-scope fences (compiler ordering) suffice.
+    constexpr bool kLdsX = (GM == 0 || GM == 4 || GM == 5) && TILE == 256;
+    constexpr int kXs = 258;
+    __shared__ double xbuf[kLdsX ? 4 * kXs : 1];
+    auto wave_sync = []() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    // out[j] = x[q] of quad lane j
+    auto exchange = [&](const double* x, double* out) {
+        if constexpr (kLdsX) {
+            const uint32_t tid = threadIdx.x, base = tid & ~3u;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xbuf[i * kXs + tid] = x[i];
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < 4; ++j) out[j] = xbuf[q * kXs + base + j];
+            wave_sync();
+        } else {
+            quad_transpose(x, out, q);
+        }
+    };
+    // out[j] = f[fq] of quad lane j, fq < FQ
+    auto gather = [&](const double* f, double (*out)[4]) {
+        if constexpr (kLdsX && FQ <= 4) {
+            const uint32_t tid = threadIdx.x, base = tid & ~3u;
+#pragma unroll
+            for (int fq = 0; fq < FQ; ++fq) xbuf[fq * kXs + tid] = f[fq];
+            wave_sync();
+#pragma unroll
+            for (int fq = 0; fq < FQ; ++fq)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) out[fq][j] = xbuf[fq * kXs + base + j];
+            wave_sync();
+        } else {
+#pragma unroll
+            for (int fq = 0; fq < FQ; ++fq) quad_gather(f[fq], out[fq], q);
+        }
+    };
     // the member update a += A+_i (ta_i (u_i - u_c)) of member i of this
     // lane's stencil q; a wave's 64 lanes read 64 consecutive entries
     auto member = [&](int i, double (&a)[KP][4]) {
@@ -322,11 +368,11 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
             }
         }
         double y[4];
-        quad_transpose(acc[0], y, q);
+        exchange(acc[0], y);
 #pragma unroll
         for (int fq = 0; fq < (FQ < 4 ? FQ : 4); ++fq) f[fq] = ucv + y[fq];
         if constexpr (FQ > 4) {
-            quad_transpose(acc[1], y, q);
+            exchange(acc[1], y);
 #pragma unroll
             for (int fq = 4; fq < FQ; ++fq) f[fq] = ucv + y[fq - 4];
         }
@@ -368,11 +414,11 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         // ---- exchange: lane q = variable
         double ws[4];
         double at[4][KP];
-        quad_transpose(w, ws, q);
+        exchange(w, ws);
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
             double x[4] = {a[k][0], a[k][1], a[k][2], a[k][3]}, y[4];
-            quad_transpose(x, y, q);
+            exchange(x, y);
 #pragma unroll
             for (int st = 0; st < 4; ++st) at[st][k] = y[st];
         }
@@ -494,8 +540,7 @@ __device__ __forceinline__ void teno_block(const Geom& g, const Teno& t, const d
         // positivity scaling of the cell's face states (Zhang & Shu 2010),
         // computed redundantly by the 4 lanes, in the oracle's order
         double uf[FQ][4];
-#pragma unroll
-        for (int fq = 0; fq < FQ; ++fq) quad_gather(f[fq], uf[fq], q);
+        gather(f, uf);
         const double Uc[4] = {uc.v[0], uc.v[1], uc.v[2], uc.v[3]};
         const double pc = pressure_raw(ph.gamma, Uc);
         const double eps = fmin(1.0e-13, fmin(Uc[0], pc));
