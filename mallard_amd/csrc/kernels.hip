@@ -1428,10 +1428,12 @@ hipError_t teno_faces(const Geom& g, const Teno& t, const double* U, double* fs,
             if (Mp == 11 && small && t.gate && t.blk_mixed) {
                 // split gated pass: the gated cells (modes 3 / 4, no
                 // full-TENO code), then the ungated ones of the marked
-                // blocks (mode 5); both at the compiler's occupancy (the
-                // occupancy target does not apply)
+                // blocks (mode 5). The evaluate pass targets 6 waves per
+                // SIMD (77 VGPRs, no spill; the compiler's own choice, 75
+                // VGPRs + 8 AGPRs, gives 5); the others run at the
+                // compiler's occupancy (the tune's target does not apply)
                 if (gate_reuse) MLR_TENO_S(5, 11, 3, 4, 0);
-                else MLR_TENO_S(5, 11, 3, 3, 0);
+                else MLR_TENO_S(5, 11, 3, 3, 6);
                 if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
                 hipLaunchKernelGGL((teno_kernel<5, 11, 3, uint8_t, 256, 5, 0>), grid, block, 256 * 32, stream, g, t, U,
                                    fs, dec, ph, per_xcd);
@@ -1593,7 +1595,7 @@ hipError_t teno_batch(const CaseBufs* cases, int n_cases, uint32_t total_blocks,
     if (FQ == 3 && Kp == 5 && Mp == 11 && small && gated && split) {
         // split gated pass (teno_faces' modes 3 / 4, then 5)
         if (gate_reuse) MLR_TB_S(5, 11, 3, 4, 0);
-        else MLR_TB_S(5, 11, 3, 3, 0);
+        else MLR_TB_S(5, 11, 3, 3, 6);  // the evaluate pass at 6 waves per SIMD, as teno_faces
         if (const hipError_t e = hipGetLastError(); e != hipSuccess) return e;
         hipLaunchKernelGGL((teno_batch_kernel<5, 11, 3, uint8_t, 256, 5, 0>), grid, block, 256 * 32, stream, cases,
                            n_cases, in_role, total_blocks, ph, per_xcd);
