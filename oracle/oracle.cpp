@@ -25,10 +25,10 @@
  * unjittered meshes, the ENO branch on a straight jump (both weightings,
  * raw weights and the +ipt/ta0 offset included), the directional stencils
  * against a brute-force Type-4 search, wall / fixed-state / p_out residuals,
- * and Toro's HLL flux; each pin fails on planted mistakes
- * (tests/test_oracle_mutations.py). The Roe flux (no reference counterpart)
- * is pinned only by textbook properties: parity unpinned against the
- * reference. Also parity unpinned beyond the invariants above: the order of
+ * Toro's HLL flux, and the Roe flux (no reference counterpart) at exact
+ * Rankine-Hugoniot shocks; each pin fails on planted mistakes
+ * (tests/test_oracle_mutations.py). Also parity unpinned beyond the
+ * invariants above: the order of
  * equal-distance ring members (libstdc++ std::sort ties,
  * face_reconstruction.cpp:253-257).
  */
@@ -766,7 +766,7 @@ void flux_hllc(double* F, const double* n, double rl, const double* ul, double p
     }
 }
 // Roe flux with Harten's entropy fix (textbook form, Toro ch. 11). No
-// reference counterpart: parity unpinned.
+// reference counterpart; pinned by its exact resolution of isolated shocks.
 void flux_roe(double* F, const double* n, double rl, const double* ul, double pl, double gl,
               double hl, double rr, const double* ur, double pr, double gr, double hr) {
     LR s;

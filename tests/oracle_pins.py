@@ -604,6 +604,19 @@ def check_positivity_limiter(orc):
             np.testing.assert_allclose(got, exp, rtol=0, atol=1e-13 * max(1.0, abs(mom)), err_msg=str((b, got, exp)))
 
 
+def check_roe_shocks(orc):
+    """Roe's flux (no reference counterpart; Roe 1981) at lone normal shocks
+    built from the Rankine-Hugoniot relations: the jump is an eigenvector of
+    the Roe matrix with the shock speed as eigenvalue, so the flux is the
+    upwind side's physical flux. Shocks of Mach 1.5, 2 and 4, both
+    directions, at rest and moving, three normals, tangential velocity."""
+    from shocks import cases, enthalpy
+    for n, L, R, want in cases(GAMMA):
+        F = orc.riemann_flux("Roe", list(n), L[0], list(L[1]), L[2], GAMMA, enthalpy(GAMMA, L[0], L[2]), R[0],
+                             list(R[1]), R[2], GAMMA, enthalpy(GAMMA, R[0], R[2]))
+        np.testing.assert_allclose(F, want, rtol=1e-12, atol=1e-12 * np.abs(want).max(), err_msg=str((n, L, R)))
+
+
 CHECKS = {
     "dt": check_dt_closed_form,
     "eno": check_eno_branch,
@@ -611,4 +624,5 @@ CHECKS = {
     "boundary": check_boundary_fluxes,
     "hll": check_hll_toro,
     "positivity": check_positivity_limiter,
+    "roe": check_roe_shocks,
 }

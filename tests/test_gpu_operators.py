@@ -312,3 +312,19 @@ def test_integrator_with_device_residual_equals_solver_step(mlr, integrator):
     got[perm] = sol[0].get()
     dev.take_step()
     assert np.array_equal(got, dev.state()[0])
+
+
+def test_roe_flux_resolves_isolated_shocks_exactly(mlr):
+    """The HIP Roe flux at lone Rankine-Hugoniot shocks (both directions, at
+    rest and moving, three normals) equals the upwind side's physical flux."""
+    from shocks import cases, enthalpy
+    g = 1.4
+    cs = cases(g)
+    n = np.array([c[0] for c in cs])
+    side = lambda S: [S[0], S[1][0], S[1][1], S[2], g, enthalpy(g, S[0], S[2])]
+    L = np.array([side(c[1]) for c in cs])
+    R = np.array([side(c[2]) for c in cs])
+    want = np.array([c[3] for c in cs])
+    F = mlr.riemann_flux("Roe", n, L, R)
+    for i in range(len(cs)):
+        assert np.allclose(F[i], want[i], rtol=1e-12, atol=1e-12 * np.abs(want[i]).max()), i

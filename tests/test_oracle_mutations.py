@@ -69,6 +69,15 @@ MUTANTS = [
      "if (false || !std::isfinite"),
     ("hll", "linearised estimate with the velocity jump reversed",
      "ps = 0.5 * (pl + pr) + 0.5 * (ul - ur) * ravg * aavg;", "ps = 0.5 * (pl + pr) + 0.5 * (ur - ul) * ravg * aavg;"),
+    ("roe", "arithmetic mean instead of the Roe average of u",
+     "const double u = (sl * ul[0] + sr * ur[0]) * isum;", "const double u = 0.5 * (ul[0] + ur[0]);"),
+    ("roe", "arithmetic mean instead of the Roe average of H",
+     "const double H = (sl * Hl + sr * Hr) * isum;", "const double H = 0.5 * (Hl + Hr);"),
+    ("roe", "first wave strength with the wrong sign", "const double al1 = (dp - rho * a * dun) / (2.0 * a2);",
+     "const double al1 = (dp + rho * a * dun) / (2.0 * a2);"),
+    ("roe", "first eigenvector's energy with the wrong sign",
+     "const double r1[4] = {1.0, u - a * n[0], v - a * n[1], H - a * un};",
+     "const double r1[4] = {1.0, u - a * n[0], v - a * n[1], H + a * un};"),
 ]
 
 
