@@ -154,4 +154,3 @@ def test_roe_textbook_properties(oracle):
                              _h(g, 0.5, 0.4))
     assert abs(F1[0] - F0[0]) < 1e-13 and abs(F1[3] - F0[3]) < 1e-13
     assert np.allclose([F1[1], F1[2]], rot([F0[1], F0[2]]), atol=1e-13)
-
